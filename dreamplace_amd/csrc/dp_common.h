@@ -1,0 +1,148 @@
+// Internal helpers shared by the HIP translation units of libdreamplace_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <cstdarg>
+#include <type_traits>
+
+#include "../../include/dreamplace_amd.h"
+
+namespace dp {
+
+// Per-thread message of the last failure (dp_last_error).
+void set_error(const char* fmt, ...);
+
+// Status helpers.  Every entry point returns through these.
+#define DP_REQUIRE(cond, code, ...)        \
+  do {                                     \
+    if (!(cond)) {                         \
+      ::dp::set_error(__VA_ARGS__);        \
+      return (code);                       \
+    }                                      \
+  } while (0)
+
+#define DP_ARG(cond, ...) DP_REQUIRE(cond, DP_ERR_INVALID_ARGUMENT, __VA_ARGS__)
+
+#define DP_LAUNCHED(name)                                                          \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) {                                                        \
+      ::dp::set_error("%s: launch failed: %s", name, hipGetErrorString(e_));       \
+      return (int)e_;                                                              \
+    }                                                                              \
+  } while (0)
+
+#define DP_CHECK_RC(expr)        \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != DP_OK) return rc_; \
+  } while (0)
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Launch-grid cap for grid-stride kernels: enough workgroups to fill 256 CUs
+// several times over.
+constexpr int kMaxGrid = 65536;
+inline int grid_for(int64_t n, int block) {
+  int64_t g = ceil_div(n, block);
+  if (g < 1) g = 1;
+  return (int)(g > kMaxGrid ? kMaxGrid : g);
+}
+
+// Workspace carving: 256-byte aligned sub-buffers.
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// dtype dispatch: DP_FLOAT32 -> float, DP_FLOAT64 -> double.
+#define DP_DISPATCH(dtype, NAME, ...)                                           \
+  [&]() -> int {                                                                \
+    if ((dtype) == DP_FLOAT32) {                                                \
+      using T = float;                                                          \
+      return __VA_ARGS__();                                                     \
+    } else if ((dtype) == DP_FLOAT64) {                                         \
+      using T = double;                                                         \
+      return __VA_ARGS__();                                                     \
+    }                                                                           \
+    ::dp::set_error("%s: dtype %d not supported (float32=0, float64=1)", NAME,  \
+                    (int)(dtype));                                              \
+    return DP_ERR_DTYPE;                                                        \
+  }()
+
+// ---------------------------------------------------------------------------
+// Wavefront / block reductions (64-wide wavefronts).
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <typename V>
+__device__ __forceinline__ V wave_max(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    V w = __shfl_xor(v, o, 64);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// Block sum of a double in a fixed order (deterministic): waves reduce with
+// the xor butterfly, then wave 0 adds the per-wave results in wave order.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* smem) {
+  constexpr int W = BLOCK / 64;
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) r += smem[i];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double* smem) {
+  constexpr int W = BLOCK / 64;
+  v = wave_max(v);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  double r = -HUGE_VAL;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) r = r > smem[i] ? r : smem[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Fill kernel (grid-stride).  Used instead of hipMemsetAsync so that the
+// clears also behave under stream capture.
+template <typename V>
+__global__ void k_fill(V* p, int64_t n, V v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Two-stage deterministic reduction helpers (reduce.hip).
+constexpr int kReduceBlock = 256;
+constexpr int kReduceMaxBlocks = 16384;
+// Bytes of workspace used by one two-stage reduction slot.
+inline size_t reduce_slot_bytes() { return align256(sizeof(double) * kReduceMaxBlocks * 2); }
+// Final stage: out[k] (k < nout) = sum (or max when is_max[k]) of
+// partials[k*kReduceMaxBlocks + 0..nblocks), then optionally cast to the
+// dtype of out_t (DP_FLOAT32 / DP_FLOAT64 / -1 for double output).
+int finish_reduction(const double* partials, int nblocks, int nout, const int* is_max_host,
+                     void* out, int out_dtype, hipStream_t s);
+
+}  // namespace dp

@@ -1,0 +1,76 @@
+// Shared device helpers of the density scatter (density.hip, density_tiled.hip)
+// and the field gather (force.hip).
+#pragma once
+
+#include "dp_common.h"
+
+#pragma clang fp contract(off)
+
+namespace dp {
+
+template <typename T>
+struct TriArgs {
+  const T *x, *y, *wxc, *wyc, *offx, *offy, *ratio;
+  T xl, yl, bsx, bsy, inv_x, inv_y, scale;
+  int nbx, nby;
+};
+
+// Overlap of [x, x+s) with bin k along one axis:
+// min(x+s, bin_xl+b) - max(x, bin_xl), bin_xl = xl + k*b
+// (dreamplace/ops/electric_potential/src/density_function.h:12-17).
+template <typename T>
+__device__ __forceinline__ T tri(T x, T s, T xl, int k, T b) {
+  T bxl = xl + (T)k * b;
+  T e = x + s, be = bxl + b;
+  T hi = (be < e) ? be : e;
+  T lo = (x < bxl) ? bxl : x;
+  return hi - lo;
+}
+
+// Exact overlap with a bin given its centre, no boundary stretch
+// (density_function.h:19-36 with flag = true), clamped at 0.
+template <typename T>
+__device__ __forceinline__ T exact_overlap(T x, T s, T bc, T b) {
+  T bxl = bc - b / 2, bxh = bc + b / 2;
+  T e = x + s;
+  T hi = (bxh < e) ? bxh : e;
+  T lo = (x < bxl) ? bxl : x;
+  T v = hi - lo;
+  return (T(0) < v) ? v : T(0);
+}
+
+template <typename T>
+struct CellBins {
+  T x, y, sx, sy, r;
+  int bxl, bxh, byl, byh;
+};
+
+// Stretched cell origin and its clamped bin range
+// (electric_density_map.cpp:274-293: int() truncation, exclusive upper end).
+template <typename T>
+__device__ __forceinline__ CellBins<T> cell_bins(const TriArgs<T>& a, int64_t i) {
+  CellBins<T> c;
+  c.sx = a.wxc[i];
+  c.sy = a.wyc[i];
+  c.x = a.x[i] + a.offx[i];
+  c.y = a.y[i] + a.offy[i];
+  c.r = a.ratio[i];
+  c.bxl = max((int)((c.x - a.xl) * a.inv_x), 0);
+  c.bxh = min((int)((c.x + c.sx - a.xl) * a.inv_x) + 1, a.nbx);
+  c.byl = max((int)((c.y - a.yl) * a.inv_y), 0);
+  c.byh = min((int)((c.y + c.sy - a.yl) * a.inv_y) + 1, a.nby);
+  return c;
+}
+
+template <typename T>
+TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, const void* wyc,
+                         const void* offx, const void* offy, const void* ratio, double xl,
+                         double yl, double bsx, double bsy, int nbx, int nby, int64_t scale);
+
+// Tiled (LDS-binned) scatter, density_tiled.hip.
+size_t tiled_ws_bytes(int64_t num_bins);
+template <typename T>
+int scatter_tiled(const TriArgs<T>& a, const int64_t ranges[2][2], char* tiled_ws,
+                  unsigned long long* map, int* qcount, int* qlist, int64_t qcap, hipStream_t s);
+
+}  // namespace dp

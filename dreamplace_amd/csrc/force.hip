@@ -1,0 +1,160 @@
+// Electric field gather: the density op's backward.
+//
+// For each movable or filler cell i:
+//   g_x[i] = -(ratio_i * sum_{k,h} px_k*py_h*E_x[k,h]) * grad
+// (dreamplace/ops/electric_potential/src/electric_force.cpp:116-172, the
+// grad_out.mul_(grad_pos) at :96 and the negation of
+// electric_potential.py:223).  Small cells (bin span <= 3x3) accumulate in the
+// reference CPU order (k outer, h inner, in T, then *ratio); larger cells are
+// queued to a wavefront each and reduced across lanes.  Terminals get 0.
+#include "dp_density.h"
+
+#pragma clang fp contract(off)
+
+namespace dp {
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __restrict__ fx,
+                                                     const T* __restrict__ fy, const T* grad,
+                                                     int64_t num_nodes, int64_t n_mov,
+                                                     int64_t n_fill, T* out, int* qcount,
+                                                     int* qlist, int qcap) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // index space: [0, n_mov) movables, then [n_mov, n_mov + n_fix) terminals
+  // (written 0), then fillers; i.e. every node once.
+  if (t >= num_nodes) return;
+  int64_t i = t;
+  if (i >= n_mov && i < num_nodes - n_fill) {
+    out[i] = 0;
+    out[num_nodes + i] = 0;
+    return;
+  }
+  CellBins<T> c = cell_bins(a, i);
+  int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
+  T g = grad ? *grad : T(1);
+  if (nx > 3 || ny > 3) {
+    int slot = atomicAdd(qcount, 1);
+    if (slot < qcap) {
+      qlist[slot] = (int)i;
+      return;
+    }
+    T gx = 0, gy = 0;
+    for (int k = c.bxl; k < c.bxh; ++k) {
+      T px = tri(c.x, c.sx, a.xl, k, a.bsx);
+      for (int h = c.byl; h < c.byh; ++h) {
+        T area = px * tri(c.y, c.sy, a.yl, h, a.bsy);
+        int64_t idx = (int64_t)k * a.nby + h;
+        gx += area * fx[idx];
+        gy += area * fy[idx];
+      }
+    }
+    out[i] = -((gx * c.r) * g);
+    out[num_nodes + i] = -((gy * c.r) * g);
+    return;
+  }
+  T px[3], py[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    px[j] = tri(c.x, c.sx, a.xl, c.bxl + j, a.bsx);
+    py[j] = tri(c.y, c.sy, a.yl, c.byl + j, a.bsy);
+  }
+  T gx = 0, gy = 0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < nx) {
+      const int64_t row = (int64_t)(c.bxl + j) * a.nby + c.byl;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        if (l < ny) {
+          T area = px[j] * py[l];
+          gx += area * fx[row + l];
+          gy += area * fy[row + l];
+        }
+      }
+    }
+  }
+  out[i] = -((gx * c.r) * g);
+  out[num_nodes + i] = -((gy * c.r) * g);
+}
+
+// One wavefront per queued cell: lanes stride over the cell's bins, then a
+// fixed-order butterfly reduction.
+template <typename T>
+__global__ void __launch_bounds__(256) k_force_large(TriArgs<T> a, const T* __restrict__ fx,
+                                                     const T* __restrict__ fy, const T* grad,
+                                                     int64_t num_nodes, T* out,
+                                                     const int* qcount, const int* qlist,
+                                                     int qcap) {
+  int cnt = *qcount;
+  if (cnt > qcap) cnt = qcap;
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  T g = grad ? *grad : T(1);
+  for (int q = wave; q < cnt; q += nwaves) {
+    int64_t i = qlist[q];
+    CellBins<T> c = cell_bins(a, i);
+    int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
+    int total = nx * ny;
+    T gx = 0, gy = 0;
+    for (int t = lane; t < total; t += 64) {
+      int k = c.bxl + t / ny, h = c.byl + t % ny;
+      T area = tri(c.x, c.sx, a.xl, k, a.bsx) * tri(c.y, c.sy, a.yl, h, a.bsy);
+      int64_t idx = (int64_t)k * a.nby + h;
+      gx += area * fx[idx];
+      gy += area * fy[idx];
+    }
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    if (lane == 0) {
+      out[i] = -((gx * c.r) * g);
+      out[num_nodes + i] = -((gy * c.r) * g);
+    }
+  }
+}
+
+}  // namespace dp
+
+using namespace dp;
+
+extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
+                                 const void* field_map_y, const void* pos, int64_t num_nodes,
+                                 const void* node_size_x_clamped, const void* node_size_y_clamped,
+                                 const void* offset_x, const void* offset_y, const void* ratio,
+                                 double xl, double yl, double bin_size_x, double bin_size_y,
+                                 int64_t num_movable_nodes, int64_t num_filler_nodes,
+                                 int num_bins_x, int num_bins_y, int64_t large_capacity,
+                                 void* workspace, void* grad_out, void* stream) {
+  DP_ARG(field_map_x && field_map_y && pos && node_size_x_clamped && node_size_y_clamped &&
+             offset_x && offset_y && ratio && grad_out && workspace,
+         "dp_electric_force: null pointer");
+  DP_ARG(num_bins_x > 0 && num_bins_y > 0, "dp_electric_force: bad bin counts");
+  DP_ARG(num_movable_nodes >= 0 && num_filler_nodes >= 0 &&
+             num_movable_nodes + num_filler_nodes <= num_nodes && num_nodes < ((int64_t)1 << 31),
+         "dp_electric_force: bad node counts");
+  DP_ARG(large_capacity >= 0 && large_capacity < ((int64_t)1 << 31),
+         "dp_electric_force: bad large_capacity");
+  hipStream_t s = S(stream);
+  const int64_t nb = (int64_t)num_bins_x * num_bins_y;
+  // Same workspace layout as dp_density_map: [int64 map][count][queue].
+  char* p = (char*)workspace + align256(sizeof(unsigned long long) * (size_t)nb);
+  int* qcount = (int*)p;
+  int* qlist = (int*)(p + 256);
+  if (num_nodes == 0) return DP_OK;
+  return DP_DISPATCH(dtype, "dp_electric_force", [&] {
+    TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
+                                    offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
+                                    num_bins_x, num_bins_y, 1);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, qcount, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0,
+                       s, a, (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,
+                       num_nodes, num_movable_nodes, num_filler_nodes, (T*)grad_out, qcount,
+                       qlist, (int)large_capacity);
+    DP_LAUNCHED("k_force_small");
+    hipLaunchKernelGGL(k_force_large<T>, dim3(256), dim3(256), 0, s, a, (const T*)field_map_x,
+                       (const T*)field_map_y, (const T*)grad_scale, num_nodes, (T*)grad_out,
+                       (const int*)qcount, (const int*)qlist, (int)large_capacity);
+    DP_LAUNCHED("k_force_large");
+    return DP_OK;
+  });
+}

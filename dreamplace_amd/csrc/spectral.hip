@@ -1,0 +1,618 @@
+// Dense spectral transforms on the CDNA4 matrix cores, and the Poisson solve.
+//
+// Every 2-D transform of dreamplace/ops/dct/dct2_fft2.py is out = L * (x * R)
+// with constant cosine / sine matrices (weights folded in):
+//   DCT2       L = (2/M) C_M            R = (2/N) C_N^T
+//   IDCT2      L = (c C_M)^T            R = c C_N
+//   IDXST_IDCT L = 2 S_M^T              R = c C_N
+//   IDCT_IDXST L = (c C_M)^T            R = 2 S_N
+// with C_K[u,i] = cos(pi u (2i+1)/(2K)), S_K[u,i] = sin(...), c_0 = 1, c_k = 2.
+// The GEMMs run on MFMA in three arithmetic forms:
+//   FP64   v_mfma_f64_16x16x4_f64
+//   FP32   v_mfma_f32_32x32x2_f32
+//   BF16X3 v_mfma_f32_32x32x16_bf16 on a hi/lo bf16 split of both operands,
+//          hi*hi + hi*lo + lo*hi accumulated in FP32.
+// Grids whose sides are not multiples of 64 use a plain LDS-tiled FMA GEMM.
+#include <math.h>
+
+#include "dp_common.h"
+
+namespace dp {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum MatId { L_DCT = 0, L_IDCT = 1, L_IDXST = 2, R_DCT = 3, R_IDCT = 4, R_IDXST = 5 };
+
+static inline size_t elem_bytes(int precision) {
+  return precision == DP_SPECTRAL_FP64 ? sizeof(double) : sizeof(float);
+}
+
+// Plan layout: the three M x M matrices, then the three N x N matrices.
+static inline size_t plan_offset(int M, int N, int id, int precision) {
+  size_t mm = (size_t)M * M, nn = (size_t)N * N;
+  size_t off = id < 3 ? id * mm : 3 * mm + (id - 3) * nn;
+  return off * elem_bytes(precision);
+}
+
+template <typename T>
+__global__ void k_plan_fill(T* plan, int M, int N) {
+  const size_t mm = (size_t)M * M, nn = (size_t)N * N;
+  const size_t total = 3 * mm + 3 * nn;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    int id, K;
+    size_t e;
+    if (t < 3 * mm) {
+      id = (int)(t / mm);
+      e = t % mm;
+      K = M;
+    } else {
+      id = 3 + (int)((t - 3 * mm) / nn);
+      e = (t - 3 * mm) % nn;
+      K = N;
+    }
+    int r = (int)(e / K), c = (int)(e % K);
+    // angle(u, i) = pi*u*(2i+1)/(2K), reduced mod 4K exactly in integers
+    auto ang = [&](int u, int i) {
+      long long q = ((long long)u * (2 * i + 1)) % (4LL * K);
+      return M_PI * (double)q / (2.0 * K);
+    };
+    double v;
+    switch (id) {
+      case L_DCT: v = (2.0 / K) * cos(ang(r, c)); break;             // [u][i]
+      case L_IDCT: v = (c == 0 ? 1.0 : 2.0) * cos(ang(c, r)); break;  // [m][u]
+      case L_IDXST: v = 2.0 * sin(ang(c, r)); break;                  // [m][u]
+      case R_DCT: v = (2.0 / K) * cos(ang(c, r)); break;              // [j][v]
+      case R_IDCT: v = (r == 0 ? 1.0 : 2.0) * cos(ang(r, c)); break;  // [v][n]
+      default: v = 2.0 * sin(ang(r, c)); break;                       // [v][n]
+    }
+    plan[t] = (T)v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM C[MxN] = alpha * A[MxK] * B[KxN] (row-major), batched over blockIdx.z.
+// Optional epilogue: per-block FP64 partial of sum(C .* R) (the energy).
+// ---------------------------------------------------------------------------
+struct GemmDesc {
+  const void* A;
+  const void* B;
+  void* C;
+  const void* R;      // nullable: energy reference map [M x N]
+  double* partials;   // per-block partials of sum(C*R) when R != null
+};
+struct GemmBatch {
+  GemmDesc g[3];
+  int M, N, K;
+  double alpha;
+};
+
+constexpr int BT = 64;  // block tile (M and N)
+
+// ---- FP32: v_mfma_f32_32x32x2_f32, K tile 32 --------------------------------
+// LDS rows hold 32 k values permuted as pos(k) = (k&1)*16 + (k>>1), so lane
+// half h (= lane/32) reads its 16 k values (k = 2s + h) contiguously.
+constexpr int F32_BK = 32;
+constexpr int F32_LD = 36;  // padded row stride (floats)
+
+__global__ void __launch_bounds__(256) k_gemm_f32(GemmBatch bt) {
+  const GemmDesc& gd = bt.g[blockIdx.z];
+  const float* A = (const float*)gd.A;
+  const float* B = (const float*)gd.B;
+  float* C = (float*)gd.C;
+  const int M = bt.M, N = bt.N, K = bt.K;
+  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  __shared__ __attribute__((aligned(16))) float As[BT * F32_LD];
+  __shared__ __attribute__((aligned(16))) float Bs[BT * F32_LD];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  f32x16 acc = {};
+  // global load assignment
+  const int a_row = t >> 2, a_kc = (t & 3) * 8;
+  const int b_col = t & 63, b_kc = (t >> 6) * 8;
+  for (int k0 = 0; k0 < K; k0 += F32_BK) {
+    // A: 8 consecutive k of one row
+    const float4* ap = (const float4*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
+    float4 a0 = ap[0], a1 = ap[1];
+    // B: 8 consecutive k of one column (each load coalesced across lanes)
+    float bv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+    __syncthreads();
+    {
+      float* r = As + a_row * F32_LD;
+      const int e = a_kc >> 1;  // even k -> e.., odd k -> 16 + e..
+      *(float4*)(r + e) = make_float4(a0.x, a0.z, a1.x, a1.z);
+      *(float4*)(r + 16 + e) = make_float4(a0.y, a0.w, a1.y, a1.w);
+      float* q = Bs + b_col * F32_LD;
+      const int f = b_kc >> 1;
+      *(float4*)(q + f) = make_float4(bv[0], bv[2], bv[4], bv[6]);
+      *(float4*)(q + 16 + f) = make_float4(bv[1], bv[3], bv[5], bv[7]);
+    }
+    __syncthreads();
+    const float* ar = As + (wm * 32 + l32) * F32_LD + h * 16;
+    const float* br = Bs + (wn * 32 + l32) * F32_LD + h * 16;
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4) {
+      float4 av = *(const float4*)(ar + 4 * q4);
+      float4 bv4 = *(const float4*)(br + 4 * q4);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv4.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv4.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bv4.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bv4.w, acc, 0, 0, 0);
+    }
+  }
+  // epilogue: acc[r] -> row 8*(r/4) + 4*h + r%4, col l32 of the wave tile
+  const float alpha = (float)bt.alpha;
+  const float* R = (const float*)gd.R;
+  double e = 0;
+  const int col = n0 + wn * 32 + l32;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+    float v = alpha * acc[r];
+    C[(size_t)row * N + col] = v;
+    if (R) e += (double)v * (double)R[(size_t)row * N + col];
+  }
+  if (R) {
+    __shared__ double smem[4];
+    double bs = block_sum<256>(e, smem);
+    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+  }
+  (void)M;
+}
+
+// ---- BF16x3: v_mfma_f32_32x32x16_bf16, K tile 32 ---------------------------
+constexpr int BF_BK = 32;
+constexpr int BF_LD = 40;  // padded row stride (bf16 elements): 80 bytes
+
+__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    __bf16 hb = (__bf16)v[j];
+    hi[j] = hb;
+    lo[j] = (__bf16)(v[j] - (float)hb);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
+  const GemmDesc& gd = bt.g[blockIdx.z];
+  const float* A = (const float*)gd.A;
+  const float* B = (const float*)gd.B;
+  float* C = (float*)gd.C;
+  const int N = bt.N, K = bt.K;
+  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[BT * BF_LD];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  f32x16 acc = {};
+  const int a_row = t >> 2, a_kc = (t & 3) * 8;
+  const int b_col = t & 63, b_kc = (t >> 6) * 8;
+  for (int k0 = 0; k0 < K; k0 += BF_BK) {
+    const float4* ap = (const float4*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
+    float4 a0 = ap[0], a1 = ap[1];
+    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    float bv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+    bf16x8 ahi, alo, bhi, blo;
+    split8(av, ahi, alo);
+    split8(bv, bhi, blo);
+    __syncthreads();
+    *(bf16x8*)(Ah + a_row * BF_LD + a_kc) = ahi;
+    *(bf16x8*)(Al + a_row * BF_LD + a_kc) = alo;
+    *(bf16x8*)(Bh + b_col * BF_LD + b_kc) = bhi;
+    *(bf16x8*)(Bl + b_col * BF_LD + b_kc) = blo;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < BF_BK / 16; ++s) {
+      const int ko = 16 * s + 8 * h;
+      const int ra = (wm * 32 + l32) * BF_LD + ko;
+      const int rb = (wn * 32 + l32) * BF_LD + ko;
+      bf16x8 xh = *(const bf16x8*)(Ah + ra);
+      bf16x8 xl = *(const bf16x8*)(Al + ra);
+      bf16x8 yh = *(const bf16x8*)(Bh + rb);
+      bf16x8 yl = *(const bf16x8*)(Bl + rb);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc, 0, 0, 0);
+    }
+  }
+  const float alpha = (float)bt.alpha;
+  const float* R = (const float*)gd.R;
+  double e = 0;
+  const int col = n0 + wn * 32 + l32;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+    float v = alpha * acc[r];
+    C[(size_t)row * N + col] = v;
+    if (R) e += (double)v * (double)R[(size_t)row * N + col];
+  }
+  if (R) {
+    __shared__ double smem[4];
+    double bs = block_sum<256>(e, smem);
+    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+  }
+}
+
+// ---- FP64: v_mfma_f64_16x16x4_f64, K tile 16 --------------------------------
+// Each wave owns a 32x32 tile = 2x2 MFMA tiles.  LDS rows hold 16 k values
+// permuted as pos(k) = (k&3)*4 + (k>>2), so lane quarter q (= lane/16) reads
+// its 4 k values (k = 4s + q) contiguously.
+constexpr int F64_BK = 16;
+constexpr int F64_LD = 18;  // padded row stride (doubles)
+
+__global__ void __launch_bounds__(256) k_gemm_f64(GemmBatch bt) {
+  const GemmDesc& gd = bt.g[blockIdx.z];
+  const double* A = (const double*)gd.A;
+  const double* B = (const double*)gd.B;
+  double* C = (double*)gd.C;
+  const int N = bt.N, K = bt.K;
+  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  __shared__ __attribute__((aligned(16))) double As[BT * F64_LD];
+  __shared__ __attribute__((aligned(16))) double Bs[BT * F64_LD];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int q = lane >> 4, l16 = lane & 15;
+  f64x4 acc[2][2] = {};
+  const int a_row = t >> 2, a_kc = (t & 3) * 4;  // 4 consecutive k of one row
+  const int b_col = t & 63, b_kc = (t >> 6) * 4;
+  for (int k0 = 0; k0 < K; k0 += F64_BK) {
+    const double* ap = A + (size_t)(m0 + a_row) * K + k0 + a_kc;
+    double av[4] = {ap[0], ap[1], ap[2], ap[3]};
+    double bv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = a_kc + j;
+      As[a_row * F64_LD + (k & 3) * 4 + (k >> 2)] = av[j];
+      const int kb = b_kc + j;
+      Bs[b_col * F64_LD + (kb & 3) * 4 + (kb >> 2)] = bv[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < F64_BK / 4; ++s) {
+      double a0 = As[(wm * 32 + l16) * F64_LD + q * 4 + s];
+      double a1 = As[(wm * 32 + 16 + l16) * F64_LD + q * 4 + s];
+      double b0 = Bs[(wn * 32 + l16) * F64_LD + q * 4 + s];
+      double b1 = Bs[(wn * 32 + 16 + l16) * F64_LD + q * 4 + s];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  const double alpha = bt.alpha;
+  const double* R = (const double*)gd.R;
+  double e = 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // v_mfma_f64_16x16x4_f64 accumulator layout: acc[r] of lane l holds
+        // D[4*r + l/16][l%16] (measured with one-hot operands on gfx950)
+        const int row = m0 + wm * 32 + 16 * i + 4 * r + q;
+        const int col = n0 + wn * 32 + 16 * j + l16;
+        double v = alpha * acc[i][j][r];
+        C[(size_t)row * N + col] = v;
+        if (R) e += v * R[(size_t)row * N + col];
+      }
+  if (R) {
+    __shared__ double smem[4];
+    double bs = block_sum<256>(e, smem);
+    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+  }
+}
+
+// ---- Generic LDS-tiled FMA GEMM for any size (T = float or double) ---------
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemm_generic(GemmBatch bt) {
+  const GemmDesc& gd = bt.g[blockIdx.z];
+  const T* A = (const T*)gd.A;
+  const T* B = (const T*)gd.B;
+  T* C = (T*)gd.C;
+  const int M = bt.M, N = bt.N, K = bt.K;
+  __shared__ T As[16][17];
+  __shared__ T Bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  T acc = 0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    As[ty][tx] = (row < M && k0 + tx < K) ? A[(size_t)row * K + k0 + tx] : T(0);
+    Bs[ty][tx] = (k0 + ty < K && col < N) ? B[(size_t)(k0 + ty) * N + col] : T(0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    __syncthreads();
+  }
+  double e = 0;
+  if (row < M && col < N) {
+    T v = (T)bt.alpha * acc;
+    C[(size_t)row * N + col] = v;
+    if (gd.R) e = (double)v * (double)((const T*)gd.R)[(size_t)row * N + col];
+  }
+  if (gd.R) {
+    __shared__ double smem[4];
+    double bs = block_sum<256>(e, smem);
+    if (threadIdx.x == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+  }
+}
+
+static bool mfma_shape(int M, int N) { return M % BT == 0 && N % BT == 0; }
+
+// Launch a batch of `count` GEMMs of one shape; returns the number of
+// per-GEMM energy partial blocks.
+static int launch_gemm(int precision, GemmBatch& bt, int count, hipStream_t s, int* nblocks) {
+  dim3 block(256);
+  if (mfma_shape(bt.M, bt.N) && bt.K % BT == 0) {
+    dim3 grid(bt.N / BT, bt.M / BT, count);
+    if (nblocks) *nblocks = (int)(grid.x * grid.y);
+    if (precision == DP_SPECTRAL_FP64)
+      hipLaunchKernelGGL(k_gemm_f64, grid, block, 0, s, bt);
+    else if (precision == DP_SPECTRAL_FP32)
+      hipLaunchKernelGGL(k_gemm_f32, grid, block, 0, s, bt);
+    else
+      hipLaunchKernelGGL(k_gemm_bf16x3, grid, block, 0, s, bt);
+  } else {
+    dim3 grid((unsigned)ceil_div(bt.N, 16), (unsigned)ceil_div(bt.M, 16), count);
+    bool energy = false;
+    for (int i = 0; i < count; ++i) energy |= bt.g[i].R != nullptr;
+    DP_REQUIRE(!energy || (int64_t)grid.x * grid.y <= kReduceMaxBlocks, DP_ERR_UNSUPPORTED,
+               "spectral: %d x %d grid needs the matrix-core path (sides multiple of %d) for the "
+               "fused energy",
+               bt.M, bt.N, BT);
+    if (nblocks) *nblocks = (int)(grid.x * grid.y);
+    if (precision == DP_SPECTRAL_FP64)
+      hipLaunchKernelGGL(k_gemm_generic<double>, grid, block, 0, s, bt);
+    else
+      hipLaunchKernelGGL(k_gemm_generic<float>, grid, block, 0, s, bt);
+  }
+  DP_LAUNCHED("spectral gemm");
+  return DP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise pieces of the solve.
+// ---------------------------------------------------------------------------
+// Partial sums of the map (FP64) for its mean.
+template <typename T>
+__global__ void __launch_bounds__(kReduceBlock) k_sum_partial(const T* in, int64_t n,
+                                                              double* partials) {
+  __shared__ double smem[kReduceBlock / 64];
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)in[i];
+  double bs = block_sum<kReduceBlock>(acc, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// rho = map / (bx*by) (electric_potential.py:160), centred: the mean is
+// subtracted in FP64 before rounding to T.  The DC coefficient of rho does not
+// reach the fields or the potential (K[0,0] = 0, and sum(phi) = 0 so the
+// energy sum(phi*rho) is unchanged), while centring halves the rounding error
+// of every other DCT coefficient.
+template <typename T>
+__global__ void k_scale_centre(const T* in, int64_t n, double s, const double* sum, T* out) {
+  const double mean = (*sum) * s / (double)n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (T)((double)in[i] * s - mean);
+}
+
+// Spectral multipliers of electric_potential.py:504-519, evaluated in FP64
+// per element: wu = 2 pi u/M, wv = 2 pi v/N * (bx/by), K = 1/(wu^2+wv^2),
+// K[0,0] = 0.  Writes a*wu*K/2, a*wv*K/2 and (optionally) a*K.
+template <typename T>
+__global__ void k_spectral_weights(const T* a, int M, int N, double aspect, T* ax, T* ay, T* ak) {
+  const int64_t n = (int64_t)M * N;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int u = (int)(i / N), v = (int)(i % N);
+    double wu = (2.0 * M_PI / M) * u;
+    double wv = (2.0 * M_PI / N) * v * aspect;
+    double w2 = wu * wu + wv * wv;
+    double inv = (u == 0 && v == 0) ? 0.0 : 1.0 / w2;
+    T av = a[i];
+    ax[i] = av * (T)(wu * inv * 0.5);
+    ay[i] = av * (T)(wv * inv * 0.5);
+    if (ak) ak[i] = av * (T)inv;
+  }
+}
+
+// Spectral energy partials: (MN/4) sum c_u c_v K a^2, FP64.
+template <typename T>
+__global__ void __launch_bounds__(kReduceBlock) k_spectral_energy(const T* a, int M, int N,
+                                                                  double aspect, double* partials) {
+  __shared__ double smem[kReduceBlock / 64];
+  const int64_t n = (int64_t)M * N;
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int u = (int)(i / N), v = (int)(i % N);
+    if (u == 0 && v == 0) continue;
+    double wu = (2.0 * M_PI / M) * u;
+    double wv = (2.0 * M_PI / N) * v * aspect;
+    double av = (double)a[i];
+    acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * av * av / (wu * wu + wv * wv);
+  }
+  double bs = block_sum<kReduceBlock>(acc, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
+}
+
+// Workspace of the spectral ops: seven M*N slabs of the data type, then one
+// reduction slot.  The single transforms use the first slab only.
+static size_t ws_slab(int M, int N, int precision) {
+  return align256((size_t)M * N * elem_bytes(precision));
+}
+
+static int transform(int precision, const void* plan, int M, int N, const void* x, void* out,
+                     void* workspace, int lid, int rid, hipStream_t s) {
+  GemmBatch bt = {};
+  const char* P = (const char*)plan;
+  char* tmp = (char*)workspace;
+  // stage 1: tmp = x * R   (M x N) = (M x N)(N x N)
+  bt.M = M;
+  bt.N = N;
+  bt.K = N;
+  bt.alpha = 1.0;
+  bt.g[0].A = x;
+  bt.g[0].B = P + plan_offset(M, N, rid, precision);
+  bt.g[0].C = tmp;
+  DP_CHECK_RC(launch_gemm(precision, bt, 1, s, nullptr));
+  // stage 2: out = L * tmp  (M x N) = (M x M)(M x N)
+  bt.K = M;
+  bt.g[0].A = P + plan_offset(M, N, lid, precision);
+  bt.g[0].B = tmp;
+  bt.g[0].C = out;
+  return launch_gemm(precision, bt, 1, s, nullptr);
+}
+
+static int check_common(int precision, const void* plan, int M, int N, const void* x, void* out,
+                        void* ws, const char* name) {
+  DP_ARG(precision == DP_SPECTRAL_FP64 || precision == DP_SPECTRAL_FP32 ||
+             precision == DP_SPECTRAL_BF16X3,
+         "%s: bad precision %d", name, precision);
+  DP_ARG(plan && x && out && ws, "%s: null pointer", name);
+  DP_ARG(M > 0 && N > 0 && M <= 16384 && N <= 16384, "%s: bad size %d x %d", name, M, N);
+  return DP_OK;
+}
+
+}  // namespace dp
+
+using namespace dp;
+
+extern "C" {
+
+size_t dp_spectral_plan_bytes(int M, int N, int precision) {
+  return plan_offset(M, N, 6, precision);
+}
+
+int dp_spectral_plan_init(int M, int N, int precision, void* plan, void* stream) {
+  DP_ARG(plan && M > 0 && N > 0, "dp_spectral_plan_init: bad argument");
+  hipStream_t s = S(stream);
+  size_t total = 3 * (size_t)M * M + 3 * (size_t)N * N;
+  int grid = grid_for((int64_t)total, 256);
+  if (precision == DP_SPECTRAL_FP64)
+    hipLaunchKernelGGL(k_plan_fill<double>, dim3(grid), dim3(256), 0, s, (double*)plan, M, N);
+  else if (precision == DP_SPECTRAL_FP32 || precision == DP_SPECTRAL_BF16X3)
+    hipLaunchKernelGGL(k_plan_fill<float>, dim3(grid), dim3(256), 0, s, (float*)plan, M, N);
+  else
+    DP_ARG(false, "dp_spectral_plan_init: bad precision %d", precision);
+  DP_LAUNCHED("k_plan_fill");
+  return DP_OK;
+}
+
+size_t dp_spectral_workspace_bytes(int M, int N, int precision) {
+  return 7 * ws_slab(M, N, precision) + reduce_slot_bytes();
+}
+
+#define DP_TRANSFORM_ENTRY(NAME, LID, RID)                                                   \
+  int NAME(int precision, const void* plan, int M, int N, const void* x, void* out,          \
+           void* workspace, void* stream) {                                                  \
+    DP_CHECK_RC(check_common(precision, plan, M, N, x, out, workspace, #NAME));              \
+    return transform(precision, plan, M, N, x, out, workspace, LID, RID, S(stream));         \
+  }
+
+DP_TRANSFORM_ENTRY(dp_dct2, L_DCT, R_DCT)
+DP_TRANSFORM_ENTRY(dp_idct2, L_IDCT, R_IDCT)
+DP_TRANSFORM_ENTRY(dp_idxst_idct, L_IDXST, R_IDCT)
+DP_TRANSFORM_ENTRY(dp_idct_idxst, L_IDCT, R_IDXST)
+
+int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* density_map,
+                     double bin_size_x, double bin_size_y, int compute_potential, void* field_x,
+                     void* field_y, void* potential, void* energy, void* workspace,
+                     void* stream) {
+  DP_CHECK_RC(check_common(precision, plan, M, N, density_map, field_x, workspace,
+                           "dp_poisson_solve"));
+  DP_ARG(field_y, "dp_poisson_solve: null field_y");
+  DP_ARG(!compute_potential || potential, "dp_poisson_solve: compute_potential needs potential");
+  hipStream_t s = S(stream);
+  const size_t slab = ws_slab(M, N, precision);
+  char* w = (char*)workspace;
+  void* rho = w;               // map/(bx*by), kept for the energy
+  void* tmp = w + slab;        // stage-1 products
+  void* auv = w + 2 * slab;    // DCT2(rho), then Yy
+  void* ax = w + 3 * slab;     // a*wu*K/2
+  void* ay = w + 4 * slab;     // a*wv*K/2
+  void* ak = w + 5 * slab;     // a*K
+  void* yphi = w + 6 * slab;   // ak * R_IDCT
+  double* part = (double*)(w + 7 * slab);
+  const char* P = (const char*)plan;
+  const int64_t n = (int64_t)M * N;
+  const int dt = precision == DP_SPECTRAL_FP64 ? DP_FLOAT64 : DP_FLOAT32;
+  const double aspect = bin_size_x / bin_size_y;
+  const bool want_phi = compute_potential != 0;
+  // rho = map / (bx*by), centred; a = DCT2(rho); spectral multipliers
+  double* msum = part + kReduceMaxBlocks;  // second half of the reduction slot
+  DP_CHECK_RC(DP_DISPATCH(dt, "dp_poisson_solve", [&] {
+    const int nb = (int)(ceil_div(n, kReduceBlock) < 1024 ? ceil_div(n, kReduceBlock) : 1024);
+    hipLaunchKernelGGL(k_sum_partial<T>, dim3(nb), dim3(kReduceBlock), 0, s,
+                       (const T*)density_map, n, part);
+    DP_LAUNCHED("k_sum_partial");
+    DP_CHECK_RC(finish_reduction(part, nb, 1, nullptr, msum, -1, s));
+    hipLaunchKernelGGL(k_scale_centre<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const T*)density_map, n, 1.0 / (bin_size_x * bin_size_y),
+                       (const double*)msum, (T*)rho);
+    DP_LAUNCHED("k_scale_centre");
+    return DP_OK;
+  }));
+  DP_CHECK_RC(transform(precision, plan, M, N, rho, auv, tmp, L_DCT, R_DCT, s));
+  int eblk = 0;
+  DP_CHECK_RC(DP_DISPATCH(dt, "dp_poisson_solve", [&] {
+    hipLaunchKernelGGL(k_spectral_weights<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const T*)auv, M, N, aspect, (T*)ax, (T*)ay, want_phi ? (T*)ak : nullptr);
+    DP_LAUNCHED("k_spectral_weights");
+    if (energy && !want_phi) {
+      eblk = (int)(ceil_div(n, kReduceBlock) < 1024 ? ceil_div(n, kReduceBlock) : 1024);
+      hipLaunchKernelGGL(k_spectral_energy<T>, dim3(eblk), dim3(kReduceBlock), 0, s,
+                         (const T*)auv, M, N, aspect, part);
+      DP_LAUNCHED("k_spectral_energy");
+    }
+    return DP_OK;
+  }));
+  // stage 1 of the inverse transforms (one batched launch):
+  //   Yx = ax*R_IDCT -> tmp, Yy = ay*R_IDXST -> auv, Yphi = ak*R_IDCT -> yphi
+  GemmBatch bt = {};
+  bt.M = M;
+  bt.N = N;
+  bt.K = N;
+  bt.alpha = 1.0;
+  bt.g[0] = {ax, P + plan_offset(M, N, R_IDCT, precision), tmp, nullptr, nullptr};
+  bt.g[1] = {ay, P + plan_offset(M, N, R_IDXST, precision), auv, nullptr, nullptr};
+  int count = 2;
+  if (want_phi) {
+    bt.g[2] = {ak, P + plan_offset(M, N, R_IDCT, precision), yphi, nullptr, nullptr};
+    count = 3;
+  }
+  DP_CHECK_RC(launch_gemm(precision, bt, count, s, nullptr));
+  // stage 2: field_x = L_IDXST*Yx, field_y = L_IDCT*Yy, potential = L_IDCT*Yphi
+  // (+ energy partials sum(potential*rho) in the epilogue)
+  bt.K = M;
+  bt.g[0] = {P + plan_offset(M, N, L_IDXST, precision), tmp, field_x, nullptr, nullptr};
+  bt.g[1] = {P + plan_offset(M, N, L_IDCT, precision), auv, field_y, nullptr, nullptr};
+  if (want_phi)
+    bt.g[2] = {P + plan_offset(M, N, L_IDCT, precision), yphi, potential,
+               energy ? rho : nullptr, part};
+  int nblk = 0;
+  DP_CHECK_RC(launch_gemm(precision, bt, count, s, &nblk));
+  if (energy) {
+    DP_CHECK_RC(finish_reduction(part, want_phi ? nblk : eblk, 1, nullptr, energy, dt, s));
+  }
+  return DP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+/*
+ * dreamplace_amd.h -- C ABI of the MI355X (gfx950) global-placement inner loop.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), enqueues its kernels on that stream and returns
+ * DP_OK (0) or an error code; it never synchronises the stream and never
+ * allocates device memory.  Scratch memory comes from the caller as a
+ * `workspace` pointer sized by the matching *_workspace_bytes() query.
+ * Negative codes are argument errors; positive codes are hipError_t values
+ * from the launch.  dp_last_error() returns a message for the calling
+ * thread's last failure.
+ *
+ * Floating-point arrays are typed by `dtype` (DP_FLOAT32 or DP_FLOAT64,
+ * mirroring DREAMPLACE_DISPATCH_FLOATING_TYPES in
+ * dreamplace/ops/utility/src/torch.h:83-92, which raises for other types).
+ * Positions use the reference layout: pos[2*num_nodes] holds all x then all
+ * y, nodes ordered movable, fixed (terminals), filler.  Bin maps are
+ * [num_bins_x][num_bins_y] row-major (x-bin major).
+ *
+ * Each declaration names the reference native op it replaces (file:line).
+ */
+#ifndef DREAMPLACE_AMD_H
+#define DREAMPLACE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DP_OK 0
+#define DP_ERR_INVALID_ARGUMENT (-1)
+#define DP_ERR_DTYPE (-2)
+#define DP_ERR_UNSUPPORTED (-3)
+#define DP_ERR_WORKSPACE (-4)
+
+enum { DP_FLOAT32 = 0, DP_FLOAT64 = 1 };
+enum { DP_INDEX_INT32 = 0, DP_INDEX_INT64 = 1 };
+/* Conversion of the int64 fixed-point map back to floating point:
+ * DP_CONVERT_CPU follows electric_density_map.cpp:125-140 (init + sum*2^-f in
+ * double, rounded to T); DP_CONVERT_GPU follows
+ * electric_density_map_cuda_kernel.cu:433-463 (init quantised first, then
+ * T(u64) * T(2^-f)). */
+enum { DP_CONVERT_CPU = 0, DP_CONVERT_GPU = 1 };
+/* Arithmetic of the dense spectral transforms. */
+enum { DP_SPECTRAL_FP64 = 0, DP_SPECTRAL_FP32 = 1, DP_SPECTRAL_BF16X3 = 2 };
+/* Density scatter algorithm: per-cell global int64 atomics, or cells binned
+ * into LDS-resident bin tiles first. */
+enum { DP_SCATTER_GLOBAL = 0, DP_SCATTER_TILED = 1 };
+
+const char* dp_last_error(void);
+int dp_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Density map (fixed-point int64 accumulation; always deterministic).
+ * ------------------------------------------------------------------------- */
+
+/* Fixed-point scale 2^(64 - max(ceil(log2(area))+1, 32)) for a die
+ * (electric_density_map.cpp:117-123). */
+int64_t dp_fixed_point_scale(double xl, double yl, double xh, double yh);
+
+size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, int64_t num_nodes,
+                                      int64_t large_capacity);
+
+/* Replaces electric_potential_cuda.density_map
+ * (dreamplace/ops/electric_potential/src/electric_density_map_cuda.cpp:71-136;
+ * CPU twin electric_density_map.cpp:100-152).  Scatters movable cells
+ * [0, num_movable_nodes) and fillers [num_nodes-num_filler_nodes, num_nodes)
+ * with the triangle (stretched-cell) model into an int64 fixed-point map and
+ * writes density_map_out = initial_density_map + map (per `convention`).
+ * `fixed_map_out` (nullable) receives the raw int64 sum (no initial map).
+ * `sorted_node_map` is accepted for signature parity and not needed: the
+ * integer sum is order-free.  large_capacity bounds the queue of cells
+ * spanning more than 3x3 bins (any count is correct; a larger queue only
+ * moves more of them off the per-thread path). */
+int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
+                   const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
+                   const void* ratio, const void* initial_density_map, double xl, double yl,
+                   double xh, double yh, double bin_size_x, double bin_size_y,
+                   int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                   int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                   void* workspace, void* density_map_out, int64_t* fixed_map_out, void* stream);
+
+/* Replaces electric_potential_cuda.fixed_density_map
+ * (electric_density_map_cuda.cpp:139-182, electric_density_map.cpp:155-213)
+ * followed by the Python-side scaling of electric_overflow.py:258:
+ * out = min(exact overlap map of terminals, bin area) * target_density.
+ * Terminals are nodes [num_movable_nodes, num_movable_nodes+num_terminals). */
+int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x,
+                         const void* node_size_y, const void* bin_center_x,
+                         const void* bin_center_y, double xl, double yl, double xh, double yh,
+                         double bin_size_x, double bin_size_y, int64_t num_movable_nodes,
+                         int64_t num_terminals, int num_bins_x, int num_bins_y,
+                         double target_density, int convention, void* workspace, void* out,
+                         void* stream);
+
+/* Replaces -electric_potential_cuda.electric_force(...) as used by
+ * ElectricPotentialFunction.backward (electric_potential.py:219-245;
+ * electric_force_cuda.cpp:56-126, electric_force.cpp:76-172).
+ * grad_out[2N] = -(ratio_i * sum_bins px*py*E) * (*grad_scale) for movables and
+ * fillers, 0 for terminals.  grad_scale is a device scalar of `dtype`
+ * (nullable: 1).  workspace as for dp_density_map (same large_capacity). */
+int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
+                      const void* field_map_y, const void* pos, int64_t num_nodes,
+                      const void* node_size_x_clamped, const void* node_size_y_clamped,
+                      const void* offset_x, const void* offset_y, const void* ratio, double xl,
+                      double yl, double bin_size_x, double bin_size_y, int64_t num_movable_nodes,
+                      int64_t num_filler_nodes, int num_bins_x, int num_bins_y,
+                      int64_t large_capacity, void* workspace, void* grad_out, void* stream);
+
+/* ElectricOverflow metric (electric_overflow.py:260-278): out2[0] =
+ * sum(max(map - target_area, 0)), out2[1] = max(map); FP64 accumulation,
+ * deterministic.  out2 is a device double[2]. */
+size_t dp_reduce_workspace_bytes(int64_t n);
+int dp_density_overflow(int dtype, const void* density_map, int64_t num_bins, double target_area,
+                        void* workspace, double* out2, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Spectral transforms (dense, on the matrix cores) and the Poisson solve.
+ * Data arrays are double for DP_SPECTRAL_FP64 and float otherwise.
+ * ------------------------------------------------------------------------- */
+
+/* A plan holds the cosine/sine transform matrices for an M x N grid. */
+size_t dp_spectral_plan_bytes(int M, int N, int precision);
+int dp_spectral_plan_init(int M, int N, int precision, void* plan, void* stream);
+size_t dp_spectral_workspace_bytes(int M, int N, int precision);
+
+/* Replace dct2_fft2_cuda.{dct2_fft2, idct2_fft2, idct_idxst, idxst_idct}
+ * (dreamplace/ops/dct/src/dct2_fft2_cuda.cpp:14-141); conventions of
+ * dreamplace/ops/dct/dct2_fft2.py:21-150:
+ *   DCT2       out[u,v] = 4/(MN) sum x[i,j] cos(pi u(2i+1)/2M) cos(pi v(2j+1)/2N)
+ *   IDCT2      out[m,n] = sum c_u c_v x[u,v] cos(.. m ..) cos(.. n ..), c_0=1, c_k=2
+ *   IDXST_IDCT out[m,n] = 2 sum c_v x[u,v] sin(pi u(2m+1)/2M) cos(pi v(2n+1)/2N)
+ *   IDCT_IDXST out[m,n] = 2 sum c_u x[u,v] cos(pi u(2m+1)/2M) sin(pi v(2n+1)/2N) */
+int dp_dct2(int precision, const void* plan, int M, int N, const void* x, void* out,
+            void* workspace, void* stream);
+int dp_idct2(int precision, const void* plan, int M, int N, const void* x, void* out,
+             void* workspace, void* stream);
+int dp_idxst_idct(int precision, const void* plan, int M, int N, const void* x, void* out,
+                  void* workspace, void* stream);
+int dp_idct_idxst(int precision, const void* plan, int M, int N, const void* x, void* out,
+                  void* workspace, void* stream);
+
+/* The solve of ElectricPotentialFunction.forward (electric_potential.py:137-186):
+ * rho = density_map/(bx*by); a = DCT2(rho); field_x = IDXST_IDCT(a*wu/(2(wu^2+wv^2)));
+ * field_y = IDCT_IDXST(a*wv/(2(wu^2+wv^2))); optionally potential = IDCT2(a/(wu^2+wv^2))
+ * and *energy = sum(potential*rho) (device scalar of the data type; nullable).
+ * With compute_potential = 0 and energy non-null, the energy is evaluated in
+ * the spectral domain, (MN/4) sum c_u c_v a^2/(wu^2+wv^2), which equals
+ * sum(potential*rho) exactly in exact arithmetic. */
+int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* density_map,
+                     double bin_size_x, double bin_size_y, int compute_potential, void* field_x,
+                     void* field_y, void* potential, void* energy, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Pin positions and weighted-average wirelength.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces pin_pos_cuda.forward (dreamplace/ops/pin_pos/src/pin_pos_cuda.cpp:27-54,
+ * kernel pin_pos_cuda_kernel.cu:11-26): pin_pos[p] = offset[p] + pos[pin2node[p]]. */
+int dp_pin_pos_forward(int dtype, const void* pos, int64_t num_nodes, const void* pin_offset_x,
+                       const void* pin_offset_y, const void* pin2node_map, int index_type,
+                       int64_t num_pins, void* pin_pos_out, void* stream);
+
+/* Replaces pin_pos_cuda.backward (pin_pos_cuda.cpp:57-93, kernel
+ * pin_pos_cuda_kernel.cu:50-100): grad_pos[n] = sum of grad_pin_pos over the
+ * CSR pins of physical node n; fillers get 0. */
+int dp_pin_pos_backward(int dtype, const void* grad_pin_pos, int64_t num_pins,
+                        const int32_t* flat_node2pin_map, const int32_t* flat_node2pin_start_map,
+                        int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
+                        void* stream);
+
+size_t dp_wa_workspace_bytes(int64_t num_nets);
+
+/* Replaces weighted_average_wirelength_cuda_merged.forward
+ * (weighted_average_wirelength_cuda_merged.cpp:63-110, kernel
+ * weighted_average_wirelength_cuda_merged_kernel.cu:16-85).  Writes the
+ * pre-weight per-net WL (partial_wl, nullable), the per-pin derivative of the
+ * net WL (grad_intermediate[2P], entries of masked nets untouched) and
+ * *wl_out = sum_n w_n*WL_n (device scalar of dtype, FP64 accumulation,
+ * deterministic).  net_weights nullable (all 1).  inv_gamma is a device scalar. */
+int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
+                         const int32_t* flat_netpin, const int32_t* netpin_start,
+                         int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                         const void* inv_gamma, void* partial_wl, void* grad_intermediate,
+                         void* wl_out, void* workspace, void* stream);
+
+/* Replaces weighted_average_wirelength_cuda_merged.backward
+ * (weighted_average_wirelength_cuda_merged.cpp:121-160,
+ * integrate_net_weights_cuda_kernel.cu:17-48) plus the pin_mask fill of
+ * weighted_average_wirelength.py:207-208, in place on grad_intermediate:
+ * g[p] = gi[p]*(*grad)*w[net(p)] (weight applied for unmasked nets), 0 where
+ * pin_mask[p] != 0. */
+int dp_wa_merged_backward(int dtype, const void* grad, void* grad_intermediate, int64_t num_pins,
+                          const int32_t* pin2net_map, const void* net_weights,
+                          const uint8_t* net_mask, const uint8_t* pin_mask, void* stream);
+
+/* New (no counterpart in the reference, which returns None for the weights,
+ * weighted_average_wirelength.py:213): d(sum_n w_n WL_n)/d(w_n) times *grad,
+ * = (*grad)*partial_wl[n] for unmasked nets, 0 otherwise. */
+int dp_wa_net_weight_grad(int dtype, const void* grad, const void* partial_wl,
+                          const uint8_t* net_mask, int64_t num_nets, void* grad_net_weights,
+                          void* stream);
+
+/* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
+ * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64
+ * accumulation into a device double, deterministic. */
+int dp_hpwl(int dtype, const void* pin_pos, int64_t num_pins, const int32_t* flat_netpin,
+            const int32_t* netpin_start, int64_t num_nets, const void* net_weights,
+            const uint8_t* net_mask, void* workspace, double* hpwl_out, void* stream);
+
+/* Replaces move_boundary_cuda.forward (dreamplace/ops/move_boundary/src/
+ * move_boundary_cuda_kernel.cu:10-75): in-place clamp of movables and fillers. */
+int dp_move_boundary(int dtype, void* pos, int64_t num_nodes, const void* node_size_x,
+                     const void* node_size_y, double xl, double yl, double xh, double yh,
+                     int64_t num_movable_nodes, int64_t num_filler_nodes, void* stream);
+
+/* Replaces pws_cuda.forward (dreamplace/ops/pin_weight_sum/src/pws_cuda_kernel.cu:10-21):
+ * out[n] = sum of net_weights over the nets of physical node n's pins; out has
+ * num_nodes entries, fillers 0. */
+int dp_pin_weight_sum(int dtype, const void* net_weights, const int32_t* flat_nodepin,
+                      const int32_t* nodepin_start, const int32_t* pin2net_map,
+                      int64_t num_physical_nodes, int64_t num_nodes, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DREAMPLACE_AMD_H */

@@ -1,0 +1,114 @@
+"""GPU parity of the density scatter, fixed-cell map, overflow and field gather
+against the oracle.  Integer work is bit-exact; the gather is compared within
+a floating-point tolerance stated per test."""
+import numpy as np
+import pytest
+import torch
+
+from _util import density_params, design, to_dev
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (0, {}, "config0"),
+    (1, {"scale": 0.002}, "config1-small"),
+    (4, {"scale": 0.004, "max_fillers": 200000}, "config4-small-macros"),
+]
+
+
+def _op(d, t, dtype, convention="cpu", algorithm="global"):
+    from dreamplace_amd.electric_potential import ElectricPotential
+    return ElectricPotential(
+        t["node_size_x"], t["node_size_y"], t["bin_center_x"], t["bin_center_y"],
+        d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+        d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+        t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"],
+        convention=convention, algorithm=algorithm)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("convention", ["cpu", "gpu"])
+@pytest.mark.parametrize("case", CASES, ids=[c[2] for c in CASES])
+def test_density_map_bit_exact(oracle, gpu, case, dtype, convention):
+    cfg, kw, name = case
+    print("case density_map %s %s %s" % (name, np.dtype(dtype).name, convention), flush=True)
+    d = design(cfg, dtype=dtype, **kw)
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype, convention)
+    op.compute_initial_density_map(t["pos"])
+    raw = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
+    dmap = op.density_map(t["pos"], fixed_map_out=raw)
+    torch.cuda.synchronize()
+    dp = density_params(oracle, d)
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    assert np.array_equal(op.initial_density_map.cpu().numpy(), init), "fixed map differs"
+    ref_raw = oracle.density_fixed_sum(dp, d.pos)
+    assert np.array_equal(raw.cpu().numpy(), ref_raw), "int64 fixed-point sum differs"
+    ref_map = oracle.density_map(dp, d.pos, init, convention=convention)
+    assert np.array_equal(dmap.cpu().numpy(), ref_map), "converted map differs"
+
+
+def test_density_map_order_free(oracle, gpu):
+    """Permuting movables gives a bit-identical int64 map (any launch order)."""
+    print("case density_map permutation", flush=True)
+    d = design(0)
+    t = to_dev(d, gpu)
+    op = _op(d, t, torch.float64)
+    op.compute_initial_density_map(t["pos"])
+    raw0 = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
+    op.density_map(t["pos"], fixed_map_out=raw0)
+    perm = np.random.default_rng(1).permutation(d.num_movable_nodes)
+    n = d.num_nodes
+    idx = np.concatenate([perm, np.arange(d.num_movable_nodes, n)])
+    d2 = dict(d)
+    for k in ("node_size_x", "node_size_y"):
+        d2[k] = d[k][idx]
+    pos2 = np.concatenate([d.pos[:n][idx], d.pos[n:][idx]])
+    t2 = {k: torch.from_numpy(np.ascontiguousarray(v)).to(gpu) for k, v in
+          [("node_size_x", d2["node_size_x"]), ("node_size_y", d2["node_size_y"]), ("pos", pos2)]}
+    t3 = dict(t)
+    t3.update(t2)
+    op2 = _op(d, t3, torch.float64)
+    op2.initial_density_map = op.initial_density_map
+    raw1 = torch.empty_like(raw0)
+    op2.density_map(t3["pos"], fixed_map_out=raw1)
+    assert torch.equal(raw0, raw1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_overflow_metric(oracle, gpu, dtype):
+    print("case overflow %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype)
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype)
+    cost, mx = op(t["pos"], mode="overflow")
+    dp = density_params(oracle, d)
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    m = oracle.density_map(dp, d.pos, init, num_filler_nodes=0)
+    a = d.bin_size_x * d.bin_size_y
+    ref_cost = np.sum(np.maximum(m.astype(np.float64) - d.target_density * a, 0))
+    assert abs(cost.item() - ref_cost) <= 1e-6 * ref_cost
+    assert abs(mx.item() - m.max() / a) <= 1e-6 * (m.max() / a)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)], ids=["f64", "f32"])
+@pytest.mark.parametrize("case", CASES, ids=[c[2] for c in CASES])
+def test_electric_force(oracle, gpu, case, dtype, tol):
+    cfg, kw, name = case
+    print("case electric_force %s %s" % (name, np.dtype(dtype).name), flush=True)
+    d = design(cfg, dtype=dtype, **kw)
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype)
+    rng = np.random.default_rng(7)
+    fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    g = torch.tensor(0.75, dtype=t["pos"].dtype, device=gpu)
+    out = op.electric_force(t["pos"], torch.from_numpy(fx).to(gpu), torch.from_numpy(fy).to(gpu), g)
+    dp = density_params(oracle, d)
+    ref = oracle.electric_force(dp, d.pos, fx, fy, grad=0.75)
+    got = out.cpu().numpy()
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(got - ref)) <= tol * scale
+    nm, nt = d.num_movable_nodes, d.num_terminals
+    n = d.num_nodes
+    assert np.all(got[nm:nm + nt] == 0) and np.all(got[n + nm:n + nm + nt] == 0)
