@@ -56,6 +56,10 @@ _SIGNATURES = {
     "dp_wa_merged_forward": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_merged_backward": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_wa_net_weight_grad": (_I, [_I, _P, _P, _P, _L, _P, _P]),
+    "dp_wa_forward_from_nodes": (_I, [_I, _P, _L, _P, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P,
+                                      _P, _P, _P, _P]),
+    "dp_wa_backward_to_nodes": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P]),
+    "dp_profile_kernel": (_I, [_I, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),

@@ -23,19 +23,23 @@ namespace dp {
 // are pushed to a queue that one workgroup per cell drains, so a macro's
 // thousands of bins do not serialise one lane of a wavefront.
 // ---------------------------------------------------------------------------
+// One launch covers both index ranges: t in [0, n_mov) are movables and
+// t in [n_mov, n_mov + n_fill) map to the fillers at fill_begin + (t - n_mov).
 template <typename T>
-__global__ void __launch_bounds__(256) k_tri_scatter(TriArgs<T> a, int64_t begin, int64_t end,
+__global__ void __launch_bounds__(256) k_tri_scatter(TriArgs<T> a, int64_t n_mov,
+                                                     int64_t fill_begin, int64_t total,
                                                      unsigned long long* map, int* qcount,
                                                      int* qlist, int qcap) {
-  int64_t i = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int64_t i = t < n_mov ? t : fill_begin + (t - n_mov);
   CellBins<T> c = cell_bins(a, i);
   int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
   if (nx <= 0 || ny <= 0) return;
   if (nx > 3 || ny > 3) {
     int slot = atomicAdd(qcount, 1);
     if (slot < qcap) {
-      qlist[slot] = (int)(i - begin);
+      qlist[slot] = (int)i;
       return;
     }
     // queue full: do the cell here (slow path, same arithmetic)
@@ -72,14 +76,13 @@ __global__ void __launch_bounds__(256) k_tri_scatter(TriArgs<T> a, int64_t begin
 
 // One workgroup per queued (large) cell; lanes stride over its bins.
 template <typename T>
-__global__ void __launch_bounds__(256) k_tri_scatter_large(TriArgs<T> a, int64_t begin,
-                                                           unsigned long long* map,
+__global__ void __launch_bounds__(256) k_tri_scatter_large(TriArgs<T> a, unsigned long long* map,
                                                            const int* qcount, const int* qlist,
                                                            int qcap) {
   int cnt = *qcount;
   if (cnt > qcap) cnt = qcap;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
-    int64_t i = begin + qlist[q];
+    int64_t i = qlist[q];
     CellBins<T> c = cell_bins(a, i);
     int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
     int total = nx * ny;
@@ -260,14 +263,17 @@ template TriArgs<double> make_tri_args<double>(const void*, int64_t, const void*
                                                double, double, double, int, int, int64_t);
 
 template <typename T>
-static int scatter_global(const TriArgs<T>& a, int64_t b, int64_t e, DensityWorkspace& w,
-                          int64_t qcap, hipStream_t s) {
-  if (e <= b) return DP_OK;
+static int scatter_global(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
+                          DensityWorkspace& w, int64_t qcap, hipStream_t s) {
+  const int64_t total = n_mov + n_fill;
+  if (total <= 0) return DP_OK;
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, w.qcount, (int64_t)1, 0);
-  hipLaunchKernelGGL(k_tri_scatter<T>, dim3((unsigned)ceil_div(e - b, 256)), dim3(256), 0, s, a,
-                     b, e, w.map, w.qcount, w.qlist, (int)qcap);
+  DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
+            hipLaunchKernelGGL(k_tri_scatter<T>, dim3((unsigned)ceil_div(total, 256)), dim3(256),
+                               0, s, a, n_mov, fill_begin, total, w.map, w.qcount, w.qlist,
+                               (int)qcap));
   DP_LAUNCHED("k_tri_scatter");
-  hipLaunchKernelGGL(k_tri_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, b, w.map,
+  hipLaunchKernelGGL(k_tri_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, w.map,
                      (const int*)w.qcount, (const int*)w.qlist, (int)qcap);
   DP_LAUNCHED("k_tri_scatter_large");
   return DP_OK;
@@ -339,16 +345,19 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
       DP_CHECK_RC(scatter_tiled<T>(a, ranges, density_tiled_region(workspace, nb, large_capacity),
                                    w.map, w.qcount, w.qlist, large_capacity, s));
     } else {
-      for (int r = 0; r < 2; ++r)
-        DP_CHECK_RC(scatter_global<T>(a, ranges[r][0], ranges[r][1], w, large_capacity, s));
+      DP_CHECK_RC(scatter_global<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
+                                    num_filler_nodes, w, large_capacity, s));
     }
     if (convention == DP_CONVERT_CPU) {
-      hipLaunchKernelGGL(k_convert_cpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map, init,
-                         nb, 1.0 / (double)scale, (T*)density_map_out, (long long*)fixed_map_out);
+      DP_PROBED(DP_KERNEL_DENSITY_CONVERT, s,
+                hipLaunchKernelGGL(k_convert_cpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
+                                   w.map, init, nb, 1.0 / (double)scale, (T*)density_map_out,
+                                   (long long*)fixed_map_out));
     } else {
-      hipLaunchKernelGGL(k_convert_gpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map, init,
-                         nb, (T)scale, (T)(1.0 / (double)scale), (T*)density_map_out,
-                         (long long*)fixed_map_out);
+      DP_PROBED(DP_KERNEL_DENSITY_CONVERT, s,
+                hipLaunchKernelGGL(k_convert_gpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
+                                   w.map, init, nb, (T)scale, (T)(1.0 / (double)scale),
+                                   (T*)density_map_out, (long long*)fixed_map_out));
     }
     DP_LAUNCHED("density convert");
     return DP_OK;

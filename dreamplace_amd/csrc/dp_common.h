@@ -15,6 +15,21 @@ namespace dp {
 // Per-thread message of the last failure (dp_last_error).
 void set_error(const char* fmt, ...);
 
+// Kernel probe state (dp_profile_kernel).  DP_PROBE(id, s, launch) records the
+// probe's start event, launches, then records the stop event, all on `s`.
+struct Probe {
+  int kernel_id = -1;
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern Probe g_probe;
+#define DP_PROBED(id, s, ...)                                                     \
+  do {                                                                            \
+    const bool probe_ = ::dp::g_probe.kernel_id == (id);                          \
+    if (probe_) (void)hipEventRecord(::dp::g_probe.start, (s));                   \
+    __VA_ARGS__;                                                                  \
+    if (probe_) (void)hipEventRecord(::dp::g_probe.stop, (s));                    \
+  } while (0)
+
 // Status helpers.  Every entry point returns through these.
 #define DP_REQUIRE(cond, code, ...)        \
   do {                                     \

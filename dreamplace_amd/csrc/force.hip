@@ -146,10 +146,12 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
                                     num_bins_x, num_bins_y, 1);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, qcount, (int64_t)1, 0);
-    hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0,
-                       s, a, (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,
-                       num_nodes, num_movable_nodes, num_filler_nodes, (T*)grad_out, qcount,
-                       qlist, (int)large_capacity);
+    DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
+              hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
+                                 dim3(256), 0, s, a, (const T*)field_map_x,
+                                 (const T*)field_map_y, (const T*)grad_scale, num_nodes,
+                                 num_movable_nodes, num_filler_nodes, (T*)grad_out, qcount, qlist,
+                                 (int)large_capacity));
     DP_LAUNCHED("k_force_small");
     hipLaunchKernelGGL(k_force_large<T>, dim3(256), dim3(256), 0, s, a, (const T*)field_map_x,
                        (const T*)field_map_y, (const T*)grad_scale, num_nodes, (T*)grad_out,

@@ -7,6 +7,7 @@
 namespace dp {
 
 static thread_local char g_err[512] = "";
+Probe g_probe;
 
 void set_error(const char* fmt, ...) {
   va_list ap;
@@ -57,5 +58,20 @@ extern "C" {
 const char* dp_last_error(void) { return dp::g_err; }
 
 int dp_version(void) { return 1; }
+
+int dp_profile_kernel(int kernel_id, void* start_event, void* stop_event) {
+  if (kernel_id >= DP_KERNEL_COUNT) {
+    dp::set_error("dp_profile_kernel: unknown kernel id %d", kernel_id);
+    return DP_ERR_INVALID_ARGUMENT;
+  }
+  if (kernel_id >= 0 && (!start_event || !stop_event)) {
+    dp::set_error("dp_profile_kernel: null event");
+    return DP_ERR_INVALID_ARGUMENT;
+  }
+  dp::g_probe.kernel_id = kernel_id < 0 ? -1 : kernel_id;
+  dp::g_probe.start = (hipEvent_t)start_event;
+  dp::g_probe.stop = (hipEvent_t)stop_event;
+  return DP_OK;
+}
 
 }  // extern "C"

@@ -176,6 +176,119 @@ __global__ void __launch_bounds__(256) k_hpwl(const T* __restrict__ px, const T*
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
+// Fused pin_pos + merged WA forward: the same per-net arithmetic as k_wa_fwd,
+// with each pin position formed on the fly as offset[p] + pos[pin2node[p]].
+template <typename T>
+__global__ void __launch_bounds__(256) k_wa_fwd_nodes(
+    const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ offx,
+    const T* __restrict__ offy, const int* __restrict__ pin2node,
+    const int* __restrict__ flat_netpin, const int* __restrict__ netpin_start, int64_t E,
+    const T* __restrict__ w, const uint8_t* __restrict__ mask, const T* __restrict__ inv_gamma_p,
+    T* partial, T* gix, T* giy, double* partials) {
+  __shared__ double smem[4];
+  double contrib = 0;
+  const T ig = *inv_gamma_p;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (!mask[e]) {
+      if (partial) partial[e] = 0;
+      continue;
+    }
+    const int b = netpin_start[e], en = netpin_start[e + 1];
+    T xmax = -tmax_value<T>(), xmin = tmax_value<T>();
+    T ymax = -tmax_value<T>(), ymin = tmax_value<T>();
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      int nd = pin2node[p];
+      T xx = offx[p] + x[nd], yy = offy[p] + y[nd];
+      xmax = xx > xmax ? xx : xmax;
+      xmin = xx < xmin ? xx : xmin;
+      ymax = yy > ymax ? yy : ymax;
+      ymin = yy < ymin ? yy : ymin;
+    }
+    T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      int nd = pin2node[p];
+      T xx = offx[p] + x[nd], yy = offy[p] + y[nd];
+      T ex = exp((xx - xmax) * ig), enx = exp((xmin - xx) * ig);
+      xs += xx * ex;
+      xns += xx * enx;
+      es += ex;
+      ens += enx;
+      T ey = exp((yy - ymax) * ig), eny = exp((ymin - yy) * ig);
+      ys += yy * ey;
+      yns += yy * eny;
+      fs += ey;
+      fns += eny;
+    }
+    T wl = xs / es - xns / ens + ys / fs - yns / fns;
+    if (partial) partial[e] = wl;
+    contrib += (double)(w ? wl * w[e] : wl);
+    T bx = ig / es, ax = (T)((1.0 - bx * xs) / es);
+    T bnx = -ig / ens, anx = (T)((1.0 - bnx * xns) / ens);
+    T by = ig / fs, ay = (T)((1.0 - by * ys) / fs);
+    T bny = -ig / fns, any = (T)((1.0 - bny * yns) / fns);
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      int nd = pin2node[p];
+      T xx = offx[p] + x[nd], yy = offy[p] + y[nd];
+      T ex = exp((xx - xmax) * ig), enx = exp((xmin - xx) * ig);
+      gix[p] = (ax + bx * xx) * ex - (anx + bnx * xx) * enx;
+      T ey = exp((yy - ymax) * ig), eny = exp((ymin - yy) * ig);
+      giy[p] = (ay + by * yy) * ey - (any + bny * yy) * eny;
+    }
+  }
+  double bs = block_sum<256>(contrib, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// Fused WA backward + pin_pos backward + gradient combine, node-parallel:
+// g[n] = sum over the CSR pins p of n of (pin_mask[p] ? 0 : (gi[p]*grad)*w[net(p)])
+//        + add[n]
+// Per pin this is the arithmetic of k_wa_bwd, and the pins are summed in CSR
+// order as k_pin_pos_bwd does, so the result equals the unfused sequence.
+template <typename T>
+__global__ void k_wa_pin_pos_bwd(const T* __restrict__ gix, const T* __restrict__ giy,
+                                 const T* __restrict__ grad, const int* __restrict__ flat_node2pin,
+                                 const int* __restrict__ node2pin_start,
+                                 const int* __restrict__ pin2net, const T* __restrict__ w,
+                                 const uint8_t* __restrict__ net_mask,
+                                 const uint8_t* __restrict__ pin_mask, const T* __restrict__ add,
+                                 int64_t nphys, int64_t nnodes, T* gx, T* gy) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const T g = *grad;
+  T sx = 0, sy = 0;
+  if (n < nphys) {
+    for (int j = node2pin_start[n]; j < node2pin_start[n + 1]; ++j) {
+      int p = flat_node2pin[j];
+      if (pin_mask && pin_mask[p]) {
+        sx += T(0);
+        sy += T(0);
+        continue;
+      }
+      T px = gix[p] * g, py = giy[p] * g;
+      if (w) {
+        int e = pin2net[p];
+        if (net_mask[e]) {
+          T wn = w[e];
+          px *= wn;
+          py *= wn;
+        }
+      }
+      sx += px;
+      sy += py;
+    }
+  }
+  if (add) {
+    sx += add[n];
+    sy += add[nnodes + n];
+  }
+  gx[n] = sx;
+  gy[n] = sy;
+}
+
 static int reduce_grid(int64_t n) {
   int64_t g = ceil_div(n, 256);
   if (g < 1) g = 1;
@@ -202,13 +315,16 @@ int dp_pin_pos_forward(int dtype, const void* pos, int64_t num_nodes, const void
     T* out = (T*)pin_pos_out;
     dim3 g((unsigned)ceil_div(num_pins, 256)), b(256);
     if (index_type == DP_INDEX_INT32)
-      hipLaunchKernelGGL((k_pin_pos_fwd<T, int>), g, b, 0, s, P, P + num_nodes,
-                         (const T*)pin_offset_x, (const T*)pin_offset_y,
-                         (const int*)pin2node_map, num_pins, out, out + num_pins);
+      DP_PROBED(DP_KERNEL_PIN_POS_FORWARD, s,
+                hipLaunchKernelGGL((k_pin_pos_fwd<T, int>), g, b, 0, s, P, P + num_nodes,
+                                   (const T*)pin_offset_x, (const T*)pin_offset_y,
+                                   (const int*)pin2node_map, num_pins, out, out + num_pins));
     else
-      hipLaunchKernelGGL((k_pin_pos_fwd<T, long long>), g, b, 0, s, P, P + num_nodes,
-                         (const T*)pin_offset_x, (const T*)pin_offset_y,
-                         (const long long*)pin2node_map, num_pins, out, out + num_pins);
+      DP_PROBED(DP_KERNEL_PIN_POS_FORWARD, s,
+                hipLaunchKernelGGL((k_pin_pos_fwd<T, long long>), g, b, 0, s, P, P + num_nodes,
+                                   (const T*)pin_offset_x, (const T*)pin_offset_y,
+                                   (const long long*)pin2node_map, num_pins, out,
+                                   out + num_pins));
     DP_LAUNCHED("k_pin_pos_fwd");
     return DP_OK;
   });
@@ -226,10 +342,11 @@ int dp_pin_pos_backward(int dtype, const void* grad_pin_pos, int64_t num_pins,
   return DP_DISPATCH(dtype, "dp_pin_pos_backward", [&] {
     const T* g = (const T*)grad_pin_pos;
     T* out = (T*)grad_pos_out;
-    hipLaunchKernelGGL(k_pin_pos_bwd<T>, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0,
-                       s, g, g + num_pins, (const int*)flat_node2pin_map,
-                       (const int*)flat_node2pin_start_map, num_physical_nodes, num_nodes, out,
-                       out + num_nodes);
+    DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
+              hipLaunchKernelGGL(k_pin_pos_bwd<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
+                                 dim3(256), 0, s, g, g + num_pins, (const int*)flat_node2pin_map,
+                                 (const int*)flat_node2pin_start_map, num_physical_nodes,
+                                 num_nodes, out, out + num_nodes));
     DP_LAUNCHED("k_pin_pos_bwd");
     return DP_OK;
   });
@@ -254,10 +371,11 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
   return DP_DISPATCH(dtype, "dp_wa_merged_forward", [&] {
     const T* pp = (const T*)pin_pos;
     T* gi = (T*)grad_intermediate;
-    hipLaunchKernelGGL(k_wa_fwd<T>, dim3(grid), dim3(256), 0, s, pp, pp + num_pins,
-                       (const int*)flat_netpin, (const int*)netpin_start, num_nets,
-                       (const T*)net_weights, net_mask, (const T*)inv_gamma, (T*)partial_wl, gi,
-                       gi + num_pins, partials);
+    DP_PROBED(DP_KERNEL_WA_FORWARD, s,
+              hipLaunchKernelGGL(k_wa_fwd<T>, dim3(grid), dim3(256), 0, s, pp, pp + num_pins,
+                                 (const int*)flat_netpin, (const int*)netpin_start, num_nets,
+                                 (const T*)net_weights, net_mask, (const T*)inv_gamma,
+                                 (T*)partial_wl, gi, gi + num_pins, partials));
     DP_LAUNCHED("k_wa_fwd");
     return finish_reduction(partials, grid, 1, nullptr, wl_out, dtype, s);
   });
@@ -272,9 +390,11 @@ int dp_wa_merged_backward(int dtype, const void* grad, void* grad_intermediate, 
   if (num_pins == 0) return DP_OK;
   hipStream_t s = S(stream);
   return DP_DISPATCH(dtype, "dp_wa_merged_backward", [&] {
-    hipLaunchKernelGGL(k_wa_bwd<T>, dim3((unsigned)ceil_div(num_pins, 256)), dim3(256), 0, s,
-                       (const T*)grad, (T*)grad_intermediate, num_pins, (const int*)pin2net_map,
-                       (const T*)net_weights, net_mask, pin_mask);
+    DP_PROBED(DP_KERNEL_WA_BACKWARD, s,
+              hipLaunchKernelGGL(k_wa_bwd<T>, dim3((unsigned)ceil_div(num_pins, 256)), dim3(256),
+                                 0, s, (const T*)grad, (T*)grad_intermediate, num_pins,
+                                 (const int*)pin2net_map, (const T*)net_weights, net_mask,
+                                 pin_mask));
     DP_LAUNCHED("k_wa_bwd");
     return DP_OK;
   });
@@ -291,6 +411,63 @@ int dp_wa_net_weight_grad(int dtype, const void* grad, const void* partial_wl,
                        (const T*)grad, (const T*)partial_wl, net_mask, num_nets,
                        (T*)grad_net_weights);
     DP_LAUNCHED("k_wa_wgrad");
+    return DP_OK;
+  });
+}
+
+int dp_wa_forward_from_nodes(int dtype, const void* pos, int64_t num_nodes,
+                             const void* pin_offset_x, const void* pin_offset_y,
+                             const int32_t* pin2node_map, int64_t num_pins,
+                             const int32_t* flat_netpin, const int32_t* netpin_start,
+                             int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                             const void* inv_gamma, void* partial_wl, void* grad_intermediate,
+                             void* wl_out, void* workspace, void* stream) {
+  DP_ARG(pos && pin_offset_x && pin_offset_y && pin2node_map && flat_netpin && netpin_start &&
+             net_mask && inv_gamma && grad_intermediate && wl_out && workspace,
+         "dp_wa_forward_from_nodes: null pointer");
+  hipStream_t s = S(stream);
+  double* partials = (double*)workspace;
+  int grid = reduce_grid(num_nets);
+  return DP_DISPATCH(dtype, "dp_wa_forward_from_nodes", [&] {
+    const T* P = (const T*)pos;
+    T* gi = (T*)grad_intermediate;
+    DP_PROBED(DP_KERNEL_WA_FORWARD, s,
+              hipLaunchKernelGGL(k_wa_fwd_nodes<T>, dim3(grid), dim3(256), 0, s, P, P + num_nodes,
+                                 (const T*)pin_offset_x, (const T*)pin_offset_y,
+                                 (const int*)pin2node_map, (const int*)flat_netpin,
+                                 (const int*)netpin_start, num_nets, (const T*)net_weights,
+                                 net_mask, (const T*)inv_gamma, (T*)partial_wl, gi, gi + num_pins,
+                                 partials));
+    DP_LAUNCHED("k_wa_fwd_nodes");
+    return finish_reduction(partials, grid, 1, nullptr, wl_out, dtype, s);
+  });
+}
+
+int dp_wa_backward_to_nodes(int dtype, const void* grad, const void* grad_intermediate,
+                            int64_t num_pins, const int32_t* flat_node2pin_map,
+                            const int32_t* flat_node2pin_start_map, const int32_t* pin2net_map,
+                            const void* net_weights, const uint8_t* net_mask,
+                            const uint8_t* pin_mask, const void* add, int64_t num_physical_nodes,
+                            int64_t num_nodes, void* grad_pos_out, void* stream) {
+  DP_ARG(grad && grad_intermediate && flat_node2pin_map && flat_node2pin_start_map &&
+             grad_pos_out,
+         "dp_wa_backward_to_nodes: null pointer");
+  DP_ARG(!net_weights || (pin2net_map && net_mask),
+         "dp_wa_backward_to_nodes: net weights need pin2net_map and net_mask");
+  DP_ARG(num_physical_nodes <= num_nodes, "dp_wa_backward_to_nodes: bad node counts");
+  if (num_nodes == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  return DP_DISPATCH(dtype, "dp_wa_backward_to_nodes", [&] {
+    const T* gi = (const T*)grad_intermediate;
+    T* out = (T*)grad_pos_out;
+    DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
+              hipLaunchKernelGGL(k_wa_pin_pos_bwd<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
+                                 dim3(256), 0, s, gi, gi + num_pins, (const T*)grad,
+                                 (const int*)flat_node2pin_map,
+                                 (const int*)flat_node2pin_start_map, (const int*)pin2net_map,
+                                 (const T*)net_weights, net_mask, pin_mask, (const T*)add,
+                                 num_physical_nodes, num_nodes, out, out + num_nodes));
+    DP_LAUNCHED("k_wa_pin_pos_bwd");
     return DP_OK;
   });
 }

@@ -52,6 +52,24 @@ enum { DP_SCATTER_GLOBAL = 0, DP_SCATTER_TILED = 1 };
 const char* dp_last_error(void);
 int dp_version(void);
 
+/* Kernel probe (no counterpart in the reference): when enabled, the library
+ * records `start_event` immediately before and `stop_event` immediately after
+ * every launch of the named kernel, on the stream it is launched on (a launch
+ * overwrites the previous pair).  kernel_id < 0 disables.  Events are
+ * hipEvent_t passed as void*.  Used by the benchmark to time one kernel live. */
+enum {
+  DP_KERNEL_DENSITY_SCATTER = 0,
+  DP_KERNEL_ELECTRIC_FORCE = 1,
+  DP_KERNEL_WA_FORWARD = 2,
+  DP_KERNEL_WA_BACKWARD = 3,
+  DP_KERNEL_PIN_POS_FORWARD = 4,
+  DP_KERNEL_PIN_POS_BACKWARD = 5,
+  DP_KERNEL_SPECTRAL_GEMM = 6,
+  DP_KERNEL_DENSITY_CONVERT = 7,
+  DP_KERNEL_COUNT = 8
+};
+int dp_profile_kernel(int kernel_id, void* start_event, void* stop_event);
+
 /* ---------------------------------------------------------------------------
  * Density map (fixed-point int64 accumulation; always deterministic).
  * ------------------------------------------------------------------------- */
@@ -202,6 +220,31 @@ int dp_wa_merged_backward(int dtype, const void* grad, void* grad_intermediate, 
 int dp_wa_net_weight_grad(int dtype, const void* grad, const void* partial_wl,
                           const uint8_t* net_mask, int64_t num_nets, void* grad_net_weights,
                           void* stream);
+
+/* Fused forward of the wirelength op chain WA(PinPos(pos))
+ * (dreamplace/PlaceObj.py:478-479): dp_pin_pos_forward followed by
+ * dp_wa_merged_forward, with each pin position formed on the fly from
+ * pos[pin2node[p]] + offset[p] instead of being written to memory.  Outputs as
+ * for dp_wa_merged_forward. */
+int dp_wa_forward_from_nodes(int dtype, const void* pos, int64_t num_nodes,
+                             const void* pin_offset_x, const void* pin_offset_y,
+                             const int32_t* pin2node_map, int64_t num_pins,
+                             const int32_t* flat_netpin, const int32_t* netpin_start,
+                             int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                             const void* inv_gamma, void* partial_wl, void* grad_intermediate,
+                             void* wl_out, void* workspace, void* stream);
+
+/* Fused backward of that chain plus the gradient combine of
+ * PlaceObj.obj_and_grad_fn (dreamplace/PlaceObj.py:394-411): per pin the
+ * arithmetic of dp_wa_merged_backward (not in place), summed over each
+ * physical node's CSR pins as dp_pin_pos_backward does, plus add[2N]
+ * (nullable; e.g. the density gradient).  grad_intermediate is read only. */
+int dp_wa_backward_to_nodes(int dtype, const void* grad, const void* grad_intermediate,
+                            int64_t num_pins, const int32_t* flat_node2pin_map,
+                            const int32_t* flat_node2pin_start_map, const int32_t* pin2net_map,
+                            const void* net_weights, const uint8_t* net_mask,
+                            const uint8_t* pin_mask, const void* add, int64_t num_physical_nodes,
+                            int64_t num_nodes, void* grad_pos_out, void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
  * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64

@@ -486,6 +486,35 @@ def pin_weight_sum(net_weights, flat_nodepin, nodepin_start, pin2net_map, num_no
     return out
 
 
+def objective_and_gradient(d, density_weight, gamma):
+    """obj = WA(pin_pos(pos)) + lambda*energy and its gradient for a design
+    dict (dreamplace_amd.synth layout), composed from the functions above as
+    PlaceObj.obj_fn / obj_and_grad_fn compose the ops
+    (dreamplace/PlaceObj.py:293-320,394-411), FP64 spectral solve.
+    Returns (obj, grad[2N], parts)."""
+    dp = DensityParams(d["node_size_x"], d["node_size_y"], d["xl"], d["yl"], d["xh"], d["yh"],
+                       d["bin_size_x"], d["bin_size_y"], d["num_movable_nodes"], d["num_terminals"],
+                       d["num_filler_nodes"], d["target_density"], d["movable_macro_mask"])
+    pos = d["pos"]
+    T = pos.dtype
+    pin_pos = pin_pos_forward(pos, d["pin_offset_x"], d["pin_offset_y"], d["pin2node_map"])
+    wl, partial, gi = wa_merged_forward(pin_pos, d["flat_net2pin_map"], d["flat_net2pin_start_map"],
+                                        d["net_weights"], d["net_mask_ignore_large_degrees"], gamma)
+    init = fixed_density_map(dp, pos, d["bin_center_x"], d["bin_center_y"])
+    dmap = density_map(dp, pos, init)
+    sol = poisson_solve(dmap, d["bin_size_x"], d["bin_size_y"])
+    dgrad = electric_force(dp, pos, sol["field_x"].astype(T), sol["field_y"].astype(T),
+                           grad=density_weight)
+    gpin = wa_merged_backward(1.0, gi, d["pin2net_map"], d["net_weights"],
+                              d["net_mask_ignore_large_degrees"], d["pin_mask_ignore_fixed_macros"])
+    wgrad = pin_pos_backward(gpin, d["flat_node2pin_map"], d["flat_node2pin_start_map"],
+                             d["num_physical_nodes"], d["num_nodes"])
+    obj = wl + density_weight * sol["energy"]
+    parts = dict(sol)
+    parts.update(wl=wl, density_map=dmap, density_grad=dgrad, wl_grad=wgrad)
+    return obj, wgrad + dgrad, parts
+
+
 def overflow(density_map_movable, target_density, bin_size_x, bin_size_y, total_movable_area):
     """ElectricOverflow.forward metric (electric_overflow.py:260-278,
     EvalMetrics.py:116-128): sum(max(D - t*A_bin, 0)) / movable area, and

@@ -1,0 +1,162 @@
+"""One density + WA forward/backward iteration through the reference's own CPU
+ops (the pybind modules built into oracle/_ref by oracle/build_ref.py).
+
+TEST INFRASTRUCTURE ONLY: the CPU-baseline leg of bench.py and the tests use
+it; the product path never does.
+
+The glue restates, in the same op order and with the same arguments, the
+Python wrappers that drive those ops:
+  PinPosFunction.forward/backward      dreamplace/ops/pin_pos/pin_pos.py:27-56
+  WeightedAverageWirelengthMerged...   weighted_average_wirelength.py:165-213
+  ElectricPotentialFunction.forward    electric_potential.py:52-217
+  ElectricPotentialFunction.backward   electric_potential.py:219-245
+  ElectricOverflow.reset / fixed map   electric_overflow.py:162-258
+  get_exact_expk                       dreamplace/ops/dct/discrete_spectral_transform.py:63-69
+  PlaceObj.obj_fn / obj_and_grad_fn    dreamplace/PlaceObj.py:293-320,394-411
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+MODULES = ("electric_potential_cpp", "dct2_fft2_cpp", "weighted_average_wirelength_cpp_merged",
+           "pin_pos_cpp", "hpwl_cpp", "move_boundary_cpp", "pws_cpp")
+
+
+def available():
+    return os.path.isdir(REF_DIR) and all(
+        any(f.startswith(m + ".") and f.endswith(".so") for f in os.listdir(REF_DIR)) for m in MODULES)
+
+
+def load():
+    if REF_DIR not in sys.path:
+        sys.path.insert(0, REF_DIR)
+    import importlib
+    return {m: importlib.import_module(m) for m in MODULES}
+
+
+def exact_expk(N, dtype):
+    """exp(-j*pi*k/(2N)) as [N, 2] (cos, -sin)."""
+    k = torch.arange(N, dtype=dtype) * (np.pi / (2 * N))
+    return torch.stack([k.cos(), -k.sin()], dim=-1).contiguous()
+
+
+class RefIteration:
+    """Holds the CPU tensors of a design and runs reference iterations."""
+
+    def __init__(self, d, dtype=torch.float32, num_threads=None, density_weight=1.0):
+        self.m = load()
+        if num_threads:
+            torch.set_num_threads(num_threads)
+        self.nt = torch.get_num_threads()
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype)
+        I = lambda a: torch.from_numpy(np.ascontiguousarray(a).astype(np.int32))
+        self.d = d
+        self.dtype = dtype
+        self.pos = T(d["pos"])
+        self.node_size_x, self.node_size_y = T(d["node_size_x"]), T(d["node_size_y"])
+        self.pin_offset_x, self.pin_offset_y = T(d["pin_offset_x"]), T(d["pin_offset_y"])
+        self.pin2node = torch.from_numpy(d["pin2node_map"].astype(np.int64))
+        self.flat_node2pin, self.node2pin_start = I(d["flat_node2pin_map"]), I(d["flat_node2pin_start_map"])
+        self.flat_netpin, self.netpin_start = I(d["flat_net2pin_map"]), I(d["flat_net2pin_start_map"])
+        self.pin2net = I(d["pin2net_map"])
+        self.net_weights = T(d["net_weights"])
+        self.net_mask = torch.from_numpy(d["net_mask_ignore_large_degrees"].astype(np.uint8))
+        self.pin_mask = torch.from_numpy(d["pin_mask_ignore_fixed_macros"].astype(bool))
+        self.inv_gamma = torch.tensor(1.0 / d["wa_gamma"], dtype=dtype)
+        self.bcx, self.bcy = T(d["bin_center_x"]), T(d["bin_center_y"])
+        self.lam = float(density_weight)
+        # ElectricOverflow.reset (electric_overflow.py:162-227)
+        bx, by = d["bin_size_x"], d["bin_size_y"]
+        sqrt2 = math.sqrt(2)
+        self.wxc = self.node_size_x.clamp(min=bx * sqrt2)
+        self.offx = (self.node_size_x - self.wxc).mul(0.5)
+        self.wyc = self.node_size_y.clamp(min=by * sqrt2)
+        self.offy = (self.node_size_y - self.wyc).mul(0.5)
+        self.ratio = (self.node_size_x * self.node_size_y) / (self.wxc * self.wyc)
+        t = d["target_density"]
+        nm = d["num_movable_nodes"]
+        if t < 1:
+            mm = torch.from_numpy(d["movable_macro_mask"].astype(bool))
+            self.ratio[:nm][mm] = t
+        self.M, self.N = d["num_bins_x"], d["num_bins_y"]
+        nt = d["num_terminals"]
+        ep = self.m["electric_potential_cpp"]
+        if nt:
+            msx = float(self.node_size_x[nm:nm + nt].max())
+            msy = float(self.node_size_y[nm:nm + nt].max())
+            ibx = int(min(math.ceil((msx + bx) / bx), self.M))
+            iby = int(min(math.ceil((msy + by) / by), self.N))
+        else:
+            ibx = iby = 0
+        self.initial = ep.fixed_density_map(self.pos, self.node_size_x, self.node_size_y, self.bcx, self.bcy,
+                                            d["xl"], d["yl"], d["xh"], d["yh"], bx, by, nm, nt, self.M, self.N,
+                                            ibx, iby, 1)
+        self.initial.mul_(t)
+        self.imp = (3, 3, 3, 3)
+        self.expkM, self.expkN = exact_expk(self.M, dtype), exact_expk(self.N, dtype)
+        M, N = self.M, self.N
+        wu = torch.arange(M, dtype=dtype).mul(2 * np.pi / M).view([M, 1])
+        wv = torch.arange(N, dtype=dtype).mul(2 * np.pi / N).view([1, N]).mul_(bx / by)
+        w2 = wu.pow(2) + wv.pow(2)
+        w2[0, 0] = 1.0
+        self.inv_w2 = 1.0 / w2
+        self.inv_w2[0, 0] = 0.0
+        self.wu_h = wu.mul(self.inv_w2).mul_(1. / 2)
+        self.wv_h = wv.mul(self.inv_w2).mul_(1. / 2)
+        self.bufs = {k: (torch.empty(M, N, dtype=dtype), torch.empty(M, N // 2 + 1, 2, dtype=dtype))
+                     for k in ("dct2", "idct2", "idct_idxst", "idxst_idct")}
+
+    def _dct(self, name, x):
+        out, buf = self.bufs[name]
+        fn = {"dct2": "dct2_fft2", "idct2": "idct2_fft2", "idct_idxst": "idct_idxst",
+              "idxst_idct": "idxst_idct"}[name]
+        getattr(self.m["dct2_fft2_cpp"], fn)(x.contiguous(), self.expkM, self.expkN, out, buf, self.nt)
+        return out.clone()
+
+    def iterate(self, pos=None):
+        """(obj, grad[2N], details) of one obj_and_grad_fn without preconditioning."""
+        d = self.d
+        pos = self.pos if pos is None else pos
+        n = pos.numel() // 2
+        m = self.m
+        # wirelength forward
+        pin_pos = m["pin_pos_cpp"].forward(pos, self.pin_offset_x, self.pin_offset_y, self.pin2node,
+                                            self.flat_node2pin, self.node2pin_start)
+        wl, gi = m["weighted_average_wirelength_cpp_merged"].forward(
+            pin_pos, self.flat_netpin, self.netpin_start, self.pin2net, self.net_weights, self.net_mask,
+            self.inv_gamma)
+        # density forward
+        bx, by = d["bin_size_x"], d["bin_size_y"]
+        dmap = m["electric_potential_cpp"].density_map(
+            pos, self.wxc, self.wyc, self.offx, self.offy, self.ratio, self.bcx, self.bcy, self.initial,
+            d["target_density"], d["xl"], d["yl"], d["xh"], d["yh"], bx, by, d["num_movable_nodes"],
+            d["num_filler_nodes"], 0, self.M, self.N, *self.imp, 1).view(self.M, self.N)
+        dmap.mul_(1.0 / (bx * by))
+        auv = self._dct("dct2", dmap)
+        fx = self._dct("idxst_idct", auv.mul(self.wu_h))
+        fy = self._dct("idct_idxst", auv.mul(self.wv_h))
+        phi = self._dct("idct2", auv.mul(self.inv_w2))
+        energy = phi.mul(dmap).sum()
+        obj = wl + self.lam * energy
+        # backward: density force scaled by lambda, WA backward, pin_pos backward
+        g = torch.tensor([self.lam], dtype=self.dtype)
+        dgrad = -m["electric_potential_cpp"].electric_force(
+            g, self.M, self.N, *self.imp, fx.view(-1), fy.view(-1), pos, self.wxc, self.wyc, self.offx,
+            self.offy, self.ratio, self.bcx, self.bcy, d["xl"], d["yl"], d["xh"], d["yh"], bx, by,
+            d["num_movable_nodes"], d["num_filler_nodes"])
+        gpin = m["weighted_average_wirelength_cpp_merged"].backward(
+            torch.ones((), dtype=self.dtype), pin_pos, gi, self.flat_netpin, self.netpin_start,
+            self.pin2net, self.net_weights, self.net_mask, self.inv_gamma)
+        P = gpin.numel() // 2
+        gpin[:P].masked_fill_(self.pin_mask, 0.0)
+        gpin[P:].masked_fill_(self.pin_mask, 0.0)
+        wgrad = m["pin_pos_cpp"].backward(gpin, pos, self.pin_offset_x, self.pin_offset_y, self.pin2node,
+                                          self.flat_node2pin, self.node2pin_start, d["num_physical_nodes"])
+        grad = wgrad + dgrad
+        return obj, grad, dict(wl=wl, energy=energy, density_map=dmap, field_x=fx, field_y=fy, phi=phi,
+                               density_grad=dgrad, wl_grad=wgrad)

@@ -1,0 +1,76 @@
+"""GPU parity of one full objective+gradient evaluation (the bench step)
+against the oracle and against the reference's own CPU ops (oracle/_ref)."""
+import numpy as np
+import pytest
+import torch
+
+from _util import design, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _objective(d, dev, lam, fused, precision, dtype=None):
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    dd = DeviceDesign(d, dev, dtype=dtype)
+    return dd, PlacementObjective(dd, d.wa_gamma, density_weight=lam, precision=precision, fused=fused)
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
+@pytest.mark.parametrize("cfg,kw", [(0, {}), (4, {"scale": 0.003, "max_fillers": 100000})],
+                         ids=["config0", "config4-small"])
+def test_engine_fp64_matches_oracle(oracle, gpu, cfg, kw, fused):
+    print("case engine fp64 cfg%d fused=%s" % (cfg, fused), flush=True)
+    d = design(cfg, **kw)
+    lam = 0.25
+    dd, obj = _objective(d, gpu, lam, fused, "fp64")
+    f, g = obj.evaluate(dd.pos)
+    rf, rg, parts = oracle.objective_and_gradient(d, lam, d.wa_gamma)
+    assert abs(f.item() - rf) <= 1e-10 * abs(rf)
+    assert rel_l2(g.cpu().numpy(), rg) <= 1e-10
+    assert np.array_equal(obj.density_map.cpu().numpy(), parts["density_map"])
+
+
+def test_engine_fused_equals_unfused(gpu):
+    print("case engine fused == unfused", flush=True)
+    d = design(0)
+    dd1, o1 = _objective(d, gpu, 0.3, True, "fp64")
+    dd2, o2 = _objective(d, gpu, 0.3, False, "fp64")
+    f1, g1 = o1.evaluate(dd1.pos)
+    f2, g2 = o2.evaluate(dd2.pos)
+    assert torch.equal(g1, g2)
+    assert f1.item() == f2.item()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 4e-5)])
+def test_engine_fp32_forms(oracle, gpu, precision, tol):
+    """FP32 positions with an FP32 or BF16x3 spectral solve, against the FP64
+    oracle: relative L2 of the density gradient and of the total gradient."""
+    print("case engine fp32/%s" % precision, flush=True)
+    d = design(0, dtype=np.float32)
+    lam = 0.25
+    dd, obj = _objective(d, gpu, lam, True, precision)
+    f, g = obj.evaluate(dd.pos)
+    d64 = design(0)
+    rf, rg, parts = oracle.objective_and_gradient(d64, lam, d64.wa_gamma)
+    assert rel_l2(obj.density_grad.double().cpu().numpy(), parts["density_grad"]) <= tol
+    assert rel_l2(g.double().cpu().numpy(), rg) <= tol
+    assert abs(f.item() - rf) <= tol * abs(rf)
+
+
+def test_engine_matches_reference_cpu_ops(gpu):
+    """Against the reference's own CPU ops (oracle/_ref), config 0, FP64."""
+    from oracle import ref_iteration as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built")
+    print("case engine vs oracle/_ref", flush=True)
+    d = design(0)
+    lam = 0.25
+    dd, obj = _objective(d, gpu, lam, True, "fp64")
+    f, g = obj.evaluate(dd.pos)
+    it = R.RefIteration(d, dtype=torch.float64, density_weight=lam)
+    rf, rg, det = it.iterate()
+    # the reference scales the map in place by 1/(bx*by) before the DCT
+    assert np.array_equal(obj.density_map.cpu().numpy() * (1.0 / (d.bin_size_x * d.bin_size_y)),
+                          det["density_map"].numpy())
+    assert abs(f.item() - rf.item()) <= 1e-10 * abs(rf.item())
+    assert rel_l2(g.cpu().numpy(), rg.numpy()) <= 1e-10

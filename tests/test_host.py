@@ -1,0 +1,117 @@
+"""Host-side logic without a GPU: the synthetic designs follow the reference's
+formulas, the op modules prepare the reference's derived arrays, and every op
+refuses CPU tensors instead of falling back."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from dreamplace_amd import synth
+
+
+@pytest.mark.parametrize("cfg,kw", [(0, {}), (1, {"scale": 0.003}), (3, {"scale": 0.001}),
+                                    (4, {"scale": 0.003, "max_fillers": 20000})])
+def test_design_invariants(cfg, kw):
+    d = synth.make_design(cfg, **kw)
+    n, P, E = d.num_nodes, d.num_pins, d.num_nets
+    assert d.pos.shape == (2 * n,) and d.node_size_x.shape == (n,)
+    assert d.num_movable_nodes + d.num_terminals + d.num_filler_nodes == n
+    # CSR maps are consistent
+    assert d.flat_net2pin_start_map[-1] == P and np.array_equal(np.sort(d.flat_net2pin_map), np.arange(P))
+    for e in range(0, E, max(1, E // 50)):
+        pins = d.flat_net2pin_map[d.flat_net2pin_start_map[e]:d.flat_net2pin_start_map[e + 1]]
+        assert np.all(d.pin2net_map[pins] == e)
+        assert len(set(d.pin2node_map[pins])) == len(pins)  # distinct nodes per net
+    assert d.flat_node2pin_start_map[-1] == P
+    nodes = np.repeat(np.arange(d.num_physical_nodes), np.diff(d.flat_node2pin_start_map))
+    assert np.array_equal(d.pin2node_map[d.flat_node2pin_map], nodes)
+    # pins inside their nodes
+    assert np.all(d.pin_offset_x >= 0) and np.all(d.pin_offset_x <= d.node_size_x[d.pin2node_map])
+    # masks (BasicPlace.py:159-183)
+    deg = np.diff(d.flat_net2pin_start_map)
+    assert np.array_equal(d.net_mask_all, (deg >= 2).astype(np.uint8))
+    assert np.array_equal(d.net_mask_ignore_large_degrees, ((deg >= 2) & (deg < 100)).astype(np.uint8))
+    assert np.array_equal(d.pin_mask_ignore_fixed_macros, d.pin2node_map >= d.num_movable_nodes)
+    # fixed blocks do not overlap each other
+    m0, m1 = d.num_movable_nodes, d.num_movable_nodes + d.num_terminals
+    fx, fy = d.pos[m0:m1], d.pos[n + m0:n + m1]
+    fw, fh = d.node_size_x[m0:m1], d.node_size_y[m0:m1]
+    if cfg != 4 and m1 > m0:
+        k = min(300, m1 - m0)
+        ox = np.minimum(fx[:k, None] + fw[:k, None], fx[None, :k] + fw[None, :k]) - np.maximum(fx[:k, None], fx[None, :k])
+        oy = np.minimum(fy[:k, None] + fh[:k, None], fy[None, :k] + fh[None, :k]) - np.maximum(fy[:k, None], fy[None, :k])
+        over = (ox > 1e-9) & (oy > 1e-9)
+        np.fill_diagonal(over, False)
+        assert not over.any()
+
+
+def test_filler_formula():
+    """PlaceDB.py:917-936: filler width = mean of the 5%..95% width band,
+    count = round(max(space*t - movable cell area, 0) / filler area)."""
+    d = synth.make_design(0)
+    w = d.node_size_x[:d.num_movable_nodes]
+    order = np.argsort(w, kind="stable")
+    lb, ub = int(len(w) * 0.05), int(len(w) * 0.95)
+    fw = np.mean(w[order[lb:ub]])
+    assert d.filler_size_x == pytest.approx(fw)
+    space = (d.xh - d.xl) * (d.yh - d.yl) - d.total_fixed_node_area
+    expect = int(round(max(space * d.target_density - (w * d.node_size_y[:d.num_movable_nodes]).sum(), 0) / fw))
+    assert d.num_filler_nodes == expect
+
+
+def test_degree_distribution():
+    d = synth.make_design(0, seed=3)
+    deg = np.diff(d.flat_net2pin_start_map)
+    assert abs(np.mean(deg == 2) - 0.60) < 0.02
+    assert abs(np.mean(deg == 3) - 0.20) < 0.02
+    assert abs(np.mean((deg >= 4) & (deg <= 10)) - 0.15) < 0.02
+    assert abs(np.mean(deg >= 11) - 0.05) < 0.01
+    assert deg.max() <= 50
+
+
+def test_overflow_module_reset_matches_oracle(oracle):
+    """ElectricOverflow.reset derives the reference's stretched sizes, offsets
+    and ratios (electric_overflow.py:162-227) -- host tensors are enough."""
+    from dreamplace_amd.electric_overflow import ElectricOverflow
+    d = synth.make_design(4, scale=0.003, max_fillers=20000)
+    T = lambda a: torch.from_numpy(a)
+    op = ElectricOverflow(T(d.node_size_x), T(d.node_size_y), T(d.bin_center_x), T(d.bin_center_y),
+                          d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+                          d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+                          T(d.sorted_node_map), movable_macro_mask=T(d.movable_macro_mask))
+    dp = oracle.DensityParams(d.node_size_x, d.node_size_y, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
+                              d.bin_size_y, d.num_movable_nodes, d.num_terminals, d.num_filler_nodes,
+                              d.target_density, d.movable_macro_mask)
+    for a, b in [(op.node_size_x_clamped, dp.wxc), (op.offset_x, dp.offx), (op.ratio, dp.ratio)]:
+        assert np.array_equal(a.numpy(), b)
+    assert op.num_bins_x == d.num_bins_x
+    big = (dp.wxc > 1.9 * d.bin_size_x) | (dp.wyc > 1.9 * d.bin_size_y)
+    assert op.large_capacity == int(big.sum()) + 64
+    with pytest.raises(RuntimeError, match="GPU"):
+        op.compute_initial_density_map(T(d.pos))
+
+
+def test_ops_refuse_cpu_tensors():
+    from dreamplace_amd.hpwl import hpwl_fp64
+    from dreamplace_amd.pin_pos import PinPos
+    d = synth.make_design(0)
+    T = lambda a: torch.from_numpy(a)
+    pp = PinPos(T(d.pin_offset_x), T(d.pin_offset_y), T(d.pin2node_map), T(d.flat_node2pin_map),
+                T(d.flat_node2pin_start_map), d.num_physical_nodes)
+    with pytest.raises(RuntimeError, match="GPU"):
+        pp(T(d.pos))
+    with pytest.raises(RuntimeError, match="GPU"):
+        hpwl_fp64(T(d.pos), T(d.flat_net2pin_map), T(d.flat_net2pin_start_map), T(d.net_weights),
+                  T(d.net_mask_all))
+
+
+def test_fence_regions_not_supported():
+    from dreamplace_amd.electric_potential import ElectricPotential
+    d = synth.make_design(0)
+    T = lambda a: torch.from_numpy(a)
+    with pytest.raises(NotImplementedError):
+        ElectricPotential(T(d.node_size_x), T(d.node_size_y), T(d.bin_center_x), T(d.bin_center_y),
+                          1.0, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+                          d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+                          T(d.sorted_node_map), region_id=0)

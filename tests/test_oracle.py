@@ -1,0 +1,477 @@
+"""Pinning the oracle (CPU, no GPU):
+
+1. against the golden vectors of the reference's own unit tests (tests/golden);
+2. against the reference's own CPU ops compiled into oracle/_ref (skipped when
+   the reference sources were not available to build them);
+3. against what the mathematics fixes: brute-force overlaps, mass
+   conservation, order independence, closed-form single-mode solutions of the
+   Poisson solve, the DCT inversion identity, finite differences, the
+   gamma -> 0 limit of WA, adjoint identities.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from _util import density_params, design, rel_l2
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _gold():
+    with open(os.path.join(GOLD, "ops_golden.json")) as f:
+        return json.load(f)
+
+
+def _csr(groups, n_items):
+    flat = np.array([i for g in groups for i in g], dtype=np.int32)
+    start = np.zeros(len(groups) + 1, dtype=np.int32)
+    start[1:] = np.cumsum([len(g) for g in groups])
+    owner = np.zeros(n_items, dtype=np.int32)
+    for k, g in enumerate(groups):
+        owner[g] = k
+    return flat, start, owner
+
+
+# ---------------------------------------------------------------------------
+# 1. golden vectors of the reference's unit tests
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype,atol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_wa_golden(oracle, dtype, atol):
+    g = _gold()["weighted_average_wirelength"]
+    pin = np.array(g["pin_pos"], dtype=dtype)
+    flat, start, _ = _csr(g["net2pin"], len(pin))
+    mask = np.array([len(n) < g["ignore_net_degree"] for n in g["net2pin"]], dtype=np.uint8)
+    wl, _, _ = oracle.wa_merged_forward(np.concatenate([pin[:, 0], pin[:, 1]]), flat, start,
+                                        np.array(g["net_weights"], dtype=dtype), mask, g["gamma"])
+    assert abs(wl - g["expected"]) <= atol
+
+
+def test_hpwl_golden(oracle):
+    g = _gold()["hpwl"]
+    pin = np.array(g["pin_pos"])
+    flat, start, _ = _csr(g["net2pin"], len(pin))
+    v = oracle.hpwl(np.concatenate([pin[:, 0], pin[:, 1]]), flat, start, np.array(g["net_weights"]),
+                    np.ones(2, dtype=np.uint8))
+    assert v == pytest.approx(g["expected"], rel=1e-15)
+
+
+def test_pin_pos_golden(oracle):
+    g = _gold()["pin_pos"]
+    pos = np.array(g["pos"])
+    pos = np.concatenate([pos[:, 0], pos[:, 1]])
+    out = oracle.pin_pos_forward(pos, g["pin_offset_x"], g["pin_offset_y"], g["pin2node"])
+    assert np.allclose(out, g["expected_pin_pos"], rtol=0, atol=1e-15)
+    flat, start, _ = _csr(g["node2pin"], 5)
+    grad = oracle.pin_pos_backward(np.array(g["grad_pin_pos"]), flat, start, g["num_physical_nodes"], 3)
+    assert np.array_equal(grad, np.array(g["expected_grad_pos"]))
+
+
+# ---------------------------------------------------------------------------
+# 2. against oracle/_ref (the reference's own CPU ops)
+# ---------------------------------------------------------------------------
+def _ref_density_inputs(d, dp, dtype):
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=dtype))
+    return dict(pos=T(d.pos), wxc=T(dp.wxc), wyc=T(dp.wyc), offx=T(dp.offx), offy=T(dp.offy),
+                ratio=T(dp.ratio), bcx=T(d.bin_center_x), bcy=T(d.bin_center_y))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("cfg,kw", [(0, {}), (1, {"scale": 0.002}), (4, {"scale": 0.002, "max_fillers": 50000})],
+                         ids=["config0", "config1-small", "config4-small"])
+def test_density_maps_match_reference(oracle, ref, cfg, kw, dtype):
+    d = design(cfg, dtype=dtype, **kw)
+    dp = density_params(oracle, d)
+    t = _ref_density_inputs(d, dp, dtype)
+    ep = ref["electric_potential_cpp"]
+    nm, nt = d.num_movable_nodes, d.num_terminals
+    init_ref = ep.fixed_density_map(t["pos"], torch.from_numpy(d.node_size_x), torch.from_numpy(d.node_size_y),
+                                    t["bcx"], t["bcy"], d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+                                    nm, nt, d.num_bins_x, d.num_bins_y, 3, 3, 1)
+    init_ref.mul_(d.target_density)
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    assert np.array_equal(init, init_ref.numpy())
+    m_ref = ep.density_map(t["pos"], t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"], t["bcx"], t["bcy"],
+                           torch.from_numpy(init), d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
+                           d.bin_size_y, nm, d.num_filler_nodes, 0, d.num_bins_x, d.num_bins_y, 3, 3, 3, 3, 1)
+    m = oracle.density_map(dp, d.pos, init)
+    assert np.array_equal(m.reshape(-1), m_ref.numpy().reshape(-1))
+
+
+def test_density_84_node_fixture_matches_reference(oracle, ref):
+    """The reference unit test's 84-node fixture (1 movable, 83 fixed, 1024^2 bins)."""
+    f = np.load(os.path.join(GOLD, "electric_potential_84.npz"))
+    nm = int(f["num_movable_nodes"])
+    n = len(f["x"])
+    xl, yl, xh, yh = (float(f[k]) for k in ("xl", "yl", "xh", "yh"))
+    nb = int(f["num_bins_x"])
+    bsx, bsy = (xh - xl) / nb, (yh - yl) / nb
+    bcx = oracle.bin_centers(xl, xh, bsx)
+    bcy = oracle.bin_centers(yl, yh, bsy)
+    dp = oracle.DensityParams(f["node_size_x"], f["node_size_y"], xl, yl, xh, yh, bsx, bsy, nm, n - nm, 0,
+                              float(f["target_density"]))
+    pos = np.concatenate([f["x"], f["y"]])
+    init = oracle.fixed_density_map(dp, pos, bcx, bcy)
+    ep = ref["electric_potential_cpp"]
+    init_ref = ep.fixed_density_map(torch.from_numpy(pos), torch.from_numpy(f["node_size_x"]),
+                                    torch.from_numpy(f["node_size_y"]), torch.from_numpy(bcx),
+                                    torch.from_numpy(bcy), xl, yl, xh, yh, bsx, bsy, nm, n - nm, nb, nb, 3, 3, 1)
+    init_ref.mul_(float(f["target_density"]))
+    assert np.array_equal(init, init_ref.numpy())
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)) for k, v in
+         dict(wxc=dp.wxc, wyc=dp.wyc, offx=dp.offx, offy=dp.offy, ratio=dp.ratio).items()}
+    m_ref = ep.density_map(torch.from_numpy(pos), t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"],
+                           torch.from_numpy(bcx), torch.from_numpy(bcy), torch.from_numpy(init),
+                           float(f["target_density"]), xl, yl, xh, yh, bsx, bsy, nm, 0, 0, nb, nb, 3, 3, 3, 3, 1)
+    m = oracle.density_map(dp, pos, init, num_filler_nodes=0)
+    assert np.array_equal(m.reshape(-1), m_ref.numpy().reshape(-1))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_electric_force_matches_reference(oracle, ref, dtype):
+    d = design(0, dtype=dtype)
+    dp = density_params(oracle, d)
+    t = _ref_density_inputs(d, dp, dtype)
+    rng = np.random.default_rng(11)
+    fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    g = torch.tensor([0.5], dtype=torch.from_numpy(fx).dtype)
+    out = ref["electric_potential_cpp"].electric_force(
+        g, d.num_bins_x, d.num_bins_y, 3, 3, 3, 3, torch.from_numpy(fx).view(-1), torch.from_numpy(fy).view(-1),
+        t["pos"], t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"], t["bcx"], t["bcy"], d.xl, d.yl, d.xh,
+        d.yh, d.bin_size_x, d.bin_size_y, d.num_movable_nodes, d.num_filler_nodes)
+    mine = oracle.electric_force(dp, d.pos, fx, fy, grad=0.5)
+    assert np.array_equal(mine, -out.numpy())
+
+
+@pytest.mark.parametrize("shape", [(4, 8), (4, 4), (64, 32), (30, 18)])
+@pytest.mark.parametrize("name,fn", [("dct2", "dct2_fft2"), ("idct2", "idct2_fft2"),
+                                     ("idxst_idct", "idxst_idct"), ("idct_idxst", "idct_idxst")])
+def test_transforms_match_reference(oracle, ref, name, fn, shape):
+    from oracle.ref_iteration import exact_expk
+    M, N = shape
+    x = np.random.default_rng(5).standard_normal((M, N))
+    out = torch.empty(M, N, dtype=torch.float64)
+    buf = torch.empty(M, N // 2 + 1, 2, dtype=torch.float64)
+    getattr(ref["dct2_fft2_cpp"], fn)(torch.from_numpy(x), exact_expk(M, torch.float64),
+                                      exact_expk(N, torch.float64), out, buf, 1)
+    assert rel_l2(getattr(oracle, name)(x), out.numpy()) <= 1e-13
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)], ids=["f64", "f32"])
+def test_wa_matches_reference(oracle, ref, dtype, tol):
+    d = design(0, dtype=dtype)
+    pin = oracle.pin_pos_forward(d.pos, d.pin_offset_x, d.pin_offset_y, d.pin2node_map)
+    m = ref["weighted_average_wirelength_cpp_merged"]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    ig = torch.tensor(1.0 / d.wa_gamma, dtype=T(pin).dtype)
+    wl_ref, gi_ref = m.forward(T(pin), T(d.flat_net2pin_map), T(d.flat_net2pin_start_map), T(d.pin2net_map),
+                               T(d.net_weights), T(d.net_mask_ignore_large_degrees), ig)
+    wl, partial, gi = oracle.wa_merged_forward(pin, d.flat_net2pin_map, d.flat_net2pin_start_map,
+                                               d.net_weights, d.net_mask_ignore_large_degrees, d.wa_gamma)
+    assert abs(wl - wl_ref.item()) <= tol * abs(wl)
+    assert np.max(np.abs(gi - gi_ref.numpy())) <= tol * np.abs(gi).max()
+    g_ref = m.backward(torch.tensor(1.0, dtype=ig.dtype), T(pin), gi_ref.clone(), T(d.flat_net2pin_map),
+                       T(d.flat_net2pin_start_map), T(d.pin2net_map), T(d.net_weights),
+                       T(d.net_mask_ignore_large_degrees), ig)
+    P = d.num_pins
+    g_ref[:P].masked_fill_(T(d.pin_mask_ignore_fixed_macros), 0)
+    g_ref[P:].masked_fill_(T(d.pin_mask_ignore_fixed_macros), 0)
+    g = oracle.wa_merged_backward(1.0, gi, d.pin2net_map, d.net_weights, d.net_mask_ignore_large_degrees,
+                                  d.pin_mask_ignore_fixed_macros)
+    assert np.max(np.abs(g - g_ref.numpy())) <= tol * np.abs(g).max()
+
+
+def test_small_ops_match_reference(oracle, ref):
+    d = design(0)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    pin_ref = ref["pin_pos_cpp"].forward(T(d.pos), T(d.pin_offset_x), T(d.pin_offset_y),
+                                         T(d.pin2node_map.astype(np.int64)), T(d.flat_node2pin_map),
+                                         T(d.flat_node2pin_start_map))
+    pin = oracle.pin_pos_forward(d.pos, d.pin_offset_x, d.pin_offset_y, d.pin2node_map)
+    assert np.array_equal(pin, pin_ref.numpy())
+    g = np.random.default_rng(1).standard_normal(2 * d.num_pins)
+    gpos_ref = ref["pin_pos_cpp"].backward(T(g), T(d.pos), T(d.pin_offset_x), T(d.pin_offset_y),
+                                           T(d.pin2node_map.astype(np.int64)), T(d.flat_node2pin_map),
+                                           T(d.flat_node2pin_start_map), d.num_physical_nodes)
+    gpos = oracle.pin_pos_backward(g, d.flat_node2pin_map, d.flat_node2pin_start_map, d.num_physical_nodes,
+                                   d.num_nodes)
+    assert np.array_equal(gpos, gpos_ref.numpy())
+    h_ref = ref["hpwl_cpp"].forward(T(pin), T(d.flat_net2pin_map), T(d.flat_net2pin_start_map),
+                                    T(d.net_weights), T(d.net_mask_all))
+    h = oracle.hpwl(pin, d.flat_net2pin_map, d.flat_net2pin_start_map, d.net_weights, d.net_mask_all)
+    assert h == pytest.approx(h_ref.item(), rel=1e-13)
+    pos = d.pos + np.random.default_rng(2).normal(0, 30, d.pos.shape)
+    pr = T(pos.copy())
+    ref["move_boundary_cpp"].forward(pr, T(d.node_size_x), T(d.node_size_y), d.xl, d.yl, d.xh, d.yh,
+                                     d.num_movable_nodes, d.num_filler_nodes)
+    mine = oracle.move_boundary(pos.copy(), d.node_size_x, d.node_size_y, d.xl, d.yl, d.xh, d.yh,
+                                d.num_movable_nodes, d.num_filler_nodes)
+    assert np.array_equal(mine, pr.numpy())
+    w = np.exp(np.random.default_rng(3).uniform(-0.7, 1.4, d.num_nets))
+    pws_ref = ref["pws_cpp"].forward(T(w), T(d.flat_node2pin_map), T(d.flat_node2pin_start_map),
+                                     T(d.pin2net_map), d.num_nodes)
+    pws = oracle.pin_weight_sum(w, d.flat_node2pin_map, d.flat_node2pin_start_map, d.pin2net_map, d.num_nodes)
+    assert np.array_equal(pws, pws_ref.numpy())
+
+
+def test_full_iteration_matches_reference(oracle, ref):
+    from oracle import ref_iteration as R
+    d = design(0)
+    it = R.RefIteration(d, dtype=torch.float64, density_weight=0.25)
+    rf, rg, det = it.iterate()
+    f, g, parts = oracle.objective_and_gradient(d, 0.25, d.wa_gamma)
+    assert f == pytest.approx(rf.item(), rel=1e-13)
+    assert rel_l2(g, rg.numpy()) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# 3. mathematics
+# ---------------------------------------------------------------------------
+def _tiny_design(rng, n=40, nb=8, die=16.0):
+    bs = die / nb
+    w = rng.uniform(0.3, 3.5, n)
+    h = rng.uniform(0.3, 3.5, n)
+    x = rng.uniform(0, die - w)
+    y = rng.uniform(0, die - h)
+    dp = type("DP", (), {})()
+    dp.wxc = np.maximum(w, bs * math.sqrt(2))
+    dp.wyc = np.maximum(h, bs * math.sqrt(2))
+    dp.offx = (w - dp.wxc) * 0.5
+    dp.offy = (h - dp.wyc) * 0.5
+    dp.ratio = w * h / (dp.wxc * dp.wyc)
+    return x, y, w, h, dp, bs, nb, die
+
+
+def test_triangle_scatter_brute_force(oracle):
+    """Each (cell, bin) adds the overlap of the stretched cell with the bin,
+    times the area ratio (within the 2^-32 fixed-point step per pair)."""
+    rng = np.random.default_rng(0)
+    x, y, w, h, dp, bs, nb, die = _tiny_design(rng)
+    scale = oracle.fixed_scale(0, 0, die, die)
+    buf = oracle.triangle_density_fixed(x, y, dp.wxc, dp.wyc, dp.offx, dp.offy, dp.ratio, nb, nb, 0.0, 0.0, bs,
+                                        bs, scale)
+    brute = np.zeros((nb, nb))
+    for i in range(len(x)):
+        x0, y0 = x[i] + dp.offx[i], y[i] + dp.offy[i]
+        for k in range(nb):
+            ox = max(0.0, min(x0 + dp.wxc[i], (k + 1) * bs) - max(x0, k * bs))
+            for l in range(nb):
+                oy = max(0.0, min(y0 + dp.wyc[i], (l + 1) * bs) - max(y0, l * bs))
+                brute[k, l] += ox * oy * dp.ratio[i]
+    got = buf.reshape(nb, nb) / scale
+    assert np.max(np.abs(got - brute)) <= len(x) * 9 * 2.0 ** -32 + 1e-12
+
+
+def test_triangle_scatter_mass_and_order(oracle):
+    """Cells inside the die: the map sums to the cells' true area; any
+    permutation of the cells gives the identical int64 map."""
+    rng = np.random.default_rng(1)
+    die, nb = 64.0, 32
+    bs = die / nb
+    n = 500
+    w, h = rng.uniform(0.5, 6, n), rng.uniform(0.5, 6, n)
+    wxc, wyc = np.maximum(w, bs * math.sqrt(2)), np.maximum(h, bs * math.sqrt(2))
+    offx, offy = (w - wxc) / 2, (h - wyc) / 2
+    ratio = w * h / (wxc * wyc)
+    x = rng.uniform(-offx, die - wxc - offx)
+    y = rng.uniform(-offy, die - wyc - offy)
+    scale = oracle.fixed_scale(0, 0, die, die)
+    buf = oracle.triangle_density_fixed(x, y, wxc, wyc, offx, offy, ratio, nb, nb, 0.0, 0.0, bs, bs, scale)
+    assert abs(buf.sum() / scale - (w * h).sum()) <= 1e-6
+    p = rng.permutation(n)
+    buf2 = oracle.triangle_density_fixed(x[p], y[p], wxc[p], wyc[p], offx[p], offy[p], ratio[p], nb, nb, 0.0,
+                                         0.0, bs, bs, scale)
+    assert np.array_equal(buf, buf2)
+
+
+def test_fixed_point_scale(oracle):
+    assert oracle.fixed_scale(0, 0, 512, 512) == 2 ** 32
+    assert oracle.fixed_scale(0, 0, 1e4, 1e4) == 2 ** 32
+    assert oracle.fixed_scale(0, 0, 2e4, 2e4) == 2 ** 32
+    # area 2^40 -> ceil(log2)+1 = 41 integer bits -> 23 fraction bits
+    assert oracle.fixed_scale(0, 0, 2.0 ** 20, 2.0 ** 20) == 2 ** 23
+
+
+def test_dct_brute_force(oracle):
+    """The four transforms against their defining double sums (4x8)."""
+    M, N = 4, 8
+    x = np.random.default_rng(2).standard_normal((M, N))
+    cu = lambda k: 1.0 if k == 0 else 2.0
+    C = lambda u, i, K: math.cos(math.pi * u * (2 * i + 1) / (2 * K))
+    S = lambda u, i, K: math.sin(math.pi * u * (2 * i + 1) / (2 * K))
+    d2 = np.zeros((M, N))
+    i2 = np.zeros((M, N))
+    xs = np.zeros((M, N))
+    cs = np.zeros((M, N))
+    for a in range(M):
+        for b in range(N):
+            d2[a, b] = 4 / (M * N) * sum(x[i, j] * C(a, i, M) * C(b, j, N) for i in range(M) for j in range(N))
+            i2[a, b] = sum(cu(u) * cu(v) * x[u, v] * C(u, a, M) * C(v, b, N) for u in range(M) for v in range(N))
+            xs[a, b] = 2 * sum(cu(v) * x[u, v] * S(u, a, M) * C(v, b, N) for u in range(M) for v in range(N))
+            cs[a, b] = 2 * sum(cu(u) * x[u, v] * C(u, a, M) * S(v, b, N) for u in range(M) for v in range(N))
+    assert np.allclose(oracle.dct2(x), d2, atol=1e-13)
+    assert np.allclose(oracle.idct2(x), i2, atol=1e-13)
+    assert np.allclose(oracle.idxst_idct(x), xs, atol=1e-13)
+    assert np.allclose(oracle.idct_idxst(x), cs, atol=1e-13)
+
+
+def test_dct_inversion(oracle):
+    """IDCT2(DCT2(x)) = 4x for these normalisations (c_0=1, c_k=2 per axis)."""
+    x = np.random.default_rng(3).standard_normal((16, 12))
+    assert np.allclose(oracle.idct2(oracle.dct2(x)), 4 * x, atol=1e-12)
+
+
+@pytest.mark.parametrize("u,v", [(1, 0), (3, 2), (0, 5), (7, 7)])
+def test_poisson_single_mode(oracle, u, v):
+    """rho = C_M[u,m] C_N[v,n].  From sum_m C[u,m]C[u',m] = delta*(M if u=0
+    else M/2) and the transform definitions:
+      a = DCT2(rho) = 4/(c_u c_v) e_uv,   phi = 4 K rho,
+      E_x = (4/c_u) wu K S_M[u,m] C_N[v,n],   E_y = (4/c_v) wv K C_M[u,m] S_N[v,n],
+      energy = sum(phi*rho) = 4 K sum(rho^2),
+    with K = 1/(wu^2+wv^2); the bin aspect 2 exercises the wv scaling."""
+    M, N, bx, by = 16, 32, 2.0, 1.0
+    m = np.arange(M)[:, None]
+    n = np.arange(N)[None, :]
+    Cu = np.cos(np.pi * u * (2 * m + 1) / (2 * M))
+    Cv = np.cos(np.pi * v * (2 * n + 1) / (2 * N))
+    Su = np.sin(np.pi * u * (2 * m + 1) / (2 * M))
+    Sv = np.sin(np.pi * v * (2 * n + 1) / (2 * N))
+    rho = Cu * Cv
+    r = oracle.poisson_solve(rho * (bx * by), bx, by)
+    wu = 2 * np.pi * u / M
+    wv = 2 * np.pi * v / N * (bx / by)
+    K = 1.0 / (wu ** 2 + wv ** 2)
+    cu = 1.0 if u == 0 else 2.0
+    cv = 1.0 if v == 0 else 2.0
+    a_expect = np.zeros((M, N))
+    a_expect[u, v] = 4.0 / (cu * cv)
+    assert np.allclose(r["auv"], a_expect, atol=1e-12)
+    assert np.allclose(r["phi"], 4.0 * K * rho, atol=1e-12)
+    assert np.allclose(r["field_x"], (4.0 / cu) * wu * K * Su * Cv, atol=1e-12)
+    assert np.allclose(r["field_y"], (4.0 / cv) * wv * K * Cu * Sv, atol=1e-12)
+    assert r["energy"] == pytest.approx(4.0 * K * float((rho * rho).sum()), rel=1e-12)
+
+
+def test_poisson_constant_map(oracle):
+    """A uniform density has no field and no energy."""
+    r = oracle.poisson_solve(np.full((16, 16), 3.7), 1.5, 1.5)
+    assert np.abs(r["field_x"]).max() < 1e-12 and np.abs(r["field_y"]).max() < 1e-12
+    assert abs(r["energy"]) < 1e-9
+
+
+def test_poisson_field_is_potential_gradient(oracle):
+    """E_x = -d(phi)/dm of phi's cosine interpolant, on a random map.
+    phi = sum c_u c_v b[u,v] C_M[u,m] C_N[v,n] with b = DCT2(phi)/4, so
+    d(phi)/dm = sum c_u c_v b (-(pi u/M) S_M[u,m]) C_N[v,n]."""
+    rng = np.random.default_rng(4)
+    M, N = 12, 10
+    r = oracle.poisson_solve(rng.standard_normal((M, N)), 1.0, 1.0)
+    b = oracle.dct2(r["phi"]) / 4.0
+    u = np.arange(M)
+    v = np.arange(N)
+    c_u = np.where(u == 0, 1.0, 2.0)
+    c_v = np.where(v == 0, 1.0, 2.0)
+    mm = np.arange(M)
+    nn = np.arange(N)
+    dS = -(np.pi * u[:, None] / M) * np.sin(np.pi * u[:, None] * (2 * mm[None, :] + 1) / (2 * M))  # [u, m]
+    Cn = np.cos(np.pi * v[:, None] * (2 * nn[None, :] + 1) / (2 * N))  # [v, n]
+    dphi = dS.T @ (c_u[:, None] * c_v[None, :] * b) @ Cn
+    assert np.allclose(r["field_x"], -dphi, atol=1e-10 * np.abs(r["field_x"]).max())
+
+
+def _wa_fixture(rng, n_nets=30, dtype=np.float64):
+    deg = rng.integers(2, 7, n_nets)
+    P = int(deg.sum())
+    flat = np.arange(P, dtype=np.int32)
+    start = np.zeros(n_nets + 1, dtype=np.int32)
+    start[1:] = np.cumsum(deg)
+    pin2net = np.repeat(np.arange(n_nets, dtype=np.int32), deg)
+    pos = rng.uniform(0, 50, 2 * P).astype(dtype)
+    w = rng.uniform(0.5, 4, n_nets).astype(dtype)
+    mask = np.ones(n_nets, dtype=np.uint8)
+    mask[::7] = 0
+    return pos, flat, start, pin2net, w, mask
+
+
+def test_wa_gradient_finite_difference(oracle):
+    rng = np.random.default_rng(5)
+    pos, flat, start, pin2net, w, mask = _wa_fixture(rng)
+    gamma = 4.0
+    wl, partial, gi = oracle.wa_merged_forward(pos, flat, start, w, mask, gamma)
+    g = oracle.wa_merged_backward(1.0, gi, pin2net, w, mask, np.zeros(len(pos) // 2, dtype=bool))
+    h = 1e-6
+    for k in rng.choice(len(pos), 40, replace=False):
+        pp, pm = pos.copy(), pos.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (oracle.wa_merged_forward(pp, flat, start, w, mask, gamma)[0] -
+              oracle.wa_merged_forward(pm, flat, start, w, mask, gamma)[0]) / (2 * h)
+        assert fd == pytest.approx(g[k], abs=1e-6)
+
+
+def test_wa_net_weight_adjoint_finite_difference(oracle):
+    rng = np.random.default_rng(6)
+    pos, flat, start, pin2net, w, mask = _wa_fixture(rng)
+    wl, partial, gi = oracle.wa_merged_forward(pos, flat, start, w, mask, 2.0)
+    gw = oracle.wa_net_weight_grad(1.0, partial, mask)
+    h = 1e-5
+    for e in range(len(w)):
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (oracle.wa_merged_forward(pos, flat, start, wp, mask, 2.0)[0] -
+              oracle.wa_merged_forward(pos, flat, start, wm, mask, 2.0)[0]) / (2 * h)
+        assert fd == pytest.approx(gw[e], abs=1e-6)
+
+
+def test_wa_limits_and_invariance(oracle):
+    """gamma -> 0 gives HPWL; shifting all pins leaves WA and its gradient unchanged."""
+    rng = np.random.default_rng(7)
+    pos, flat, start, pin2net, w, mask = _wa_fixture(rng)
+    wl = oracle.wa_merged_forward(pos, flat, start, w, mask, 1e-3)[0]
+    hp = oracle.hpwl(pos, flat, start, w, mask)
+    assert wl == pytest.approx(hp, rel=1e-9)
+    P = len(pos) // 2
+    shifted = pos.copy()
+    shifted[:P] += 123.0
+    shifted[P:] -= 77.0
+    a = oracle.wa_merged_forward(pos, flat, start, w, mask, 3.0)
+    b = oracle.wa_merged_forward(shifted, flat, start, w, mask, 3.0)
+    assert a[0] == pytest.approx(b[0], rel=1e-10)
+    assert np.allclose(a[2], b[2], atol=1e-10)
+
+
+def test_pin_pos_adjoint(oracle):
+    d = design(0)
+    rng = np.random.default_rng(8)
+    dx = rng.standard_normal(2 * d.num_nodes)
+    dx[d.num_physical_nodes:d.num_nodes] = 0
+    dx[d.num_nodes + d.num_physical_nodes:] = 0
+    zero = np.zeros(d.num_pins)
+    fwd = oracle.pin_pos_forward(dx, zero, zero, d.pin2node_map)
+    g = rng.standard_normal(2 * d.num_pins)
+    bwd = oracle.pin_pos_backward(g, d.flat_node2pin_map, d.flat_node2pin_start_map, d.num_physical_nodes,
+                                  d.num_nodes)
+    assert float(fwd @ g) == pytest.approx(float(dx @ bwd), rel=1e-12)
+
+
+def test_pin_weight_sum_closed_form(oracle):
+    d = design(0)
+    w = np.random.default_rng(9).uniform(0.5, 4, d.num_nets)
+    expect = np.zeros(d.num_nodes)
+    np.add.at(expect, d.pin2node_map, w[d.pin2net_map])
+    got = oracle.pin_weight_sum(w, d.flat_node2pin_map, d.flat_node2pin_start_map, d.pin2net_map, d.num_nodes)
+    assert np.allclose(got, expect, rtol=1e-13)
+
+
+def test_planted_sign_error_is_caught(oracle, monkeypatch):
+    """A sign flip in the oracle's field would be caught by the single-mode
+    closed form (this plants the mistake in the oracle, on the CPU)."""
+    orig = oracle.idxst_idct
+    monkeypatch.setattr(oracle, "idxst_idct", lambda x: -orig(x))
+    with pytest.raises(AssertionError):
+        test_poisson_single_mode(oracle, 3, 2)
