@@ -34,23 +34,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(d, kernel):
-    """Algorithmic HBM bytes of one launch (FP32 data, int32 indices)."""
+def algorithmic_bytes(d, kernel, schedule="pins", algorithm="tiled"):
+    """Algorithmic HBM bytes of one launch of `kernel` (FP32 data, int32
+    indices): every array element the kernel must read or write once."""
     Ns = d.num_movable_nodes + d.num_filler_nodes
     B = d.num_bins_x * d.num_bins_y
     P, E, n, nphys = d.num_pins, d.num_nets, d.num_nodes, d.num_physical_nodes
-    return {
-        # 7 cell arrays read per scattered cell; int64 map read + written once
-        "density_scatter": 28 * Ns + 16 * B,
-        # 7 cell arrays read, 2 gradients written per node; both FP32 field maps read
-        "electric_force": 28 * Ns + 8 * n + 8 * B,
-        # per pin: netpin index, pin2node, 2 offsets, gathered x/y, 2 derivatives written;
-        # per net: start, mask, weight, partial WL
-        "wa_forward": 32 * P + 13 * E,
-        # per pin: node2pin index, 2 derivatives, pin2net, weight, pin mask;
-        # per node: start, 2 density-gradient reads, 2 gradient writes
-        "wa_pin_pos_backward": 21 * P + 4 * nphys + 16 * n,
-    }[kernel]
+    if kernel == "density_scatter":
+        if algorithm == "tiled":  # one 24-byte record per cell in, int64 map in + out
+            return 24 * Ns + 16 * B
+        return 28 * Ns + 16 * B  # 7 cell arrays in, int64 map in + out
+    if kernel == "electric_force":
+        if algorithm == "tiled":  # record in, 2 gradients out, both FP32 field maps in
+            return 24 * Ns + 8 * Ns + 8 * B
+        return 28 * Ns + 8 * n + 8 * B
+    if kernel == "wa_forward":
+        if schedule == "pins":
+            # per pin: netpin index, pin2net, pin2node, 2 offsets, gathered (x,y),
+            # pin mask, node-order index, (gx,gy) out; per net: start, mask, weight, WL
+            return 41 * P + 13 * E
+        return 32 * P + 13 * E
+    if kernel == "wa_pin_pos_backward":
+        if schedule == "pins":  # (gx,gy) per pin in node order; start per node; add in, grad out
+            return 8 * P + 4 * nphys + 16 * n
+        return 21 * P + 4 * nphys + 16 * n
+    raise KeyError(kernel)
 
 
 KERNELS = {  # name -> probe id (include/dreamplace_amd.h)
@@ -129,7 +137,10 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
-    ap.add_argument("--unfused", action="store_true", help="reference op sequence, no fusion")
+    ap.add_argument("--schedule", default="pins", choices=["pins", "fused", "unfused"],
+                    help="pins: pin-parallel WA; fused: net-parallel fused; unfused: reference op order")
+    ap.add_argument("--algorithm", default="tiled", choices=["tiled", "global"],
+                    help="density scatter / field gather: LDS-binned tiles or per-cell global atomics")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -153,7 +164,7 @@ def main():
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
-                               fused=not args.unfused)
+                               algorithm=args.algorithm, schedule=args.schedule)
     pos32 = dd32.pos
     ms32, per32 = time_steps(torch, obj32, pos32, args.steps, args.warmup)
     log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32, statistics.median(per32)))
@@ -162,7 +173,8 @@ def main():
     kms = {k: probe_kernel(torch, N, obj32, pos32, kid, 3) for k, kid in KERNELS.items()}
     dom = max(kms, key=kms.get)
     dom_ms = probe_kernel(torch, N, obj32, pos32, KERNELS[dom], max(args.steps, 10))
-    achieved = algorithmic_bytes(d, dom) / (dom_ms * 1e-3) / 1e9
+    dom_bytes = algorithmic_bytes(d, dom, args.schedule, args.algorithm)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     log("[bench] kernel ms: " + ", ".join("%s %.3f" % kv for kv in kms.items()))
 
     result = {}
@@ -172,7 +184,7 @@ def main():
         fx32 = obj32.field_x.double().clone()
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
-                                   fused=not args.unfused)
+                                   algorithm=args.algorithm, schedule=args.schedule)
         ms64, per64 = time_steps(torch, obj64, dd64.pos, max(args.steps // 2, 5), 2)
         log("[bench] fp64: %.3f ms/step" % ms64)
         obj64.evaluate(dd64.pos)
@@ -207,7 +219,8 @@ def main():
                         % (args.config, "" if args.scale == 1.0 else " scale %g" % args.scale,
                            d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, d.num_nets,
                            d.num_pins, d.num_bins_x, d.num_bins_y),
-            "form": "fp32 + %s spectral solve, %s" % (args.precision, "unfused" if args.unfused else "fused"),
+            "form": "fp32 + %s spectral solve, %s schedule, %s density" % (
+                args.precision, args.schedule, args.algorithm),
             "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
             **result,
         },
@@ -220,7 +233,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "kernel_ms": round(dom_ms, 4),
-            "algorithmic_bytes": algorithmic_bytes(d, dom),
+            "algorithmic_bytes": dom_bytes,
         },
         "cpu_baseline": cpu,
     }

@@ -33,13 +33,13 @@ _SIGNATURES = {
     "dp_last_error": (ctypes.c_char_p, []),
     "dp_version": (_I, []),
     "dp_fixed_point_scale": (_L, [_D, _D, _D, _D]),
-    "dp_density_map_workspace_bytes": (_S, [_L, _L, _L, _L]),
+    "dp_density_map_workspace_bytes": (_S, [_L, _L, _L, _L, _I, _I]),
     "dp_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L,
                             _I, _I, _I, _I, _L, _P, _P, _P, _P]),
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
-                               _I, _I, _L, _P, _P, _P]),
+                               _I, _I, _I, _L, _P, _P, _P]),
     "dp_reduce_workspace_bytes": (_S, [_L]),
     "dp_density_overflow": (_I, [_I, _P, _L, _D, _P, _P, _P]),
     "dp_spectral_plan_bytes": (_S, [_I, _I, _I]),
@@ -60,6 +60,11 @@ _SIGNATURES = {
                                       _P, _P, _P, _P]),
     "dp_wa_backward_to_nodes": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_profile_kernel": (_I, [_I, _P, _P]),
+    "dp_wa_pack_nets": (_I, [_P, _L, _P, _P, _P, _P]),
+    "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
+    "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _P, _P, _L, _P, _L, _P, _P,
+                                _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),

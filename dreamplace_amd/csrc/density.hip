@@ -207,27 +207,17 @@ struct DensityWorkspace {
   unsigned long long* map;
   int* qcount;
   int* qlist;
+  char* tiled;
 };
-
-static size_t density_ws_bytes(int64_t nb, int64_t qcap) {
-  return align256(sizeof(unsigned long long) * (size_t)nb) + 256 +
-         align256(sizeof(int) * (size_t)(qcap > 0 ? qcap : 1)) + tiled_ws_bytes(nb);
-}
 
 static DensityWorkspace carve(void* ws, int64_t nb, int64_t qcap) {
   char* p = (char*)ws;
   DensityWorkspace w;
   w.map = (unsigned long long*)p;
-  p += align256(sizeof(unsigned long long) * (size_t)nb);
-  w.qcount = (int*)p;
-  p += 256;
-  w.qlist = (int*)p;
+  w.qcount = (int*)(p + density_queue_offset(nb));
+  w.qlist = (int*)(p + density_queue_offset(nb) + 256);
+  w.tiled = p + density_tiled_offset(nb, qcap);
   return w;
-}
-
-char* density_tiled_region(void* ws, int64_t nb, int64_t qcap) {
-  return (char*)ws + align256(sizeof(unsigned long long) * (size_t)nb) + 256 +
-         align256(sizeof(int) * (size_t)(qcap > 0 ? qcap : 1));
 }
 
 template <typename T>
@@ -295,9 +285,12 @@ int64_t dp_fixed_point_scale(double xl, double yl, double xh, double yh) {
 }
 
 size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, int64_t num_nodes,
-                                      int64_t large_capacity) {
-  (void)num_nodes;
-  return density_ws_bytes(num_bins_x * num_bins_y, large_capacity);
+                                      int64_t large_capacity, int algorithm, int dtype) {
+  const int64_t nb = num_bins_x * num_bins_y;
+  size_t b = density_tiled_offset(nb, large_capacity);
+  if (algorithm == DP_SCATTER_TILED && tiled_supported((int)num_bins_x, (int)num_bins_y))
+    b += tiled_ws_bytes((int)num_bins_x, (int)num_bins_y, num_nodes, dtype);
+  return b;
 }
 
 int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
@@ -339,11 +332,9 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
       hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map, nb, 0ull);
     }
     DP_LAUNCHED("density init");
-    const int64_t ranges[2][2] = {{0, num_movable_nodes},
-                                  {num_nodes - num_filler_nodes, num_nodes}};
-    if (algorithm == DP_SCATTER_TILED) {
-      DP_CHECK_RC(scatter_tiled<T>(a, ranges, density_tiled_region(workspace, nb, large_capacity),
-                                   w.map, w.qcount, w.qlist, large_capacity, s));
+    if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y)) {
+      DP_CHECK_RC(scatter_tiled<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
+                                   num_filler_nodes, w.tiled, w.map, s));
     } else {
       DP_CHECK_RC(scatter_global<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
                                     num_filler_nodes, w, large_capacity, s));

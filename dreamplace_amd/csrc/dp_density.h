@@ -67,10 +67,20 @@ TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, co
                          const void* offx, const void* offy, const void* ratio, double xl,
                          double yl, double bsx, double bsy, int nbx, int nby, int64_t scale);
 
-// Tiled (LDS-binned) scatter, density_tiled.hip.
-size_t tiled_ws_bytes(int64_t num_bins);
+// Tiled (LDS-binned) scatter and gather, density_tiled.hip.
+bool tiled_supported(int nbx, int nby);
+size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype);
 template <typename T>
-int scatter_tiled(const TriArgs<T>& a, const int64_t ranges[2][2], char* tiled_ws,
-                  unsigned long long* map, int* qcount, int* qlist, int64_t qcap, hipStream_t s);
+int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
+                  char* tiled_ws, unsigned long long* map, hipStream_t s);
+template <typename T>
+int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
+                 const T* fx, const T* fy, const T* grad, char* tiled_ws, T* out, hipStream_t s);
+
+// Density workspace layout: [int64 map][queue count (256 B)][queue][tiled region].
+inline size_t density_queue_offset(int64_t nb) { return align256(8 * (size_t)nb); }
+inline size_t density_tiled_offset(int64_t nb, int64_t qcap) {
+  return density_queue_offset(nb) + 256 + align256(4 * (size_t)(qcap > 0 ? qcap : 1));
+}
 
 }  // namespace dp

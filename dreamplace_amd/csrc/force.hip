@@ -123,8 +123,9 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                  const void* offset_x, const void* offset_y, const void* ratio,
                                  double xl, double yl, double bin_size_x, double bin_size_y,
                                  int64_t num_movable_nodes, int64_t num_filler_nodes,
-                                 int num_bins_x, int num_bins_y, int64_t large_capacity,
-                                 void* workspace, void* grad_out, void* stream) {
+                                 int num_bins_x, int num_bins_y, int algorithm,
+                                 int64_t large_capacity, void* workspace, void* grad_out,
+                                 void* stream) {
   DP_ARG(field_map_x && field_map_y && pos && node_size_x_clamped && node_size_y_clamped &&
              offset_x && offset_y && ratio && grad_out && workspace,
          "dp_electric_force: null pointer");
@@ -136,15 +137,22 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
          "dp_electric_force: bad large_capacity");
   hipStream_t s = S(stream);
   const int64_t nb = (int64_t)num_bins_x * num_bins_y;
-  // Same workspace layout as dp_density_map: [int64 map][count][queue].
-  char* p = (char*)workspace + align256(sizeof(unsigned long long) * (size_t)nb);
+  DP_ARG(algorithm == DP_SCATTER_GLOBAL || algorithm == DP_SCATTER_TILED,
+         "dp_electric_force: bad algorithm %d", algorithm);
+  // Same workspace layout as dp_density_map: [int64 map][count][queue][tiled].
+  char* p = (char*)workspace + density_queue_offset(nb);
   int* qcount = (int*)p;
   int* qlist = (int*)(p + 256);
+  char* tiled = (char*)workspace + density_tiled_offset(nb, large_capacity);
   if (num_nodes == 0) return DP_OK;
   return DP_DISPATCH(dtype, "dp_electric_force", [&] {
     TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
                                     num_bins_x, num_bins_y, 1);
+    if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y))
+      return gather_tiled<T>(a, num_nodes, num_movable_nodes, num_filler_nodes,
+                             (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,
+                             tiled, (T*)grad_out, s);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, qcount, (int64_t)1, 0);
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
               hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)),

@@ -52,7 +52,9 @@ class ElectricDensityMapFunction:
         nbx, nby = int(num_bins_x), int(num_bins_y)
         if large_capacity is None:
             large_capacity = num_nodes
-        nbytes = N.query("dp_density_map_workspace_bytes", nbx, nby, num_nodes, int(large_capacity))
+        alg = N.DP_SCATTER_TILED if algorithm == "tiled" else N.DP_SCATTER_GLOBAL
+        nbytes = N.query("dp_density_map_workspace_bytes", nbx, nby, num_nodes, int(large_capacity),
+                         alg, N.dtype_code(pos))
         if workspace is None or workspace.numel() < nbytes:
             workspace = N.workspace.get("density_map", nbytes, pos.device)
         init = initial_density_map
@@ -63,8 +65,7 @@ class ElectricDensityMapFunction:
                N.ptr(node_size_x_clamped), N.ptr(node_size_y_clamped), N.ptr(offset_x),
                N.ptr(offset_y), N.ptr(ratio), N.ptr(init), _f(xl), _f(yl), _f(xh), _f(yh),
                _f(bin_size_x), _f(bin_size_y), int(num_movable_nodes), int(num_filler_nodes), nbx,
-               nby, N.DP_CONVERT_CPU if convention == "cpu" else N.DP_CONVERT_GPU,
-               N.DP_SCATTER_TILED if algorithm == "tiled" else N.DP_SCATTER_GLOBAL,
+               nby, N.DP_CONVERT_CPU if convention == "cpu" else N.DP_CONVERT_GPU, alg,
                int(large_capacity), N.ptr(workspace), N.ptr(out), N.ptr(fixed_map_out),
                N.stream_ptr(pos.device))
         if padding > 0:
@@ -156,9 +157,14 @@ class ElectricOverflow(nn.Module):
         self.large_capacity = int(big.sum().item()) + 64
         self.initial_density_map = None
 
+    @property
+    def algorithm_code(self):
+        return N.DP_SCATTER_TILED if self.algorithm == "tiled" else N.DP_SCATTER_GLOBAL
+
     def workspace(self, device):
         nbytes = N.query("dp_density_map_workspace_bytes", self.num_bins_x, self.num_bins_y,
-                         self.node_size_x.numel(), self.large_capacity)
+                         self.node_size_x.numel(), self.large_capacity, self.algorithm_code,
+                         N.dtype_code(self.node_size_x))
         if self._ws is None or self._ws.numel() < nbytes:
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
         return self._ws

@@ -114,7 +114,7 @@ class ElectricPotential(ElectricOverflow):
                N.ptr(self.node_size_y_clamped), N.ptr(self.offset_x), N.ptr(self.offset_y),
                N.ptr(self.ratio), float(self.xl), float(self.yl), float(self.bin_size_x),
                float(self.bin_size_y), int(self.num_movable_nodes), int(self.num_filler_nodes),
-               self.num_bins_x, self.num_bins_y, self.large_capacity,
+               self.num_bins_x, self.num_bins_y, self.algorithm_code, self.large_capacity,
                N.ptr(self.workspace(pos.device)), N.ptr(out), N.stream_ptr(pos.device))
         return out
 

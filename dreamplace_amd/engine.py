@@ -7,8 +7,12 @@
 
 exactly as ``PlaceObj.obj_and_grad_fn`` does (dreamplace/PlaceObj.py:394-411,
 without the preconditioner), on preallocated device buffers and with no
-autograd graph or host synchronisation.  Two schedules:
+autograd graph or host synchronisation.  Three schedules:
 
+* ``schedule="pins"``: the WA forward is pin-parallel (one wavefront per
+  group of whole nets, one lane per pin) and writes each pin's weighted
+  derivative in node order, so the backward is a contiguous per-node sum
+  (dp_wa_forward_pins / dp_node_grad_from_pins);
 * ``fused=True``: pin positions are formed inside the WA kernel, and the WA
   backward, the pin-to-node sum and the gradient combine are one node-parallel
   pass (dp_wa_forward_from_nodes / dp_wa_backward_to_nodes);
@@ -64,13 +68,45 @@ class DeviceDesign:
         raise AttributeError(k)
 
 
+class PinPlan:
+    """Static plan of the pin-parallel WA kernel for a netlist: net groups of
+    at most 64 pins (dp_wa_pack_nets), the nets above that, and each pin's
+    position in node (CSR) order."""
+
+    def __init__(self, dd):
+        import ctypes
+        start = dd.flat_net2pin_start_map.cpu().numpy().astype(np.int32)
+        E = len(start) - 1
+        groups = np.zeros(2 * max(E, 1), dtype=np.int32)
+        big = np.zeros(max(E, 1), dtype=np.int32)
+        ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
+        N.call("dp_wa_pack_nets", start.ctypes.data_as(ctypes.c_void_p), E,
+               groups.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ng),
+               big.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nb))
+        self.num_groups, self.num_big = ng.value, nb.value
+        dev = dd.device
+        self.groups = torch.from_numpy(groups[:2 * max(self.num_groups, 1)]).to(dev)
+        self.big = torch.from_numpy(big[:max(self.num_big, 1)]).to(dev)
+        P = dd.num_pins
+        order = torch.empty(P, dtype=torch.int32, device=dev)
+        order[dd.flat_node2pin_map.long()] = torch.arange(P, dtype=torch.int32, device=dev)
+        self.pin_node_order = order
+        self.ws = torch.empty(N.query("dp_wa_pins_workspace_bytes", dd.num_physical_nodes,
+                                      N.dtype_code(dd.pos)), dtype=torch.uint8, device=dev)
+        self.grad_node_order = torch.empty(2 * P, dtype=dd.dtype, device=dev)
+
+
 class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
-                 energy_mode="potential", convention="cpu", algorithm="global"):
+                 energy_mode="potential", convention="cpu", algorithm="global", schedule=None):
         self.dd = dd
         dt = dd.dtype
         dev = dd.device
         self.fused = fused
+        # schedule: "pins" (pin-parallel WA + contiguous node backward),
+        # "fused" (net-parallel fused kernels) or "unfused" (reference op order)
+        self.schedule = schedule or ("fused" if fused else "unfused")
+        self.pin_plan = PinPlan(dd) if self.schedule == "pins" else None
         self.density = ElectricPotential(
             dd.node_size_x, dd.node_size_y, dd.bin_center_x, dd.bin_center_y, dd.target_density,
             dd.xl, dd.yl, dd.xh, dd.yh, dd.bin_size_x, dd.bin_size_y, dd.num_movable_nodes,
@@ -122,8 +158,7 @@ class PlacementObjective:
                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
                ep.num_bins_x, ep.num_bins_y,
                N.DP_CONVERT_CPU if ep.convention == "cpu" else N.DP_CONVERT_GPU,
-               N.DP_SCATTER_TILED if ep.algorithm == "tiled" else N.DP_SCATTER_GLOBAL,
-               ep.large_capacity, N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map), None,
+               ep.algorithm_code, ep.large_capacity, N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map), None,
                N.stream_ptr(pos.device))
         dmap = self.density_map
         if dmap.dtype != self.plan.dtype:
@@ -144,7 +179,8 @@ class PlacementObjective:
                N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x), N.ptr(ep.offset_y),
                N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
-               ep.num_bins_x, ep.num_bins_y, ep.large_capacity, N.ptr(ep.workspace(pos.device)),
+               ep.num_bins_x, ep.num_bins_y, ep.algorithm_code, ep.large_capacity,
+               N.ptr(ep.workspace(pos.device)),
                N.ptr(self.density_grad), N.stream_ptr(pos.device))
 
     def evaluate(self, pos):
@@ -153,7 +189,21 @@ class PlacementObjective:
         code = N.dtype_code(pos)
         s = N.stream_ptr(pos.device)
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
-        if self.fused:
+        if self.schedule == "pins":
+            pp = self.pin_plan
+            N.call("dp_wa_forward_pins", code, N.ptr(pos), n, dd.num_physical_nodes,
+                   N.ptr(dd.pin_offset_x), N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map),
+                   N.ptr(dd.flat_net2pin_map), N.ptr(dd.flat_net2pin_start_map),
+                   N.ptr(dd.pin2net_map), N.ptr(pp.groups), pp.num_groups, N.ptr(pp.big),
+                   pp.num_big, N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.pin_mask),
+                   N.ptr(pp.pin_node_order), N.ptr(self.inv_gamma), N.ptr(self.partial_wl),
+                   N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)
+            self._density_forward(pos)
+            self._density_backward(pos)
+            N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
+                   N.ptr(dd.flat_node2pin_start_map), N.ptr(self.density_grad),
+                   dd.num_physical_nodes, n, N.ptr(self.grad), s)
+        elif self.schedule == "fused":
             N.call("dp_wa_forward_from_nodes", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), P, N.ptr(dd.flat_net2pin_map),
                    N.ptr(dd.flat_net2pin_start_map), E, N.ptr(self.w), N.ptr(self.net_mask),

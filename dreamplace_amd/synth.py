@@ -246,6 +246,10 @@ def make_design(config=0, seed=0, dtype=np.float64, scale=1.0, bins=None, ignore
             deg[idx] = np.minimum(deg[idx] + 1, c["max_degree"])
     else:
         deg = _degrees(rng, num_nets, c["max_degree"])
+    if c.get("big_nets"):
+        # a few nets above 64 pins (some at or above the ignore threshold)
+        k = min(int(c["big_nets"]), num_nets)
+        deg[rng.choice(num_nets, size=k, replace=False)] = rng.integers(65, 151, size=k)
     deg = np.minimum(deg, num_physical)
     num_pins = int(deg.sum())
     pin2node = _distinct_nodes(rng, deg, num_physical).astype(np.int32)

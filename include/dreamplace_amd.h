@@ -78,8 +78,11 @@ int dp_profile_kernel(int kernel_id, void* start_event, void* stop_event);
  * (electric_density_map.cpp:117-123). */
 int64_t dp_fixed_point_scale(double xl, double yl, double xh, double yh);
 
+/* Bytes of the workspace shared by dp_density_map, dp_fixed_density_map and
+ * dp_electric_force.  With DP_SCATTER_TILED it includes one packed record per
+ * movable/filler cell (the binning of the last dp_density_map call). */
 size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, int64_t num_nodes,
-                                      int64_t large_capacity);
+                                      int64_t large_capacity, int algorithm, int dtype);
 
 /* Replaces electric_potential_cuda.density_map
  * (dreamplace/ops/electric_potential/src/electric_density_map_cuda.cpp:71-136;
@@ -118,13 +121,16 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
  * electric_force_cuda.cpp:56-126, electric_force.cpp:76-172).
  * grad_out[2N] = -(ratio_i * sum_bins px*py*E) * (*grad_scale) for movables and
  * fillers, 0 for terminals.  grad_scale is a device scalar of `dtype`
- * (nullable: 1).  workspace as for dp_density_map (same large_capacity). */
+ * (nullable: 1).  workspace as for dp_density_map (same large_capacity).
+ * With DP_SCATTER_TILED the gather reuses the cell binning that the last
+ * dp_density_map call (DP_SCATTER_TILED, same workspace) made: `pos` must be
+ * the positions of that call. */
 int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
                       const void* field_map_y, const void* pos, int64_t num_nodes,
                       const void* node_size_x_clamped, const void* node_size_y_clamped,
                       const void* offset_x, const void* offset_y, const void* ratio, double xl,
                       double yl, double bin_size_x, double bin_size_y, int64_t num_movable_nodes,
-                      int64_t num_filler_nodes, int num_bins_x, int num_bins_y,
+                      int64_t num_filler_nodes, int num_bins_x, int num_bins_y, int algorithm,
                       int64_t large_capacity, void* workspace, void* grad_out, void* stream);
 
 /* ElectricOverflow metric (electric_overflow.py:260-278): out2[0] =
@@ -245,6 +251,40 @@ int dp_wa_backward_to_nodes(int dtype, const void* grad, const void* grad_interm
                             const void* net_weights, const uint8_t* net_mask,
                             const uint8_t* pin_mask, const void* add, int64_t num_physical_nodes,
                             int64_t num_nodes, void* grad_pos_out, void* stream);
+
+/* Pin-parallel form of the fused wirelength chain (no reference counterpart:
+ * the reference runs one thread per net, weighted_average_wirelength_cuda_
+ * merged_kernel.cu:16-85).  Host routine: packs whole nets, in order, into
+ * groups of at most 64 pins (groups[2g], groups[2g+1] = first net, one past
+ * the last) and lists nets of more than 64 pins in big_nets.  Capacities:
+ * num_nets groups and num_nets big nets.  Plain host memory; no GPU work. */
+int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* groups,
+                    int64_t* num_groups, int32_t* big_nets, int64_t* num_big);
+
+size_t dp_wa_pins_workspace_bytes(int64_t num_physical_nodes, int dtype);
+
+/* The chain WA(PinPos(pos)) with one wavefront per net group and one lane per
+ * pin.  Outputs partial_wl (nullable) and *wl_out as dp_wa_merged_forward, and
+ * grad_node_order[2P]: pin p's derivative (x, y interleaved), times its net
+ * weight, 0 for masked nets and pins, stored at index pin_node_order[p] (the
+ * pin's position in flat_node2pin_map). */
+int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_physical_nodes,
+                       const void* pin_offset_x, const void* pin_offset_y,
+                       const int32_t* pin2node_map, const int32_t* flat_netpin,
+                       const int32_t* netpin_start, const int32_t* pin2net_map,
+                       const int32_t* groups, int64_t num_groups, const int32_t* big_nets,
+                       int64_t num_big, const void* net_weights, const uint8_t* net_mask,
+                       const uint8_t* pin_mask, const int32_t* pin_node_order,
+                       const void* inv_gamma, void* partial_wl, void* grad_node_order,
+                       void* wl_out, void* workspace, void* stream);
+
+/* Backward of that chain plus the gradient combine: grad_pos[n] =
+ * (*grad) * (sum of grad_node_order over node n's CSR range) + add[n]
+ * (add nullable), read contiguously; fillers get add only. */
+int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,
+                           const int32_t* flat_node2pin_start_map, const void* add,
+                           int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
+                           void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
  * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64

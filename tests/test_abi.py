@@ -44,8 +44,8 @@ def test_fixed_point_scale_matches_oracle(lib, oracle):
 def test_workspace_queries(lib):
     from dreamplace_amd import _native as N
     nb = 1024 * 1024
-    b0 = N.query("dp_density_map_workspace_bytes", 1024, 1024, 1000, 0)
-    b1 = N.query("dp_density_map_workspace_bytes", 1024, 1024, 1000, 1000)
+    b0 = N.query("dp_density_map_workspace_bytes", 1024, 1024, 1000, 0, 0, 0)
+    b1 = N.query("dp_density_map_workspace_bytes", 1024, 1024, 1000, 1000, 0, 0)
     assert b0 >= 8 * nb and b1 - b0 >= 4 * 1000 - 256
     assert N.query("dp_spectral_plan_bytes", 1024, 512, N.DP_SPECTRAL_FP32) == 4 * 3 * (1024 ** 2 + 512 ** 2)
     assert N.query("dp_spectral_plan_bytes", 64, 64, N.DP_SPECTRAL_FP64) == 8 * 6 * 64 * 64

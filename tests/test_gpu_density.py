@@ -26,15 +26,17 @@ def _op(d, t, dtype, convention="cpu", algorithm="global"):
         convention=convention, algorithm=algorithm)
 
 
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("convention", ["cpu", "gpu"])
 @pytest.mark.parametrize("case", CASES, ids=[c[2] for c in CASES])
-def test_density_map_bit_exact(oracle, gpu, case, dtype, convention):
+def test_density_map_bit_exact(oracle, gpu, case, dtype, convention, algorithm):
     cfg, kw, name = case
-    print("case density_map %s %s %s" % (name, np.dtype(dtype).name, convention), flush=True)
+    print("case density_map %s %s %s %s" % (name, np.dtype(dtype).name, convention, algorithm),
+          flush=True)
     d = design(cfg, dtype=dtype, **kw)
     t = to_dev(d, gpu)
-    op = _op(d, t, t["pos"].dtype, convention)
+    op = _op(d, t, t["pos"].dtype, convention, algorithm)
     op.compute_initial_density_map(t["pos"])
     raw = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
     dmap = op.density_map(t["pos"], fixed_map_out=raw)
@@ -91,14 +93,18 @@ def test_overflow_metric(oracle, gpu, dtype):
     assert abs(mx.item() - m.max() / a) <= 1e-6 * (m.max() / a)
 
 
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)], ids=["f64", "f32"])
 @pytest.mark.parametrize("case", CASES, ids=[c[2] for c in CASES])
-def test_electric_force(oracle, gpu, case, dtype, tol):
+def test_electric_force(oracle, gpu, case, dtype, tol, algorithm):
     cfg, kw, name = case
-    print("case electric_force %s %s" % (name, np.dtype(dtype).name), flush=True)
+    print("case electric_force %s %s %s" % (name, np.dtype(dtype).name, algorithm), flush=True)
     d = design(cfg, dtype=dtype, **kw)
     t = to_dev(d, gpu)
-    op = _op(d, t, t["pos"].dtype)
+    op = _op(d, t, t["pos"].dtype, algorithm=algorithm)
+    # the tiled gather reuses the binning of the density call at the same positions
+    op.compute_initial_density_map(t["pos"])
+    op.density_map(t["pos"])
     rng = np.random.default_rng(7)
     fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
     fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)

@@ -9,20 +9,22 @@ from _util import design, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _objective(d, dev, lam, fused, precision, dtype=None):
+def _objective(d, dev, lam, fused, precision, dtype=None, algorithm="global", schedule=None):
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
     dd = DeviceDesign(d, dev, dtype=dtype)
-    return dd, PlacementObjective(dd, d.wa_gamma, density_weight=lam, precision=precision, fused=fused)
+    return dd, PlacementObjective(dd, d.wa_gamma, density_weight=lam, precision=precision, fused=fused,
+                                  algorithm=algorithm, schedule=schedule)
 
 
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
 @pytest.mark.parametrize("cfg,kw", [(0, {}), (4, {"scale": 0.003, "max_fillers": 100000})],
                          ids=["config0", "config4-small"])
-def test_engine_fp64_matches_oracle(oracle, gpu, cfg, kw, fused):
-    print("case engine fp64 cfg%d fused=%s" % (cfg, fused), flush=True)
+def test_engine_fp64_matches_oracle(oracle, gpu, cfg, kw, fused, algorithm):
+    print("case engine fp64 cfg%d fused=%s %s" % (cfg, fused, algorithm), flush=True)
     d = design(cfg, **kw)
     lam = 0.25
-    dd, obj = _objective(d, gpu, lam, fused, "fp64")
+    dd, obj = _objective(d, gpu, lam, fused, "fp64", algorithm=algorithm)
     f, g = obj.evaluate(dd.pos)
     rf, rg, parts = oracle.objective_and_gradient(d, lam, d.wa_gamma)
     assert abs(f.item() - rf) <= 1e-10 * abs(rf)
@@ -74,3 +76,39 @@ def test_engine_matches_reference_cpu_ops(gpu):
                           det["density_map"].numpy())
     assert abs(f.item() - rf.item()) <= 1e-10 * abs(rf.item())
     assert rel_l2(g.cpu().numpy(), rg.numpy()) <= 1e-10
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 2e-5)], ids=["f64", "f32"])
+@pytest.mark.parametrize("cfg,kw", [(0, {"big_nets": 40}), (1, {"scale": 0.003}),
+                                    (4, {"scale": 0.003, "max_fillers": 100000, "big_nets": 10})],
+                         ids=["config0-bignets", "config1-small", "config4-small"])
+def test_engine_pins_schedule(oracle, gpu, cfg, kw, dtype, tol):
+    """Pin-parallel WA + node-order backward + tiled density against the FP64
+    oracle (FP64 solve in both forms, so only the data type differs)."""
+    print("case engine pins cfg%d %s" % (cfg, np.dtype(dtype).name), flush=True)
+    d = design(cfg, dtype=dtype, **kw)
+    lam = 0.25
+    dd, obj = _objective(d, gpu, lam, True, "fp64", algorithm="tiled", schedule="pins")
+    f, g = obj.evaluate(dd.pos)
+    d64 = design(cfg, **kw)
+    rf, rg, parts = oracle.objective_and_gradient(d64, lam, d64.wa_gamma)
+    assert abs(f.item() - rf) <= tol * abs(rf)
+    assert rel_l2(obj.partial_wl.double().cpu().numpy(),
+                  oracle.wa_merged_forward(oracle.pin_pos_forward(d64.pos, d64.pin_offset_x, d64.pin_offset_y,
+                                                                  d64.pin2node_map),
+                                           d64.flat_net2pin_map, d64.flat_net2pin_start_map, d64.net_weights,
+                                           d64.net_mask_ignore_large_degrees, d64.wa_gamma)[1]) <= tol
+    assert rel_l2(g.double().cpu().numpy(), rg) <= tol
+
+
+def test_engine_pins_equals_unfused_fp64(gpu):
+    """The pin-parallel schedule keeps the per-pin arithmetic and the CSR
+    summation order: FP64 gradients equal the reference-order schedule."""
+    print("case engine pins == unfused", flush=True)
+    d = design(0, big_nets=20)
+    dd1, o1 = _objective(d, gpu, 0.3, True, "fp64", algorithm="tiled", schedule="pins")
+    dd2, o2 = _objective(d, gpu, 0.3, False, "fp64")
+    f1, g1 = o1.evaluate(dd1.pos)
+    f2, g2 = o2.evaluate(dd2.pos)
+    assert rel_l2(g1.cpu().numpy(), g2.cpu().numpy()) <= 1e-14
+    assert abs(f1.item() - f2.item()) <= 1e-13 * abs(f2.item())
