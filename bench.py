@@ -7,14 +7,23 @@ dreamplace/PlaceObj.py:394-411) on the config-1 workload: 2.2M movable std
 cells, 25k fixed blocks, ~10.9M fillers, 2.2M nets / ~9.6M pins, 1024x1024
 bins, synthetic, all inputs resident in HBM.  The timed form is FP32 with the
 spectral solve in BF16x3 (split bf16 on the matrix cores, FP32 accumulate);
-the same steps are also timed in FP64 for the speedup, and the FP32/BF16x3
-fields and gradient are checked against the FP64 run (relative L2).
+the same steps are also timed in FP64 for the speedup.
+
+Accuracy: the density map is bit-exact with the reference's deterministic
+FP32 map, so the precision forms are compared on that same map: FP64 solve
+vs the FP32/BF16x3 solve (potential, field) and the per-cell density
+gradient the FP64 gather forms from either field (l2err_solve_*, the gate).
+The FP32-form vs FP64-form differences of a whole step are reported too
+(l2err_step_*); they include the reference's own FP32 scatter rounding.
+
+Roofline: each op of the step (one C-ABI entry point, all of its kernels) is
+timed live with HIP events recorded by the library around its launches on
+its stream; achieved GB/s = the op's algorithmic bytes / its duration.
 
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
 import json
-import math
 import os
 import statistics
 import sys
@@ -27,46 +36,55 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms per density+WA fwd+bwd iter at 2.2M cells, FP32/BF16x3 vs FP64 GPU, at L2err<=1e-5"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+MFMA_PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp64": 78.6}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_config1.json")
+
+# probe ids (include/dreamplace_amd.h)
+DP_OP_DENSITY_MAP, DP_OP_ELECTRIC_FORCE, DP_OP_WA_FORWARD, DP_OP_NODE_GRAD, DP_OP_POISSON = \
+    16, 17, 18, 19, 20
+
+# kernels of each op in the pins schedule with tiled density (rocprof names)
+OP_KERNELS = {
+    "wa_forward": ["k_pin_push", "k_wa_pins", "k_wa_big", "k_finish"],
+    "density_map": ["k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill", "k_tile_scatter",
+                    "k_rec_scatter_large", "k_convert_cpu"],
+    "poisson": ["k_sum_partial", "k_scale_centre", "k_gemm_bf16x3", "k_spectral_weights"],
+    "electric_force": ["k_fill", "k_tile_gather", "k_rec_gather_large"],
+    "node_grad": ["k_node_grad"],
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(d, kernel, schedule="pins", algorithm="tiled"):
-    """Algorithmic HBM bytes of one launch of `kernel` (FP32 data, int32
-    indices): every array element the kernel must read or write once."""
+def op_bytes(d, schedule):
+    """Algorithmic HBM bytes of one call of each HBM-bound op, from the
+    per-unit figures of the hot-path analysis (FP32 data, int32 indices):
+    pin_pos fwd 28 B/pin; WA fwd (stored derivative) 20 B/pin + 13 B/net;
+    scatter 28 B/cell + 20 B/bin; force 36 B/cell + 8 B/bin; WA bwd fused
+    with pin_pos bwd 20 B/pin + 8 B/physical node, plus 16 B/node for adding
+    the density gradient into the node gradient."""
     Ns = d.num_movable_nodes + d.num_filler_nodes
     B = d.num_bins_x * d.num_bins_y
     P, E, n, nphys = d.num_pins, d.num_nets, d.num_nodes, d.num_physical_nodes
-    if kernel == "density_scatter":
-        if algorithm == "tiled":  # one 24-byte record per cell in, int64 map in + out
-            return 24 * Ns + 16 * B
-        return 28 * Ns + 16 * B  # 7 cell arrays in, int64 map in + out
-    if kernel == "electric_force":
-        if algorithm == "tiled":  # record in, 2 gradients out, both FP32 field maps in
-            return 24 * Ns + 8 * Ns + 8 * B
-        return 28 * Ns + 8 * n + 8 * B
-    if kernel == "wa_forward":
-        if schedule == "pins":
-            # per pin: netpin index, pin2net, pin2node, 2 offsets, gathered (x,y),
-            # pin mask, node-order index, (gx,gy) out; per net: start, mask, weight, WL
-            return 41 * P + 13 * E
-        return 32 * P + 13 * E
-    if kernel == "wa_pin_pos_backward":
-        if schedule == "pins":  # (gx,gy) per pin in node order; start per node; add in, grad out
-            return 8 * P + 4 * nphys + 16 * n
-        return 21 * P + 4 * nphys + 16 * n
-    raise KeyError(kernel)
+    wa = 20 * P + 13 * E
+    if schedule in ("pins", "fused"):
+        wa += 28 * P  # pin positions are formed inside the op
+    ops = {"wa_forward": wa, "density_map": 28 * Ns + 20 * B, "electric_force": 36 * Ns + 8 * B}
+    if schedule in ("pins", "fused"):
+        ops["node_grad"] = 20 * P + 8 * nphys + 16 * n
+    return ops
 
 
-KERNELS = {  # name -> probe id (include/dreamplace_amd.h)
-    "density_scatter": 0,
-    "electric_force": 1,
-    "wa_forward": 2,
-    "wa_pin_pos_backward": 5,
-}
+def poisson_flops(M, N):
+    """Dense-transform flops of one solve: 4 transforms of 2 GEMMs each."""
+    return 4 * 2 * (M * M * N + M * N * N)
+
+
+PROBES = {"wa_forward": DP_OP_WA_FORWARD, "density_map": DP_OP_DENSITY_MAP,
+          "poisson": DP_OP_POISSON, "electric_force": DP_OP_ELECTRIC_FORCE,
+          "node_grad": DP_OP_NODE_GRAD}
 
 
 def time_steps(torch, obj, pos, steps, warmup):
@@ -86,21 +104,67 @@ def time_steps(torch, obj, pos, steps, warmup):
     return wall / steps, per
 
 
-def probe_kernel(torch, N, obj, pos, kid, reps):
-    """Average duration (ms) of one kernel over `reps` evaluations, timed with
-    HIP events the library records around that launch on its stream."""
+def probe(torch, N, obj, pos, pid, reps):
+    """Average duration (ms) of one kernel or op over `reps` evaluations,
+    timed with HIP events the library records around its launches on the
+    stream they go to."""
     evs = []
     for _ in range(reps):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record()  # creates the underlying hipEvent_t; the library re-records it
         b.record()
-        N.call("dp_profile_kernel", kid, a.cuda_event, b.cuda_event)
+        N.call("dp_profile_kernel", pid, a.cuda_event, b.cuda_event)
         obj.evaluate(pos)
         evs.append((a, b))
     N.call("dp_profile_kernel", -1, None, None)
     torch.cuda.synchronize()
     return statistics.mean(a.elapsed_time(b) for a, b in evs)
+
+
+def measured_traffic(op):
+    """HBM bytes per call of `op` from the committed counter summary of the
+    same workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a plain copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ops = t.get("ops", {})
+    return ops.get(op)
+
+
+def rel(torch, a, b):
+    a = a.double().reshape(-1)
+    b = b.double().reshape(-1)
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+def accuracy(torch, obj32, obj64, dd64):
+    """Solve-only errors on the same (bit-exact) density map, and whole-step
+    FP32-form vs FP64-form differences.  Both objectives were last evaluated
+    at the same positions."""
+    ep = obj64.density
+    m = obj32.density_map.double()
+    fx64, fy64, phi64, _ = obj64.plan.poisson(m, ep.bin_size_x, ep.bin_size_y)
+    one = torch.ones((), dtype=torch.float64, device=m.device)
+    pos = dd64.pos
+    g64 = ep.electric_force(pos, fx64, fy64, one)
+    g32 = ep.electric_force(pos, obj32.field_x.double(), obj32.field_y.double(), one)
+    both = lambda a, b: torch.cat([a.reshape(-1), b.reshape(-1)])
+    out = {
+        "l2err_solve_potential": rel(torch, obj32.potential, phi64),
+        "l2err_solve_field": rel(torch, both(obj32.field_x, obj32.field_y), both(fx64, fy64)),
+        "l2err_solve_density_grad": rel(torch, g32, g64),
+        "l2err_step_field": rel(torch, both(obj32.field_x, obj32.field_y),
+                                both(obj64.field_x, obj64.field_y)),
+        "l2err_step_density_grad": rel(torch, obj32.density_grad, obj64.density_grad),
+        "l2err_step_grad": rel(torch, obj32.grad, obj64.grad),
+    }
+    out["l2err_gate"] = max(out["l2err_solve_potential"], out["l2err_solve_field"],
+                            out["l2err_solve_density_grad"])
+    return out
 
 
 def cpu_baseline(d, budget_s):
@@ -144,6 +208,9 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB -> 1 GiB device copy kernel 5 times (the known-size "
+                         "reference for FETCH_SIZE / WRITE_SIZE counter scaling) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported; this build runs on one MI355X "
@@ -151,6 +218,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        n = (1 << 30) // 4
+        a = torch.rand(n, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.mul(a, 1.0, out=b)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy": {"read_bytes": 4 * n, "write_bytes": 4 * n, "launches": 5}}))
+        return
+
     from dreamplace_amd import _native as N
     from dreamplace_amd import synth
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
@@ -167,35 +244,47 @@ def main():
                                algorithm=args.algorithm, schedule=args.schedule)
     pos32 = dd32.pos
     ms32, per32 = time_steps(torch, obj32, pos32, args.steps, args.warmup)
-    log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32, statistics.median(per32)))
+    log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32,
+                                                                statistics.median(per32)))
 
-    # kernel probes: find the dominant kernel, then time it live
-    kms = {k: probe_kernel(torch, N, obj32, pos32, kid, 3) for k, kid in KERNELS.items()}
-    dom = max(kms, key=kms.get)
-    dom_ms = probe_kernel(torch, N, obj32, pos32, KERNELS[dom], max(args.steps, 10))
-    dom_bytes = algorithmic_bytes(d, dom, args.schedule, args.algorithm)
+    # per-op live timing
+    hbm_bytes = op_bytes(d, args.schedule)
+    names = [k for k in PROBES if k in hbm_bytes or k == "poisson"]
+    op_ms = {k: probe(torch, N, obj32, pos32, PROBES[k], 3) for k in names}
+    dom = max(hbm_bytes, key=lambda k: op_ms[k])
+    dom_ms = probe(torch, N, obj32, pos32, PROBES[dom], max(args.steps, 10))
+    op_ms[dom] = dom_ms
+    log("[bench] op ms: " + ", ".join("%s %.3f" % kv for kv in op_ms.items()))
+    ops = {}
+    for k in names:
+        if k == "poisson":
+            fl = poisson_flops(d.num_bins_x, d.num_bins_y)
+            tf = fl / (op_ms[k] * 1e-3) / 1e12
+            ops[k] = {"ms": round(op_ms[k], 4), "bound": "mfma", "flops": fl,
+                      "achieved_tflops": round(tf, 1),
+                      "frac_of_fp32_peak": round(tf / MFMA_PEAK_TFLOPS["fp32"], 4)}
+            if args.precision == "bf16x3":  # three bf16 products per FP32 product
+                ops[k]["frac_of_bf16_peak"] = round(3 * tf / MFMA_PEAK_TFLOPS["bf16x3"], 4)
+        else:
+            gbs = hbm_bytes[k] / (op_ms[k] * 1e-3) / 1e9
+            ops[k] = {"ms": round(op_ms[k], 4), "bound": "hbm", "algorithmic_bytes": hbm_bytes[k],
+                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom_bytes = hbm_bytes[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    log("[bench] kernel ms: " + ", ".join("%s %.3f" % kv for kv in kms.items()))
 
     result = {}
     if not args.skip_fp64:
-        obj32.evaluate(pos32)
-        g32 = obj32.grad.double().clone()
-        fx32 = obj32.field_x.double().clone()
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
                                    algorithm=args.algorithm, schedule=args.schedule)
-        ms64, per64 = time_steps(torch, obj64, dd64.pos, max(args.steps // 2, 5), 2)
+        ms64, _ = time_steps(torch, obj64, dd64.pos, max(args.steps // 2, 5), 2)
         log("[bench] fp64: %.3f ms/step" % ms64)
         obj64.evaluate(dd64.pos)
-        # density gradient alone (the WA part is identical up to FP32 rounding)
         obj32.evaluate(pos32)
-        dg32 = obj32.density_grad.double()
-        dg64 = obj64.density_grad
-        rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
-        result.update(fp64_ms_per_step=round(ms64, 4), speedup_vs_fp64=round(ms64 / ms32, 3),
-                      l2err_field_x=rel(fx32, obj64.field_x), l2err_density_grad=rel(dg32, dg64),
-                      l2err_grad=rel(g32, obj64.grad))
+        result.update(fp64_ms_per_step=round(ms64, 4), speedup_vs_fp64=round(ms64 / ms32, 3))
+        result.update(accuracy(torch, obj32, obj64, dd64))
+        log("[bench] accuracy: " + ", ".join("%s %.2e" % (k, v) for k, v in result.items()
+                                               if k.startswith("l2err")))
         del obj64, dd64
         torch.cuda.empty_cache()
 
@@ -221,17 +310,17 @@ def main():
                            d.num_pins, d.num_bins_x, d.num_bins_y),
             "form": "fp32 + %s spectral solve, %s schedule, %s density" % (
                 args.precision, args.schedule, args.algorithm),
-            "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
+            "ops": ops,
             **result,
         },
         "roofline": {
-            "kernel": dom,
+            "kernel": "%s op (%s)" % (dom, ", ".join(OP_KERNELS.get(dom, []))),
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom) if args.config == 1 and args.scale == 1.0 else None,
             "kernel_ms": round(dom_ms, 4),
             "algorithmic_bytes": dom_bytes,
         },

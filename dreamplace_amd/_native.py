@@ -19,6 +19,7 @@ DP_INDEX_INT32, DP_INDEX_INT64 = 0, 1
 DP_CONVERT_CPU, DP_CONVERT_GPU = 0, 1
 DP_SPECTRAL_FP64, DP_SPECTRAL_FP32, DP_SPECTRAL_BF16X3 = 0, 1, 2
 DP_SCATTER_GLOBAL, DP_SCATTER_TILED = 0, 1
+DP_LAYOUT_SPLIT, DP_LAYOUT_INTERLEAVED = 0, 1
 
 PRECISIONS = {"fp64": DP_SPECTRAL_FP64, "fp32": DP_SPECTRAL_FP32, "bf16x3": DP_SPECTRAL_BF16X3}
 
@@ -39,7 +40,7 @@ _SIGNATURES = {
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
-                               _I, _I, _I, _L, _P, _P, _P]),
+                               _I, _I, _I, _L, _I, _P, _P, _P]),
     "dp_reduce_workspace_bytes": (_S, [_L]),
     "dp_density_overflow": (_I, [_I, _P, _L, _D, _P, _P, _P]),
     "dp_spectral_plan_bytes": (_S, [_I, _I, _I]),
@@ -60,11 +61,14 @@ _SIGNATURES = {
                                       _P, _P, _P, _P]),
     "dp_wa_backward_to_nodes": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_profile_kernel": (_I, [_I, _P, _P]),
-    "dp_wa_pack_nets": (_I, [_P, _L, _P, _P, _P, _P]),
+    "dp_wa_pack_nets": (_I, [_P, _L, _P, _P, _P, _P, _P]),
+    "dp_wa_prepare_workspace_bytes": (_S, [_L]),
+    "dp_wa_pins_prepare": (_I, [_I, _L, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                _P, _P]),
     "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
-    "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _P, _P, _L, _P, _L, _P, _P,
-                                _P, _P, _P, _P, _P, _P, _P, _P]),
-    "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),
+    "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
+                                _P, _P, _P, _P, _P]),
+    "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),

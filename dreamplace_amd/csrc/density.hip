@@ -317,6 +317,7 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
          "dp_density_map: bad algorithm %d", algorithm);
   DP_ARG(density_map_out || fixed_map_out, "dp_density_map: no output");
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_DENSITY_MAP, s);
   const int64_t nb = (int64_t)num_bins_x * num_bins_y;
   const int64_t scale = dp_fixed_point_scale(xl, yl, xh, yh);
   DensityWorkspace w = carve(workspace, nb, large_capacity);

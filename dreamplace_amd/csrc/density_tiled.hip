@@ -1,57 +1,83 @@
 // LDS-binned density scatter and field gather (DP_SCATTER_TILED).
 //
-// Cells are binned by the 32x32-bin tile holding their lower-left bin:
+// Cells are binned by the 64x64-bin tile holding their lower-left bin:
 //   1. k_bin_count  each workgroup histograms a fixed chunk of cells over the
 //                   tiles in LDS, then reserves one range per (workgroup,
 //                   tile) with a single global atomic;
 //   2. k_bin_scan   one workgroup turns tile counts into tile offsets and a
 //                   list of work items (a tile's cells, at most kChunk each);
-//   3. k_bin_fill   the same chunks again: each cell gets its slot from an
-//                   LDS cursor and writes a packed record (origin, stretched
-//                   size, ratio, node index) in tile order;
+//   3. k_bin_fill   the same chunks again, in sub-batches staged in LDS: each
+//                   sub-batch is counting-sorted by tile in LDS and written
+//                   out as one contiguous run per tile.  Records are a
+//                   structure of arrays (stretched origin, stretched size,
+//                   ratio, node index);
 //   4. k_tile_scatter  one workgroup per work item accumulates its cells'
-//                   int64 contributions in a (32+2)^2 LDS tile (ds_add_u64),
+//                   int64 contributions in a (64+2)^2 LDS tile (ds_add_u64),
 //                   then adds the non-zero bins to the global map with one
 //                   atomic each.
 // The field gather (k_tile_gather) reuses the records of the same positions:
-// a workgroup stages the tile's (32+2)^2 field values in LDS and each cell
+// a workgroup stages the tile's (64+2)^2 (Ex, Ey) pairs in LDS and each cell
 // reads its <=3x3 bins from there.
-// Cells spanning more than 3x3 bins get their records at the tail of the
-// record array (slot total-1-q, q from a global counter) and are done one
-// workgroup (scatter) or one wavefront (gather) per cell.  Small plus large
-// cells never exceed the record count, so nothing can overflow.
-// The int64 sum is order-free, so the map is bit-identical to the
-// per-cell-atomic scatter.
+//
+// Every permutation here is a push: scattered writes cost a 32-byte sector
+// on this memory system while scattered reads cost a whole line, so records
+// are written (in runs) to tile order and the per-cell gradients are written
+// back to node order, one 8/16-byte store per cell in the interleaved layout.
+//
+// Cells spanning more than 3x3 bins, or no bin at all, get their records at
+// the tail of the record arrays (slot total-1-q, q from a global counter) and
+// are done one workgroup (scatter) or one wavefront (gather) per cell.  Tile
+// plus tail records never exceed the record count, so nothing can overflow.
+// The int64 sum is order-free, so the map is bit-identical to the per-cell
+// global-atomic scatter.
 #include "dp_density.h"
 
 #pragma clang fp contract(off)
 
 namespace dp {
 
-constexpr int kTile = 32;                 // bins per tile side
+constexpr int kTile = 64;                 // bins per tile side
 constexpr int kTileH = kTile + 2;         // with the two-bin halo of a 3x3 span
-constexpr int kMaxTiles = 8192;           // LDS histogram limit (32 KB)
+constexpr int kMaxTiles = 2048;           // LDS histogram limit
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
-constexpr int kChunk = 2048;              // cells per scatter/gather work item
+constexpr int kFillThreads = 1024;        // threads of a fill workgroup
+constexpr int kChunk = 8192;              // cells per scatter/gather work item
 
+// Cells per LDS-staged fill sub-batch: 28 (FP32) or 48 (FP64) bytes of LDS
+// per cell, ~115 KB (FP32) / ~100 KB (FP64) in all with the tile arrays.
 template <typename T>
-struct alignas(sizeof(T) == 4 ? 8 : 16) CellRec {
-  T x, y, sx, sy, r;
-  int idx;
+struct FillSub {
+  static constexpr int value = sizeof(T) == 4 ? 4096 : 2048;
+};
+
+// Records: one array per field, `total` entries each.
+template <typename T>
+struct Recs {
+  T *x, *y, *sx, *sy, *r;
+  int* idx;
+  __device__ __forceinline__ void put(int64_t j, T x_, T y_, T sx_, T sy_, T r_, int i) const {
+    x[j] = x_;
+    y[j] = y_;
+    sx[j] = sx_;
+    sy[j] = sy_;
+    r[j] = r_;
+    idx[j] = i;
+  }
 };
 
 struct TiledLayout {
   int ntx, nty, ntiles;
   int64_t max_work;
-  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec, total;
+  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec[5], off_idx, total;
 };
 
-static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t rec_bytes) {
+static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) {
   TiledLayout L;
   L.ntx = (int)ceil_div(nbx, kTile);
   L.nty = (int)ceil_div(nby, kTile);
   L.ntiles = L.ntx * L.nty;
   L.max_work = ceil_div(ncells, kChunk) + L.ntiles + 1;
+  const size_t n = (size_t)(ncells > 0 ? ncells : 1);
   size_t o = 0;
   L.off_count = o;
   o += align256(sizeof(int) * (size_t)L.ntiles);
@@ -63,10 +89,26 @@ static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t rec_byt
   o += align256(sizeof(int4) * (size_t)L.max_work);
   L.off_nwork = o;
   o += 256;
-  L.off_rec = o;
-  o += align256(rec_bytes * (size_t)(ncells > 0 ? ncells : 1));
+  for (int f = 0; f < 5; ++f) {
+    L.off_rec[f] = o;
+    o += align256(tsize * n);
+  }
+  L.off_idx = o;
+  o += align256(sizeof(int) * n);
   L.total = o;
   return L;
+}
+
+template <typename T>
+static Recs<T> recs_of(char* ws, const TiledLayout& L) {
+  Recs<T> r;
+  r.x = (T*)(ws + L.off_rec[0]);
+  r.y = (T*)(ws + L.off_rec[1]);
+  r.sx = (T*)(ws + L.off_rec[2]);
+  r.sy = (T*)(ws + L.off_rec[3]);
+  r.r = (T*)(ws + L.off_rec[4]);
+  r.idx = (int*)(ws + L.off_idx);
+  return r;
 }
 
 bool tiled_supported(int nbx, int nby) {
@@ -74,8 +116,7 @@ bool tiled_supported(int nbx, int nby) {
 }
 
 size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype) {
-  size_t rb = dtype == DP_FLOAT64 ? sizeof(CellRec<double>) : sizeof(CellRec<float>);
-  return tiled_layout(nbx, nby, ncells, rb).total;
+  return tiled_layout(nbx, nby, ncells, dtype == DP_FLOAT64 ? 8 : 4).total;
 }
 
 // index t of the (movable, filler) space -> node index
@@ -83,39 +124,34 @@ __device__ __forceinline__ int64_t node_of(int64_t t, int64_t n_mov, int64_t fil
   return t < n_mov ? t : fill_begin + (t - n_mov);
 }
 
+// Tile of a cell whose span is 1..3 bins on both axes; -1 for a tail cell.
 template <typename T>
-__device__ __forceinline__ CellRec<T> make_rec(const CellBins<T>& c, int64_t i) {
-  CellRec<T> r;
-  r.x = c.x;
-  r.y = c.y;
-  r.sx = c.sx;
-  r.sy = c.sy;
-  r.r = c.r;
-  r.idx = (int)i;
-  return r;
+__device__ __forceinline__ int tile_key(const CellBins<T>& c, int nty) {
+  const int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
+  if (nx <= 0 || ny <= 0 || nx > 3 || ny > 3) return -1;
+  return (c.bxl / kTile) * nty + c.byl / kTile;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, int64_t fill_begin,
                                                    int64_t total, int64_t chunk, int nty, int ntiles,
-                                                   int* tile_count, int* wg_base, int* nlarge,
-                                                   CellRec<T>* rec) {
-  extern __shared__ int hist[];
+                                                   int* tile_count, int* wg_base, int* ntail,
+                                                   Recs<T> rec) {
+  __shared__ int hist[kMaxTiles];
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * chunk;
   const int64_t e = b + chunk < total ? b + chunk : total;
   for (int64_t t = b + threadIdx.x; t < e; t += blockDim.x) {
     const int64_t i = node_of(t, n_mov, fill_begin);
-    CellBins<T> c = cell_bins(a, i);
-    const int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
-    if (nx <= 0 || ny <= 0) continue;
-    if (nx > 3 || ny > 3) {
-      const int q = atomicAdd(nlarge, 1);
-      rec[total - 1 - q] = make_rec(c, i);
-      continue;
+    const CellBins<T> c = cell_bins(a, i);
+    const int k = tile_key(c, nty);
+    if (k < 0) {
+      const int q = atomicAdd(ntail, 1);
+      rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i);
+    } else {
+      atomicAdd(&hist[k], 1);
     }
-    atomicAdd(&hist[(c.bxl / kTile) * nty + c.byl / kTile], 1);
   }
   __syncthreads();
   int* my = wg_base + (size_t)blockIdx.x * ntiles;
@@ -168,43 +204,164 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_bin_fill(TriArgs<T> a, int64_t n_mov, int64_t fill_begin,
-                                                  int64_t total, int64_t chunk, int nty, int ntiles,
-                                                  const int* tile_start, const int* wg_base,
-                                                  CellRec<T>* rec) {
-  extern __shared__ int cursor[];
-  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cursor[i] = 0;
+// Exclusive scan of in[0, n) into out[0, n) by a workgroup of kFillThreads
+// (n <= 2 * kFillThreads).  wsum holds kFillThreads / 64 ints.  Ends with a
+// barrier.
+__device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, int n, int* wsum) {
+  constexpr int NW = kFillThreads / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int k0 = 2 * threadIdx.x;
+  const int v0 = k0 < n ? in[k0] : 0;
+  const int v1 = k0 + 1 < n ? in[k0 + 1] : 0;
+  const int s = v0 + v1;
+  int incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
   __syncthreads();
+  if (wid == 0) {
+    const int w = lane < NW ? wsum[lane] : 0;
+    int wi = w;
+#pragma unroll
+    for (int o = 1; o < NW; o <<= 1) {
+      const int u = __shfl_up(wi, o, 64);
+      if (lane >= o) wi += u;
+    }
+    if (lane < NW) wsum[lane] = wi - w;
+  }
+  __syncthreads();
+  const int base = wsum[wid] + incl - s;
+  if (k0 < n) out[k0] = base;
+  if (k0 + 1 < n) out[k0 + 1] = base + v0;
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t n_mov,
+                                                           int64_t fill_begin, int64_t total,
+                                                           int64_t chunk, int nty, int ntiles,
+                                                           const int* tile_start,
+                                                           const int* wg_base, Recs<T> rec) {
+  constexpr int S = FillSub<T>::value;
+  constexpr int PER = S / kFillThreads;
+  __shared__ int s_base[kMaxTiles];  // next record slot of (this workgroup, tile)
+  __shared__ int s_hist[kMaxTiles];  // sub-batch cells per tile
+  __shared__ int s_off[kMaxTiles];   // exclusive scan of s_hist
+  __shared__ int s_wsum[kFillThreads / 64];
+  __shared__ T s_x[S], s_y[S], s_sx[S], s_sy[S], s_r[S];
+  __shared__ int s_idx[S];
+  __shared__ unsigned short s_key[S];
   const int* my = wg_base + (size_t)blockIdx.x * ntiles;
+  for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] = tile_start[k] + my[k];
   const int64_t b = (int64_t)blockIdx.x * chunk;
   const int64_t e = b + chunk < total ? b + chunk : total;
-  for (int64_t t = b + threadIdx.x; t < e; t += blockDim.x) {
-    const int64_t i = node_of(t, n_mov, fill_begin);
-    CellBins<T> c = cell_bins(a, i);
-    const int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
-    if (nx <= 0 || ny <= 0 || nx > 3 || ny > 3) continue;
-    const int k = (c.bxl / kTile) * nty + c.byl / kTile;
-    const int r = atomicAdd(&cursor[k], 1);
-    rec[tile_start[k] + my[k] + r] = make_rec(c, i);
+  // the next sub-batch's cells are loaded while this one is sorted in LDS
+  RawCell<T> nxt[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int64_t t = b + m * kFillThreads + threadIdx.x;
+    if (t < e) nxt[m] = load_cell(a, node_of(t, n_mov, fill_begin));
+  }
+  // the trip count depends on blockIdx only: every thread meets every barrier
+  for (int64_t sb = b; sb < e; sb += S) {
+    RawCell<T> cur[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) cur[m] = nxt[m];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int64_t t = sb + S + m * kFillThreads + threadIdx.x;
+      if (t < e) nxt[m] = load_cell(a, node_of(t, n_mov, fill_begin));
+    }
+    for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_hist[k] = 0;
+    __syncthreads();
+    CellBins<T> c[PER];
+    int key[PER], rank[PER], node[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int64_t t = sb + m * kFillThreads + threadIdx.x;
+      key[m] = -1;
+      if (t < e) {
+        c[m] = bins_of(a, cur[m]);
+        node[m] = (int)node_of(t, n_mov, fill_begin);
+        key[m] = tile_key(c[m], nty);
+        if (key[m] >= 0) rank[m] = atomicAdd(&s_hist[key[m]], 1);
+      }
+    }
+    __syncthreads();
+    block_exclusive_scan(s_hist, s_off, ntiles, s_wsum);
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      if (key[m] >= 0) {
+        const int p = s_off[key[m]] + rank[m];
+        s_x[p] = c[m].x;
+        s_y[p] = c[m].y;
+        s_sx[p] = c[m].sx;
+        s_sy[p] = c[m].sy;
+        s_r[p] = c[m].r;
+        s_idx[p] = node[m];
+        s_key[p] = (unsigned short)key[m];
+      }
+    }
+    __syncthreads();
+    const int nsmall = s_off[ntiles - 1] + s_hist[ntiles - 1];
+    for (int p = threadIdx.x; p < nsmall; p += kFillThreads) {
+      const int k = s_key[p];
+      const int64_t g = (int64_t)s_base[k] + (p - s_off[k]);
+      rec.put(g, s_x[p], s_y[p], s_sx[p], s_sy[p], s_r[p], s_idx[p]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] += s_hist[k];
+    // s_hist[k] is read above and cleared at the next sub-batch by the same thread
   }
 }
 
 // Bin range of a record (same arithmetic as cell_bins).
 template <typename T>
-__device__ __forceinline__ void rec_bins(const TriArgs<T>& a, const CellRec<T>& c, int& bxl,
+__device__ __forceinline__ void rec_bins(const TriArgs<T>& a, T x, T y, T sx, T sy, int& bxl,
                                          int& nx, int& byl, int& ny) {
-  bxl = max((int)((c.x - a.xl) * a.inv_x), 0);
-  int bxh = min((int)((c.x + c.sx - a.xl) * a.inv_x) + 1, a.nbx);
-  byl = max((int)((c.y - a.yl) * a.inv_y), 0);
-  int byh = min((int)((c.y + c.sy - a.yl) * a.inv_y) + 1, a.nby);
+  bxl = max((int)((x - a.xl) * a.inv_x), 0);
+  int bxh = min((int)((x + sx - a.xl) * a.inv_x) + 1, a.nbx);
+  byl = max((int)((y - a.yl) * a.inv_y), 0);
+  int byh = min((int)((y + sy - a.yl) * a.inv_y) + 1, a.nby);
   nx = bxh - bxl;
   ny = byh - byl;
 }
 
+// Cells per thread per pass of the scatter / gather loops: their record
+// loads are issued together before any of them is used.
+constexpr int kBatch = 4;
+
+template <typename T>
+struct RecBatch {
+  T x[kBatch], y[kBatch], sx[kBatch], sy[kBatch], r[kBatch];
+  int idx[kBatch];
+  bool ok[kBatch];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_batch(const Recs<T>& rec, int j0, int stride, int end,
+                                           bool with_idx, RecBatch<T>& b) {
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) {
+    const int j = j0 + u * stride;
+    b.ok[u] = j < end;
+    if (b.ok[u]) {
+      b.x[u] = rec.x[j];
+      b.y[u] = rec.y[j];
+      b.sx[u] = rec.sx[j];
+      b.sy[u] = rec.sy[j];
+      b.r[u] = rec.r[j];
+      if (with_idx) b.idx[u] = rec.idx[j];
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, const int4* work,
-                                                      const int* nwork, const CellRec<T>* rec,
+                                                      const int* nwork, Recs<T> rec,
                                                       unsigned long long* map) {
   __shared__ unsigned long long acc[kTileH * kTileH];
   const int W = *nwork;
@@ -213,26 +370,32 @@ __global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, con
     const int tx0 = (it.x / nty) * kTile, ty0 = (it.x % nty) * kTile;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) acc[k] = 0ull;
     __syncthreads();
-    for (int j = it.y + threadIdx.x; j < it.z; j += blockDim.x) {
-      const CellRec<T> c = rec[j];
-      int bxl, nx, byl, ny;
-      rec_bins(a, c, bxl, nx, byl, ny);
-      T pxr[3], py[3];
+    for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
+      RecBatch<T> b;
+      load_batch(rec, j0, blockDim.x, it.z, false, b);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        pxr[q] = tri(c.x, c.sx, a.xl, bxl + q, a.bsx) * c.r;
-        py[q] = tri(c.y, c.sy, a.yl, byl + q, a.bsy);
-      }
-      const int lx = bxl - tx0, ly = byl - ty0;
+      for (int u = 0; u < kBatch; ++u) {
+        if (!b.ok[u]) continue;
+        const T x = b.x[u], y = b.y[u], sx = b.sx[u], sy = b.sy[u], r = b.r[u];
+        int bxl, nx, byl, ny;
+        rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+        T pxr[3], py[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        if (q < nx) {
+        for (int q = 0; q < 3; ++q) {
+          pxr[q] = tri(x, sx, a.xl, bxl + q, a.bsx) * r;
+          py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
+        }
+        const int lx = bxl - tx0, ly = byl - ty0;
 #pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            if (l < ny) {
-              T area = pxr[q] * py[l];
-              long long v = (long long)(area * a.scale);
-              if (v != 0) atomicAdd(&acc[(lx + q) * kTileH + (ly + l)], (unsigned long long)v);
+        for (int q = 0; q < 3; ++q) {
+          if (q < nx) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              if (l < ny) {
+                T area = pxr[q] * py[l];
+                long long v = (long long)(area * a.scale);
+                if (v != 0) atomicAdd(&acc[(lx + q) * kTileH + (ly + l)], (unsigned long long)v);
+              }
             }
           }
         }
@@ -252,12 +415,12 @@ __global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, con
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_tile_gather(TriArgs<T> a, int nty, const int4* work,
-                                                     const int* nwork, const CellRec<T>* rec,
+                                                     const int* nwork, Recs<T> rec,
                                                      const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
-                                                     int64_t num_nodes, T* out) {
-  __shared__ T sfx[kTileH * kTileH];
-  __shared__ T sfy[kTileH * kTileH];
+                                                     int64_t num_nodes, int interleaved, T* out) {
+  typedef typename Vec2<T>::type V;
+  __shared__ V sf[kTileH * kTileH];
   const int W = *nwork;
   const T g = grad ? *grad : T(1);
   for (int w = blockIdx.x; w < W; w += gridDim.x) {
@@ -267,83 +430,99 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriArgs<T> a, int nty, cons
       const int gx = tx0 + k / kTileH, gy = ty0 + k % kTileH;
       const bool in = gx < a.nbx && gy < a.nby;
       const int64_t idx = (int64_t)gx * a.nby + gy;
-      sfx[k] = in ? fx[idx] : T(0);
-      sfy[k] = in ? fy[idx] : T(0);
+      V v;
+      v.x = in ? fx[idx] : T(0);
+      v.y = in ? fy[idx] : T(0);
+      sf[k] = v;
     }
     __syncthreads();
-    for (int j = it.y + threadIdx.x; j < it.z; j += blockDim.x) {
-      const CellRec<T> c = rec[j];
-      int bxl, nx, byl, ny;
-      rec_bins(a, c, bxl, nx, byl, ny);
-      T px[3], py[3];
+    for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
+      RecBatch<T> b;
+      load_batch(rec, j0, blockDim.x, it.z, true, b);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        px[q] = tri(c.x, c.sx, a.xl, bxl + q, a.bsx);
-        py[q] = tri(c.y, c.sy, a.yl, byl + q, a.bsy);
-      }
-      const int lx = bxl - tx0, ly = byl - ty0;
-      T gx = 0, gy = 0;
+      for (int u = 0; u < kBatch; ++u) {
+        if (!b.ok[u]) continue;
+        const T x = b.x[u], y = b.y[u], sx = b.sx[u], sy = b.sy[u];
+        int bxl, nx, byl, ny;
+        rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+        T px[3], py[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        if (q < nx) {
+        for (int q = 0; q < 3; ++q) {
+          px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
+          py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
+        }
+        const int lx = bxl - tx0, ly = byl - ty0;
+        T gx = 0, gy = 0;
 #pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            if (l < ny) {
-              T area = px[q] * py[l];
-              const int k = (lx + q) * kTileH + (ly + l);
-              gx += area * sfx[k];
-              gy += area * sfy[k];
+        for (int q = 0; q < 3; ++q) {
+          if (q < nx) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              if (l < ny) {
+                T area = px[q] * py[l];
+                const V f = sf[(lx + q) * kTileH + (ly + l)];
+                gx += area * f.x;
+                gy += area * f.y;
+              }
             }
           }
         }
+        const T r = b.r[u];
+        put_grad(out, num_nodes, b.idx[u], -((gx * r) * g), -((gy * r) * g), interleaved);
       }
-      out[c.idx] = -((gx * c.r) * g);
-      out[num_nodes + c.idx] = -((gy * c.r) * g);
     }
     __syncthreads();
   }
 }
 
-// Large cells (tail records): one workgroup per cell, lanes over its bins.
+// Tail records: one workgroup per cell, lanes over its bins.  Records with an
+// empty span contribute nothing.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, const CellRec<T>* rec,
-                                                           int64_t total, const int* nlarge,
+__global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T> rec,
+                                                           int64_t total, const int* ntail,
                                                            unsigned long long* map) {
-  const int cnt = *nlarge;
+  const int cnt = *ntail;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
-    const CellRec<T> c = rec[total - 1 - q];
+    const int64_t j = total - 1 - q;
+    const T x = rec.x[j], y = rec.y[j], sx = rec.sx[j], sy = rec.sy[j], r = rec.r[j];
     int bxl, nx, byl, ny;
-    rec_bins(a, c, bxl, nx, byl, ny);
+    rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+    if (nx <= 0 || ny <= 0) continue;
     for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
       const int k = bxl + t / ny, h = byl + t % ny;
-      T pxr = tri(c.x, c.sx, a.xl, k, a.bsx) * c.r;
-      T area = pxr * tri(c.y, c.sy, a.yl, h, a.bsy);
+      T pxr = tri(x, sx, a.xl, k, a.bsx) * r;
+      T area = pxr * tri(y, sy, a.yl, h, a.bsy);
       long long v = (long long)(area * a.scale);
       if (v != 0) atomicAdd(&map[(int64_t)k * a.nby + h], (unsigned long long)v);
     }
   }
 }
 
-// Large cells (tail records): one wavefront per cell, fixed-order reduction.
+// Tail records: one wavefront per cell, fixed-order reduction; a cell with an
+// empty span gets a zero gradient.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, const CellRec<T>* rec,
-                                                          int64_t total, const int* nlarge,
+__global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> rec,
+                                                          int64_t total, const int* ntail,
                                                           const T* __restrict__ fx,
                                                           const T* __restrict__ fy, const T* grad,
-                                                          int64_t num_nodes, T* out) {
-  const int cnt = *nlarge;
+                                                          int64_t num_nodes, int interleaved,
+                                                          T* out) {
+  const int cnt = *ntail;
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const T g = grad ? *grad : T(1);
   for (int q = wave; q < cnt; q += nwaves) {
-    const CellRec<T> c = rec[total - 1 - q];
+    const int64_t j = total - 1 - q;
+    const T x = rec.x[j], y = rec.y[j], sx = rec.sx[j], sy = rec.sy[j], r = rec.r[j];
     int bxl, nx, byl, ny;
-    rec_bins(a, c, bxl, nx, byl, ny);
+    rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+    const bool empty = nx <= 0 || ny <= 0;
+    const int nbins = empty ? 0 : nx * ny;
     T gx = 0, gy = 0;
-    for (int t = lane; t < nx * ny; t += 64) {
+    for (int t = lane; t < nbins; t += 64) {
       const int k = bxl + t / ny, h = byl + t % ny;
-      T area = tri(c.x, c.sx, a.xl, k, a.bsx) * tri(c.y, c.sy, a.yl, h, a.bsy);
+      T area = tri(x, sx, a.xl, k, a.bsx) * tri(y, sy, a.yl, h, a.bsy);
       const int64_t idx = (int64_t)k * a.nby + h;
       gx += area * fx[idx];
       gy += area * fy[idx];
@@ -351,28 +530,11 @@ __global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, const Ce
     gx = wave_sum(gx);
     gy = wave_sum(gy);
     if (lane == 0) {
-      out[c.idx] = -((gx * c.r) * g);
-      out[num_nodes + c.idx] = -((gy * c.r) * g);
+      if (empty)
+        put_grad(out, num_nodes, rec.idx[j], T(0), T(0), interleaved);
+      else
+        put_grad(out, num_nodes, rec.idx[j], -((gx * r) * g), -((gy * r) * g), interleaved);
     }
-  }
-}
-
-// Zero the gradient of the nodes no record covers: terminals, and movable or
-// filler cells entirely outside the bin grid (span <= 0).  Cells with a
-// record or in the large queue are written by the gather kernels.
-template <typename T>
-__global__ void k_zero_uncovered(TriArgs<T> a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
-                                 T* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= num_nodes) return;
-  bool zero = i >= n_mov && i < num_nodes - n_fill;
-  if (!zero) {
-    CellBins<T> c = cell_bins(a, i);
-    zero = (c.bxh - c.bxl) <= 0 || (c.byh - c.byl) <= 0;
-  }
-  if (zero) {
-    out[i] = 0;
-    out[num_nodes + i] = 0;
   }
 }
 
@@ -380,7 +542,7 @@ template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
                   char* tiled_ws, unsigned long long* map, hipStream_t s) {
   const int64_t total = n_mov + n_fill;
-  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(CellRec<T>));
+  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled scatter: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
   int* tile_count = (int*)(tiled_ws + L.off_count);
@@ -388,32 +550,31 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
   int* wg_base = (int*)(tiled_ws + L.off_wgbase);
   int4* work = (int4*)(tiled_ws + L.off_work);
   int* nwork = (int*)(tiled_ws + L.off_nwork);
-  int* nlarge = nwork + 1;
-  CellRec<T>* rec = (CellRec<T>*)(tiled_ws + L.off_rec);
+  int* ntail = nwork + 1;
+  Recs<T> rec = recs_of<T>(tiled_ws, L);
   hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(L.ntiles, 256)), dim3(256), 0, s, tile_count,
                      (int64_t)L.ntiles, 0);
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, nwork, (int64_t)2, 0);
   DP_LAUNCHED("tiled init");
   if (total > 0) {
     const int64_t chunk = ceil_div(total, kBinGroups);
-    const size_t lds = sizeof(int) * (size_t)L.ntiles;
-    hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256), lds, s, a, n_mov, fill_begin,
-                       total, chunk, L.nty, L.ntiles, tile_count, wg_base, nlarge, rec);
+    hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256), 0, s, a, n_mov, fill_begin,
+                       total, chunk, L.nty, L.ntiles, tile_count, wg_base, ntail, rec);
     DP_LAUNCHED("k_bin_count");
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, (const int*)tile_count, L.ntiles,
                        tile_start, work, nwork);
     DP_LAUNCHED("k_bin_scan");
-    hipLaunchKernelGGL(k_bin_fill<T>, dim3(kBinGroups), dim3(256), lds, s, a, n_mov, fill_begin,
-                       total, chunk, L.nty, L.ntiles, (const int*)tile_start,
-                       (const int*)wg_base, rec);
+    DP_PROBED(DP_KERNEL_DENSITY_BIN, s,
+              hipLaunchKernelGGL(k_bin_fill<T>, dim3(kBinGroups), dim3(kFillThreads), 0, s, a,
+                                 n_mov, fill_begin, total, chunk, L.nty, L.ntiles,
+                                 (const int*)tile_start, (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
               hipLaunchKernelGGL(k_tile_scatter<T>, dim3(2048), dim3(256), 0, s, a, L.nty,
-                                 (const int4*)work, (const int*)nwork,
-                                 (const CellRec<T>*)rec, map));
+                                 (const int4*)work, (const int*)nwork, rec, map));
     DP_LAUNCHED("k_tile_scatter");
-    hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a,
-                       (const CellRec<T>*)rec, total, (const int*)nlarge, map);
+    hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, rec, total,
+                       (const int*)ntail, map);
     DP_LAUNCHED("k_rec_scatter_large");
   }
   return DP_OK;
@@ -421,25 +582,37 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
 
 template <typename T>
 int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
-                 const T* fx, const T* fy, const T* grad, char* tiled_ws, T* out, hipStream_t s) {
+                 const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
+                 hipStream_t s) {
   const int64_t total = n_mov + n_fill;
-  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(CellRec<T>));
+  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled gather: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
   const int4* work = (const int4*)(tiled_ws + L.off_work);
   const int* nwork = (const int*)(tiled_ws + L.off_nwork);
-  const int* nlarge = nwork + 1;
-  const CellRec<T>* rec = (const CellRec<T>*)(tiled_ws + L.off_rec);
-  hipLaunchKernelGGL(k_zero_uncovered<T>, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0,
-                     s, a, num_nodes, n_mov, n_fill, out);
-  DP_LAUNCHED("k_zero_uncovered");
+  const int* ntail = nwork + 1;
+  Recs<T> rec = recs_of<T>(tiled_ws, L);
+  // terminals [n_mov, num_nodes - n_fill) get 0; every other node has a record
+  const int64_t nterm = num_nodes - n_fill - n_mov;
+  if (nterm > 0) {
+    if (interleaved) {
+      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(2 * nterm, 256)), dim3(256), 0, s,
+                         out + 2 * n_mov, 2 * nterm, T(0));
+    } else {
+      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(nterm, 256)), dim3(256), 0, s, out + n_mov,
+                         nterm, T(0));
+      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(nterm, 256)), dim3(256), 0, s,
+                         out + num_nodes + n_mov, nterm, T(0));
+    }
+    DP_LAUNCHED("terminal gradient clear");
+  }
   if (total > 0) {
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
               hipLaunchKernelGGL(k_tile_gather<T>, dim3(2048), dim3(256), 0, s, a, L.nty, work,
-                                 nwork, rec, fx, fy, grad, num_nodes, out));
+                                 nwork, rec, fx, fy, grad, num_nodes, interleaved, out));
     DP_LAUNCHED("k_tile_gather");
-    hipLaunchKernelGGL(k_rec_gather_large<T>, dim3(256), dim3(256), 0, s, a, rec, total, nlarge,
-                       fx, fy, grad, num_nodes, out);
+    hipLaunchKernelGGL(k_rec_gather_large<T>, dim3(256), dim3(256), 0, s, a, rec, total, ntail,
+                       fx, fy, grad, num_nodes, interleaved, out);
     DP_LAUNCHED("k_rec_gather_large");
   }
   return DP_OK;
@@ -449,7 +622,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
   template int scatter_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, char*,        \
                                 unsigned long long*, hipStream_t);                           \
   template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, const T*,      \
-                               const T*, const T*, char*, T*, hipStream_t);
+                               const T*, const T*, char*, int, T*, hipStream_t);
 DP_TILED_INST(float)
 DP_TILED_INST(double)
 

@@ -29,6 +29,19 @@ extern Probe g_probe;
     __VA_ARGS__;                                                                  \
     if (probe_) (void)hipEventRecord(::dp::g_probe.stop, (s));                    \
   } while (0)
+// Whole-entry-point probe: records the start event where it is declared and
+// the stop event when the entry point returns, after all of its launches.
+struct ProbeScope {
+  hipStream_t s;
+  bool on;
+  ProbeScope(int id, hipStream_t s_) : s(s_), on(g_probe.kernel_id == id) {
+    if (on) (void)hipEventRecord(g_probe.start, s);
+  }
+  ~ProbeScope() {
+    if (on) (void)hipEventRecord(g_probe.stop, s);
+  }
+};
+#define DP_PROBE_OP(id, s) ::dp::ProbeScope probe_scope_((id), (s))
 
 // Status helpers.  Every entry point returns through these.
 #define DP_REQUIRE(cond, code, ...)        \
@@ -139,6 +152,18 @@ __device__ __forceinline__ double block_max(double v, double* smem) {
   __syncthreads();
   return r;
 }
+
+// Two-component vector of T: (x, y) pairs stored as one 8- or 16-byte word.
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef float2 type;
+};
+template <>
+struct Vec2<double> {
+  typedef double2 type;
+};
 
 // Fill kernel (grid-stride).  Used instead of hipMemsetAsync so that the
 // clears also behave under stream capture.

@@ -45,21 +45,61 @@ struct CellBins {
   int bxl, bxh, byl, byh;
 };
 
+// The seven per-cell inputs of the scatter, as loaded.
+template <typename T>
+struct RawCell {
+  T x, y, wx, wy, ox, oy, r;
+};
+
+template <typename T>
+__device__ __forceinline__ RawCell<T> load_cell(const TriArgs<T>& a, int64_t i) {
+  RawCell<T> c;
+  c.x = a.x[i];
+  c.y = a.y[i];
+  c.wx = a.wxc[i];
+  c.wy = a.wyc[i];
+  c.ox = a.offx[i];
+  c.oy = a.offy[i];
+  c.r = a.ratio[i];
+  return c;
+}
+
 // Stretched cell origin and its clamped bin range
 // (electric_density_map.cpp:274-293: int() truncation, exclusive upper end).
 template <typename T>
-__device__ __forceinline__ CellBins<T> cell_bins(const TriArgs<T>& a, int64_t i) {
+__device__ __forceinline__ CellBins<T> bins_of(const TriArgs<T>& a, const RawCell<T>& v) {
   CellBins<T> c;
-  c.sx = a.wxc[i];
-  c.sy = a.wyc[i];
-  c.x = a.x[i] + a.offx[i];
-  c.y = a.y[i] + a.offy[i];
-  c.r = a.ratio[i];
+  c.sx = v.wx;
+  c.sy = v.wy;
+  c.x = v.x + v.ox;
+  c.y = v.y + v.oy;
+  c.r = v.r;
   c.bxl = max((int)((c.x - a.xl) * a.inv_x), 0);
   c.bxh = min((int)((c.x + c.sx - a.xl) * a.inv_x) + 1, a.nbx);
   c.byl = max((int)((c.y - a.yl) * a.inv_y), 0);
   c.byh = min((int)((c.y + c.sy - a.yl) * a.inv_y) + 1, a.nby);
   return c;
+}
+
+template <typename T>
+__device__ __forceinline__ CellBins<T> cell_bins(const TriArgs<T>& a, int64_t i) {
+  return bins_of(a, load_cell(a, i));
+}
+
+// Store a cell's gradient: interleaved (out[2i], out[2i+1]) as one 8/16-byte
+// word, or split (out[i], out[num_nodes+i]) as the reference lays out [2N].
+template <typename T>
+__device__ __forceinline__ void put_grad(T* out, int64_t num_nodes, int64_t i, T gx, T gy,
+                                         int interleaved) {
+  if (interleaved) {
+    typename Vec2<T>::type v;
+    v.x = gx;
+    v.y = gy;
+    reinterpret_cast<typename Vec2<T>::type*>(out)[i] = v;
+  } else {
+    out[i] = gx;
+    out[num_nodes + i] = gy;
+  }
 }
 
 template <typename T>
@@ -75,7 +115,8 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
                   char* tiled_ws, unsigned long long* map, hipStream_t s);
 template <typename T>
 int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
-                 const T* fx, const T* fy, const T* grad, char* tiled_ws, T* out, hipStream_t s);
+                 const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
+                 hipStream_t s);
 
 // Density workspace layout: [int64 map][queue count (256 B)][queue][tiled region].
 inline size_t density_queue_offset(int64_t nb) { return align256(8 * (size_t)nb); }

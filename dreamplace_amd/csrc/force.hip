@@ -17,7 +17,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
                                                      int64_t num_nodes, int64_t n_mov,
-                                                     int64_t n_fill, T* out, int* qcount,
+                                                     int64_t n_fill, int il, T* out, int* qcount,
                                                      int* qlist, int qcap) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // index space: [0, n_mov) movables, then [n_mov, n_mov + n_fix) terminals
@@ -25,8 +25,7 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
   if (t >= num_nodes) return;
   int64_t i = t;
   if (i >= n_mov && i < num_nodes - n_fill) {
-    out[i] = 0;
-    out[num_nodes + i] = 0;
+    put_grad(out, num_nodes, i, T(0), T(0), il);
     return;
   }
   CellBins<T> c = cell_bins(a, i);
@@ -48,8 +47,7 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
         gy += area * fy[idx];
       }
     }
-    out[i] = -((gx * c.r) * g);
-    out[num_nodes + i] = -((gy * c.r) * g);
+    put_grad(out, num_nodes, i, -((gx * c.r) * g), -((gy * c.r) * g), il);
     return;
   }
   T px[3], py[3];
@@ -73,8 +71,7 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
       }
     }
   }
-  out[i] = -((gx * c.r) * g);
-  out[num_nodes + i] = -((gy * c.r) * g);
+  put_grad(out, num_nodes, i, -((gx * c.r) * g), -((gy * c.r) * g), il);
 }
 
 // One wavefront per queued cell: lanes stride over the cell's bins, then a
@@ -82,7 +79,7 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
 template <typename T>
 __global__ void __launch_bounds__(256) k_force_large(TriArgs<T> a, const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
-                                                     int64_t num_nodes, T* out,
+                                                     int64_t num_nodes, int il, T* out,
                                                      const int* qcount, const int* qlist,
                                                      int qcap) {
   int cnt = *qcount;
@@ -106,10 +103,7 @@ __global__ void __launch_bounds__(256) k_force_large(TriArgs<T> a, const T* __re
     }
     gx = wave_sum(gx);
     gy = wave_sum(gy);
-    if (lane == 0) {
-      out[i] = -((gx * c.r) * g);
-      out[num_nodes + i] = -((gy * c.r) * g);
-    }
+    if (lane == 0) put_grad(out, num_nodes, i, -((gx * c.r) * g), -((gy * c.r) * g), il);
   }
 }
 
@@ -124,8 +118,8 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                  double xl, double yl, double bin_size_x, double bin_size_y,
                                  int64_t num_movable_nodes, int64_t num_filler_nodes,
                                  int num_bins_x, int num_bins_y, int algorithm,
-                                 int64_t large_capacity, void* workspace, void* grad_out,
-                                 void* stream) {
+                                 int64_t large_capacity, int grad_layout, void* workspace,
+                                 void* grad_out, void* stream) {
   DP_ARG(field_map_x && field_map_y && pos && node_size_x_clamped && node_size_y_clamped &&
              offset_x && offset_y && ratio && grad_out && workspace,
          "dp_electric_force: null pointer");
@@ -139,6 +133,10 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
   const int64_t nb = (int64_t)num_bins_x * num_bins_y;
   DP_ARG(algorithm == DP_SCATTER_GLOBAL || algorithm == DP_SCATTER_TILED,
          "dp_electric_force: bad algorithm %d", algorithm);
+  DP_ARG(grad_layout == DP_LAYOUT_SPLIT || grad_layout == DP_LAYOUT_INTERLEAVED,
+         "dp_electric_force: bad grad_layout %d", grad_layout);
+  const int il = grad_layout == DP_LAYOUT_INTERLEAVED;
+  DP_PROBE_OP(DP_OP_ELECTRIC_FORCE, s);
   // Same workspace layout as dp_density_map: [int64 map][count][queue][tiled].
   char* p = (char*)workspace + density_queue_offset(nb);
   int* qcount = (int*)p;
@@ -152,17 +150,17 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y))
       return gather_tiled<T>(a, num_nodes, num_movable_nodes, num_filler_nodes,
                              (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,
-                             tiled, (T*)grad_out, s);
+                             tiled, il, (T*)grad_out, s);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, qcount, (int64_t)1, 0);
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
               hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
                                  dim3(256), 0, s, a, (const T*)field_map_x,
                                  (const T*)field_map_y, (const T*)grad_scale, num_nodes,
-                                 num_movable_nodes, num_filler_nodes, (T*)grad_out, qcount, qlist,
-                                 (int)large_capacity));
+                                 num_movable_nodes, num_filler_nodes, il, (T*)grad_out, qcount,
+                                 qlist, (int)large_capacity));
     DP_LAUNCHED("k_force_small");
     hipLaunchKernelGGL(k_force_large<T>, dim3(256), dim3(256), 0, s, a, (const T*)field_map_x,
-                       (const T*)field_map_y, (const T*)grad_scale, num_nodes, (T*)grad_out,
+                       (const T*)field_map_y, (const T*)grad_scale, num_nodes, il, (T*)grad_out,
                        (const int*)qcount, (const int*)qlist, (int)large_capacity);
     DP_LAUNCHED("k_force_large");
     return DP_OK;

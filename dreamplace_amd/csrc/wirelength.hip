@@ -366,6 +366,7 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
              wl_out && workspace,
          "dp_wa_merged_forward: null pointer");
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_WA_FORWARD, s);
   double* partials = (double*)workspace;
   int grid = reduce_grid(num_nets);
   return DP_DISPATCH(dtype, "dp_wa_merged_forward", [&] {
@@ -426,6 +427,7 @@ int dp_wa_forward_from_nodes(int dtype, const void* pos, int64_t num_nodes,
              net_mask && inv_gamma && grad_intermediate && wl_out && workspace,
          "dp_wa_forward_from_nodes: null pointer");
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_WA_FORWARD, s);
   double* partials = (double*)workspace;
   int grid = reduce_grid(num_nets);
   return DP_DISPATCH(dtype, "dp_wa_forward_from_nodes", [&] {
@@ -457,6 +459,7 @@ int dp_wa_backward_to_nodes(int dtype, const void* grad, const void* grad_interm
   DP_ARG(num_physical_nodes <= num_nodes, "dp_wa_backward_to_nodes: bad node counts");
   if (num_nodes == 0) return DP_OK;
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_NODE_GRAD, s);
   return DP_DISPATCH(dtype, "dp_wa_backward_to_nodes", [&] {
     const T* gi = (const T*)grad_intermediate;
     T* out = (T*)grad_pos_out;

@@ -1,18 +1,32 @@
 // Pin-parallel weighted-average wirelength (the engine's WA forward) and the
 // node-parallel backward that consumes its output.
 //
-// A static plan packs whole nets, in netlist order, into groups of at most
-// 64 pins; one wavefront owns one group and one lane one pin.  Each lane
-// gathers its pin position once and evaluates its four exponentials once;
-// per-net max/min and exponential sums are reduced through LDS by the net's
-// first lane; every lane then forms its own derivative.  Nets of more than
-// 64 pins are listed apart and done by one wavefront each, in chunks.
-// The derivative of each pin, already multiplied by its net weight (and 0 on
-// masked pins), is written at the pin's position in node (CSR) order, so the
-// backward is a contiguous per-node sum.
+// Data movement: the netlist connects nodes in no spatial or index order, so
+// the forward has to permute between node order and net order twice (pin
+// positions in, pin derivatives out).  Both permutations are done as
+// scattered 8/16-byte stores of whole (x, y) words, which this memory system
+// sustains at ~1.5x the rate of scattered loads:
+//   k_pin_push   one thread per pin slot in node (CSR) order reads its node's
+//                position and the pin offset (both contiguous) and stores
+//                pos + offset at the pin's slot in "group order";
+//   k_wa_pins    reads pin positions and per-pin metadata contiguously in
+//                group order and stores each pin's derivative at the pin's
+//                slot in node order;
+//   k_node_grad  sums each node's contiguous CSR range.
+//
+// Group order is a static plan (dp_wa_pack_nets): nets of at most 64 pins are
+// packed best-fit-decreasing by degree into groups of at most 64 pins, so a
+// group's per-net serial loops are about as long as its nets' degree; one
+// wavefront (one workgroup) owns one group and one lane one pin.  Each lane
+// evaluates its four exponentials once; per-net max/min and exponential sums
+// are reduced through LDS by the net's first lane; every lane then forms its
+// own derivative.  Nets of more than 64 pins follow all groups in group order
+// and are done by one wavefront each, in chunks.
 //
 // Per net the arithmetic follows weighted_average_wirelength_merged.cpp:31-116;
 // packed nets accumulate their exponential sums in pin order, as there.
+#include <vector>
+
 #include "dp_common.h"
 
 namespace dp {
@@ -24,33 +38,81 @@ __device__ __forceinline__ float big_value<float>() { return 3.402823466e+38f; }
 template <>
 __device__ __forceinline__ double big_value<double>() { return 1.7976931348623157e+308; }
 
-template <typename T>
-struct Vec2;
-template <>
-struct Vec2<float> {
-  typedef float2 type;
-};
-template <>
-struct Vec2<double> {
-  typedef double2 type;
-};
+constexpr int kPackedGrid = 12288;  // cap of workgroups (one wavefront each) of the packed kernel
+constexpr int kBigGrid = 1024;      // cap of workgroups of the big-net kernel (4 wavefronts each)
+constexpr int kMaskBit = (int)0x80000000u;
 
-// xy[n] = (x[n], y[n]) for the physical nodes: one 8- or 16-byte gather per pin.
-template <typename T>
-__global__ void k_interleave(const T* __restrict__ x, const T* __restrict__ y, int64_t n,
-                             typename Vec2<T>::type* xy) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    typename Vec2<T>::type v;
-    v.x = x[i];
-    v.y = y[i];
-    xy[i] = v;
+// ---- static preparation ----------------------------------------------------
+// group-order slot s of pin r of net e (s = net_slot[e] + r):
+//   meta[s] = {e | (pin masked ? sign bit : 0), node-order slot, r | deg << 8, 0}
+// and inv[pin] = s.  (.y is filled by k_prep_node.)
+__global__ void k_prep_net(const int* __restrict__ flat_netpin,
+                           const int* __restrict__ netpin_start, const int* __restrict__ pin2net,
+                           const uint8_t* __restrict__ pin_mask, const int* __restrict__ net_slot,
+                           int64_t num_pins, int4* meta, int* inv) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < num_pins;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int p = flat_netpin[j];
+    const int e = pin2net[p];
+    const int b = netpin_start[e];
+    const int r = (int)j - b, deg = netpin_start[e + 1] - b;
+    const int s = net_slot[e] + r;
+    int4 m;
+    m.x = e | (pin_mask && pin_mask[p] ? kMaskBit : 0);
+    m.y = 0;
+    m.z = deg <= 64 ? (r | (deg << 8)) : 0;
+    m.w = 0;
+    meta[s] = m;
+    inv[p] = s;
   }
 }
 
-constexpr int kWaves = 4;           // wavefronts per workgroup
-constexpr int kPackedGrid = 8192;   // cap of workgroups of the packed kernel
-constexpr int kBigGrid = 1024;      // cap of workgroups of the big-net kernel
+// node-order slot i: dst[i] = group-order slot of the pin, meta[dst].y = i,
+// off[i] = (offset_x, offset_y) of the pin
+template <typename T>
+__global__ void k_prep_node(const int* __restrict__ flat_node2pin, const int* __restrict__ inv,
+                            const T* __restrict__ offx, const T* __restrict__ offy,
+                            int64_t num_pins, int* dst, int4* meta,
+                            typename Vec2<T>::type* off) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < num_pins;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int p = flat_node2pin[i];
+    const int s = inv[p];
+    dst[i] = s;
+    meta[s].y = (int)i;
+    typename Vec2<T>::type v;
+    v.x = offx[p];
+    v.y = offy[p];
+    off[i] = v;
+  }
+}
+
+// slot_node[i] = node owning node-order slot i
+__global__ void k_prep_slot_node(const int* __restrict__ node2pin_start, int64_t nphys,
+                                 int* slot_node) {
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nphys;
+       n += (int64_t)gridDim.x * blockDim.x)
+    for (int i = node2pin_start[n]; i < node2pin_start[n + 1]; ++i) slot_node[i] = (int)n;
+}
+
+// ---- forward ---------------------------------------------------------------
+// pin_xy[dst[i]] = (x[n] + off[i].x, y[n] + off[i].y), n = slot_node[i]
+template <typename T>
+__global__ void __launch_bounds__(256) k_pin_push(const T* __restrict__ x, const T* __restrict__ y,
+                                                  int64_t num_pins,
+                                                  const int* __restrict__ slot_node,
+                                                  const typename Vec2<T>::type* __restrict__ off,
+                                                  const int* __restrict__ dst,
+                                                  typename Vec2<T>::type* pin_xy) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_pins) return;
+  const int n = slot_node[i];
+  const typename Vec2<T>::type o = off[i];
+  typename Vec2<T>::type v;
+  v.x = x[n] + o.x;
+  v.y = y[n] + o.y;
+  pin_xy[dst[i]] = v;
+}
 
 template <typename T>
 __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
@@ -60,82 +122,74 @@ __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
   ((typename Vec2<T>::type*)gnode)[q] = gv;
 }
 
+// One wavefront per workgroup, one group per iteration of the grid-stride loop.
 template <typename T>
-__global__ void __launch_bounds__(256) k_wa_pins(
-    const typename Vec2<T>::type* __restrict__ xy, const T* __restrict__ offx,
-    const T* __restrict__ offy, const int* __restrict__ pin2node,
-    const int* __restrict__ flat_netpin, const int* __restrict__ netpin_start,
-    const int* __restrict__ pin2net, const int2* __restrict__ groups, int ngroups,
-    const T* __restrict__ w, const uint8_t* __restrict__ net_mask,
-    const uint8_t* __restrict__ pin_mask, const int* __restrict__ node_order,
-    const T* __restrict__ inv_gamma_p, T* partial, T* gnode, double* partials) {
-  __shared__ T s_x[kWaves][64], s_y[kWaves][64];
-  __shared__ T s_e[kWaves][4][64];
-  __shared__ T s_c[kWaves][8][64];
-  __shared__ double s_red[4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__global__ void __launch_bounds__(64) k_wa_pins(
+    const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
+    const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial,
+    T* gnode, double* partials) {
+  __shared__ T s_x[64], s_y[64];
+  __shared__ T s_e[4][64];
+  __shared__ T s_c[8][64];
+  const int lane = threadIdx.x;
   const T ig = *inv_gamma_p;
   double contrib = 0;
-  // the trip count depends on blockIdx only: every wave meets every barrier
-  for (int base = blockIdx.x * kWaves; base < ngroups; base += gridDim.x * kWaves) {
-    const int g = base + wv;
-    int pb = 0, pe = 0;
-    if (g < ngroups) {
-      const int2 gr = groups[g];
-      pb = netpin_start[gr.x];
-      pe = netpin_start[gr.y];
-    }
+  for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int pb = group_start[g], pe = group_start[g + 1];
     const int j = pb + lane;
     const bool on = j < pe;
-    int p = 0, e = 0, head = 0, deg = 0;
-    bool live = false;
+    int e = 0, head = 0, deg = 0, dst = 0;
+    bool live = false, pin_masked = false;
     T x = 0, y = 0;
     if (on) {
-      p = flat_netpin[j];
-      e = pin2net[p];
-      head = netpin_start[e] - pb;
-      deg = netpin_start[e + 1] - netpin_start[e];
+      const int4 m = meta[j];
+      e = m.x & ~kMaskBit;
+      pin_masked = m.x < 0;
+      dst = m.y;
+      head = lane - (m.z & 0xff);
+      deg = m.z >> 8;
       live = net_mask[e] != 0;
-      const typename Vec2<T>::type v = xy[pin2node[p]];
-      x = offx[p] + v.x;
-      y = offy[p] + v.y;
+      const typename Vec2<T>::type v = pin_xy[j];
+      x = v.x;
+      y = v.y;
     }
-    s_x[wv][lane] = x;
-    s_y[wv][lane] = y;
+    s_x[lane] = x;
+    s_y[lane] = y;
     __syncthreads();
     const bool is_head = on && lane == head;
     if (is_head && live) {
       T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
       for (int k = head; k < head + deg; ++k) {
-        const T xx = s_x[wv][k], yy = s_y[wv][k];
+        const T xx = s_x[k], yy = s_y[k];
         xmax = xx > xmax ? xx : xmax;
         xmin = xx < xmin ? xx : xmin;
         ymax = yy > ymax ? yy : ymax;
         ymin = yy < ymin ? yy : ymin;
       }
-      s_c[wv][0][lane] = xmax;
-      s_c[wv][1][lane] = xmin;
-      s_c[wv][2][lane] = ymax;
-      s_c[wv][3][lane] = ymin;
+      s_c[0][lane] = xmax;
+      s_c[1][lane] = xmin;
+      s_c[2][lane] = ymax;
+      s_c[3][lane] = ymin;
     }
     __syncthreads();
     T ex = 0, enx = 0, ey = 0, eny = 0;
     if (on && live) {
-      ex = exp((x - s_c[wv][0][head]) * ig);
-      enx = exp((s_c[wv][1][head] - x) * ig);
-      ey = exp((y - s_c[wv][2][head]) * ig);
-      eny = exp((s_c[wv][3][head] - y) * ig);
-      s_e[wv][0][lane] = ex;
-      s_e[wv][1][lane] = enx;
-      s_e[wv][2][lane] = ey;
-      s_e[wv][3][lane] = eny;
+      ex = exp((x - s_c[0][head]) * ig);
+      enx = exp((s_c[1][head] - x) * ig);
+      ey = exp((y - s_c[2][head]) * ig);
+      eny = exp((s_c[3][head] - y) * ig);
+      s_e[0][lane] = ex;
+      s_e[1][lane] = enx;
+      s_e[2][lane] = ey;
+      s_e[3][lane] = eny;
     }
     __syncthreads();
     if (is_head && live) {
       T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
       for (int k = head; k < head + deg; ++k) {
-        const T xx = s_x[wv][k], yy = s_y[wv][k];
-        const T a0 = s_e[wv][0][k], a1 = s_e[wv][1][k], a2 = s_e[wv][2][k], a3 = s_e[wv][3][k];
+        const T xx = s_x[k], yy = s_y[k];
+        const T a0 = s_e[0][k], a1 = s_e[1][k], a2 = s_e[2][k], a3 = s_e[3][k];
         xs += xx * a0;
         xns += xx * a1;
         es += a0;
@@ -149,47 +203,42 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       if (partial) partial[e] = wl;
       contrib += (double)(w ? wl * w[e] : wl);
       const T bx = ig / es, bnx = -ig / ens, by = ig / fs, bny = -ig / fns;
-      s_c[wv][0][lane] = (T)((1.0 - bx * xs) / es);
-      s_c[wv][1][lane] = bx;
-      s_c[wv][2][lane] = (T)((1.0 - bnx * xns) / ens);
-      s_c[wv][3][lane] = bnx;
-      s_c[wv][4][lane] = (T)((1.0 - by * ys) / fs);
-      s_c[wv][5][lane] = by;
-      s_c[wv][6][lane] = (T)((1.0 - bny * yns) / fns);
-      s_c[wv][7][lane] = bny;
+      s_c[0][lane] = (T)((1.0 - bx * xs) / es);
+      s_c[1][lane] = bx;
+      s_c[2][lane] = (T)((1.0 - bnx * xns) / ens);
+      s_c[3][lane] = bnx;
+      s_c[4][lane] = (T)((1.0 - by * ys) / fs);
+      s_c[5][lane] = by;
+      s_c[6][lane] = (T)((1.0 - bny * yns) / fns);
+      s_c[7][lane] = bny;
     }
-    if (is_head && !live && partial) partial[e] = 0;
     __syncthreads();
     if (on) {
       T gx = 0, gy = 0;
-      if (live && !(pin_mask && pin_mask[p])) {
-        gx = (s_c[wv][0][head] + s_c[wv][1][head] * x) * ex -
-             (s_c[wv][2][head] + s_c[wv][3][head] * x) * enx;
-        gy = (s_c[wv][4][head] + s_c[wv][5][head] * y) * ey -
-             (s_c[wv][6][head] + s_c[wv][7][head] * y) * eny;
+      if (live && !pin_masked) {
+        gx = (s_c[0][head] + s_c[1][head] * x) * ex - (s_c[2][head] + s_c[3][head] * x) * enx;
+        gy = (s_c[4][head] + s_c[5][head] * y) * ey - (s_c[6][head] + s_c[7][head] * y) * eny;
         if (w) {
           const T wn = w[e];
           gx *= wn;
           gy *= wn;
         }
       }
-      store_grad(gnode, node_order[p], gx, gy);
+      store_grad(gnode, dst, gx, gy);
     }
-    __syncthreads();  // LDS reuse by the next group set
+    __syncthreads();  // LDS reuse by the next group
   }
-  const double bs = block_sum<256>(contrib, s_red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+  contrib = wave_sum(contrib);
+  if (lane == 0) partials[blockIdx.x] = contrib;
 }
 
-// Nets of more than 64 pins: one wavefront per net.
+// Nets of more than 64 pins: one wavefront per net; big[q] = {net, first
+// group-order slot, degree, 0}.
 template <typename T>
 __global__ void __launch_bounds__(256) k_wa_big(
-    const typename Vec2<T>::type* __restrict__ xy, const T* __restrict__ offx,
-    const T* __restrict__ offy, const int* __restrict__ pin2node,
-    const int* __restrict__ flat_netpin, const int* __restrict__ netpin_start,
-    const int* __restrict__ big, int nbig, const T* __restrict__ w,
-    const uint8_t* __restrict__ net_mask, const uint8_t* __restrict__ pin_mask,
-    const int* __restrict__ node_order, const T* __restrict__ inv_gamma_p, T* partial, T* gnode,
+    const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
+    const int4* __restrict__ big, int nbig, const T* __restrict__ w,
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial, T* gnode,
     double* partials) {
   __shared__ double s_red[4];
   const int lane = threadIdx.x & 63;
@@ -198,18 +247,16 @@ __global__ void __launch_bounds__(256) k_wa_big(
   const T ig = *inv_gamma_p;
   double contrib = 0;
   for (int q = wave; q < nbig; q += nwaves) {
-    const int e = big[q];
-    const int pb = netpin_start[e], pe = netpin_start[e + 1];
+    const int4 bq = big[q];
+    const int e = bq.x, pb = bq.y, pe = bq.y + bq.z;
     const bool live = net_mask[e] != 0;
     T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
     for (int j = pb + lane; j < pe; j += 64) {
-      const int p = flat_netpin[j];
-      const typename Vec2<T>::type v = xy[pin2node[p]];
-      const T xx = offx[p] + v.x, yy = offy[p] + v.y;
-      xmax = xx > xmax ? xx : xmax;
-      xmin = xx < xmin ? xx : xmin;
-      ymax = yy > ymax ? yy : ymax;
-      ymin = yy < ymin ? yy : ymin;
+      const typename Vec2<T>::type v = pin_xy[j];
+      xmax = v.x > xmax ? v.x : xmax;
+      xmin = v.x < xmin ? v.x : xmin;
+      ymax = v.y > ymax ? v.y : ymax;
+      ymin = v.y < ymin ? v.y : ymin;
     }
     xmax = wave_max(xmax);
     ymax = wave_max(ymax);
@@ -217,9 +264,8 @@ __global__ void __launch_bounds__(256) k_wa_big(
     ymin = -wave_max(-ymin);
     T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
     for (int j = pb + lane; j < pe; j += 64) {
-      const int p = flat_netpin[j];
-      const typename Vec2<T>::type v = xy[pin2node[p]];
-      const T xx = offx[p] + v.x, yy = offy[p] + v.y;
+      const typename Vec2<T>::type v = pin_xy[j];
+      const T xx = v.x, yy = v.y;
       const T a0 = exp((xx - xmax) * ig), a1 = exp((xmin - xx) * ig);
       const T a2 = exp((yy - ymax) * ig), a3 = exp((ymin - yy) * ig);
       xs += xx * a0;
@@ -241,18 +287,18 @@ __global__ void __launch_bounds__(256) k_wa_big(
     fns = wave_sum(fns);
     const T wl = xs / es - xns / ens + ys / fs - yns / fns;
     if (lane == 0) {
-      if (partial) partial[e] = live ? wl : T(0);
+      if (partial && live) partial[e] = wl;
       if (live) contrib += (double)(w ? wl * w[e] : wl);
     }
     const T bx = ig / es, bnx = -ig / ens, by = ig / fs, bny = -ig / fns;
     const T ax = (T)((1.0 - bx * xs) / es), anx = (T)((1.0 - bnx * xns) / ens);
     const T ay = (T)((1.0 - by * ys) / fs), any = (T)((1.0 - bny * yns) / fns);
     for (int j = pb + lane; j < pe; j += 64) {
-      const int p = flat_netpin[j];
+      const int4 m = meta[j];
       T gx = 0, gy = 0;
-      if (live && !(pin_mask && pin_mask[p])) {
-        const typename Vec2<T>::type v = xy[pin2node[p]];
-        const T xx = offx[p] + v.x, yy = offy[p] + v.y;
+      if (live && m.x >= 0) {
+        const typename Vec2<T>::type v = pin_xy[j];
+        const T xx = v.x, yy = v.y;
         gx = (ax + bx * xx) * exp((xx - xmax) * ig) - (anx + bnx * xx) * exp((xmin - xx) * ig);
         gy = (ay + by * yy) * exp((yy - ymax) * ig) - (any + bny * yy) * exp((ymin - yy) * ig);
         if (w) {
@@ -260,20 +306,21 @@ __global__ void __launch_bounds__(256) k_wa_big(
           gy *= w[e];
         }
       }
-      store_grad(gnode, node_order[p], gx, gy);
+      store_grad(gnode, m.y, gx, gy);
     }
   }
   const double bs = block_sum<256>(contrib, s_red);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
+// ---- backward --------------------------------------------------------------
 // grad[n] = (*grad_scale) * sum of gnode over node n's CSR range + add[n]
-// (add nullable).  Fillers (n >= nphys) get add[n] only.
+// (add nullable, split or interleaved).  Fillers (n >= nphys) get add[n] only.
 template <typename T>
 __global__ void k_node_grad(const typename Vec2<T>::type* __restrict__ gnode,
                             const int* __restrict__ node2pin_start, const T* __restrict__ grad,
-                            const T* __restrict__ add, int64_t nphys, int64_t nnodes, T* gx_out,
-                            T* gy_out) {
+                            const T* __restrict__ add, int add_interleaved, int64_t nphys,
+                            int64_t nnodes, T* gx_out, T* gy_out) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
   T sx = 0, sy = 0;
@@ -286,15 +333,21 @@ __global__ void k_node_grad(const typename Vec2<T>::type* __restrict__ gnode,
     }
   }
   if (add) {
-    sx += add[n];
-    sy += add[nnodes + n];
+    if (add_interleaved) {
+      const typename Vec2<T>::type v = reinterpret_cast<const typename Vec2<T>::type*>(add)[n];
+      sx += v.x;
+      sy += v.y;
+    } else {
+      sx += add[n];
+      sy += add[nnodes + n];
+    }
   }
   gx_out[n] = sx;
   gy_out[n] = sy;
 }
 
-static int grid_cap(int64_t items, int cap) {
-  int64_t g = ceil_div(items, kWaves);
+static int grid_cap(int64_t items, int per_block, int cap) {
+  int64_t g = ceil_div(items, per_block);
   if (g < 1) g = 1;
   return (int)(g > cap ? cap : g);
 }
@@ -305,82 +358,166 @@ using namespace dp;
 
 extern "C" {
 
-int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* groups,
-                    int64_t* num_groups, int32_t* big_nets, int64_t* num_big) {
-  DP_ARG(netpin_start && groups && num_groups && big_nets && num_big,
+int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* group_start,
+                    int64_t* num_groups, int32_t* big_nets, int64_t* num_big, int32_t* net_slot) {
+  DP_ARG(netpin_start && group_start && num_groups && big_nets && num_big && net_slot,
          "dp_wa_pack_nets: null pointer");
-  int64_t g = 0, b = 0, e = 0;
-  while (e < num_nets) {
-    const int64_t d0 = (int64_t)netpin_start[e + 1] - netpin_start[e];
-    DP_ARG(d0 >= 0, "dp_wa_pack_nets: netpin_start not monotone at net %lld", (long long)e);
-    if (d0 > 64) {
-      big_nets[b++] = (int32_t)e++;
-      continue;
+  DP_ARG(num_nets >= 0 && num_nets < ((int64_t)1 << 31), "dp_wa_pack_nets: bad num_nets");
+  // nets by degree (1..64), each bucket in netlist order
+  std::vector<int64_t> count(65, 0);
+  int64_t nbig = 0, big_pins = 0;
+  for (int64_t e = 0; e < num_nets; ++e) {
+    const int64_t d = (int64_t)netpin_start[e + 1] - netpin_start[e];
+    DP_ARG(d >= 0, "dp_wa_pack_nets: netpin_start not monotone at net %lld", (long long)e);
+    if (d > 64) {
+      ++nbig;
+      big_pins += d;
+    } else {
+      ++count[d];
     }
-    const int64_t nb = e;
-    int64_t pins = d0;
-    ++e;
-    while (e < num_nets) {
+  }
+  std::vector<int64_t> head(66, 0);
+  for (int d = 1; d <= 64; ++d) head[d + 1] = head[d] + count[d];
+  std::vector<int32_t> order((size_t)(head[65] > 0 ? head[65] : 1));
+  {
+    std::vector<int64_t> fill(head.begin(), head.end());
+    for (int64_t e = 0; e < num_nets; ++e) {
       const int64_t d = (int64_t)netpin_start[e + 1] - netpin_start[e];
-      if (d > 64 || pins + d > 64) break;
-      pins += d;
-      ++e;
+      if (d >= 1 && d <= 64) order[(size_t)fill[d]++] = (int32_t)e;
     }
-    groups[2 * g] = (int32_t)nb;
-    groups[2 * g + 1] = (int32_t)e;
+  }
+  std::vector<int64_t> cur(head.begin(), head.end());  // next unplaced net of each bucket
+  uint64_t nonempty = 0;                                 // bit d-1: bucket d has nets left
+  for (int d = 1; d <= 64; ++d)
+    if (cur[d] < head[d + 1]) nonempty |= 1ull << (d - 1);
+  // best-fit decreasing: open a group with the largest remaining net, then
+  // keep adding the largest remaining net that still fits
+  int64_t g = 0, slot = 0;
+  group_start[0] = 0;
+  while (nonempty) {
+    int cap = 64;
+    while (cap > 0) {
+      const uint64_t fit = cap >= 64 ? nonempty : (nonempty & ((1ull << cap) - 1));
+      if (!fit) break;
+      const int d = 64 - __builtin_clzll(fit);  // largest degree <= cap with nets left
+      const int32_t e = order[(size_t)cur[d]++];
+      if (cur[d] == head[d + 1]) nonempty &= ~(1ull << (d - 1));
+      net_slot[e] = (int32_t)slot;
+      slot += d;
+      cap -= d;
+    }
     ++g;
+    group_start[g] = (int32_t)slot;
+  }
+  // nets of more than 64 pins after all groups; nets without pins get the
+  // end slot (they own no slot)
+  int64_t b = 0;
+  for (int64_t e = 0; e < num_nets; ++e) {
+    const int64_t d = (int64_t)netpin_start[e + 1] - netpin_start[e];
+    if (d > 64) {
+      big_nets[4 * b] = (int32_t)e;
+      big_nets[4 * b + 1] = (int32_t)slot;
+      big_nets[4 * b + 2] = (int32_t)d;
+      big_nets[4 * b + 3] = 0;
+      net_slot[e] = (int32_t)slot;
+      slot += d;
+      ++b;
+    } else if (d == 0) {
+      net_slot[e] = (int32_t)slot;
+    }
   }
   *num_groups = g;
   *num_big = b;
   return DP_OK;
 }
 
-size_t dp_wa_pins_workspace_bytes(int64_t num_physical_nodes, int dtype) {
-  const size_t eb = dtype == DP_FLOAT64 ? 16 : 8;
-  return reduce_slot_bytes() + align256(eb * (size_t)(num_physical_nodes > 0 ? num_physical_nodes : 1));
+size_t dp_wa_prepare_workspace_bytes(int64_t num_pins) {
+  return align256(sizeof(int) * (size_t)(num_pins > 0 ? num_pins : 1));
 }
 
-int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_physical_nodes,
+int dp_wa_pins_prepare(int dtype, int64_t num_pins, int64_t num_physical_nodes,
                        const void* pin_offset_x, const void* pin_offset_y,
-                       const int32_t* pin2node_map, const int32_t* flat_netpin,
-                       const int32_t* netpin_start, const int32_t* pin2net_map,
-                       const int32_t* groups, int64_t num_groups, const int32_t* big_nets,
-                       int64_t num_big, const void* net_weights, const uint8_t* net_mask,
-                       const uint8_t* pin_mask, const int32_t* pin_node_order,
-                       const void* inv_gamma, void* partial_wl, void* grad_node_order,
-                       void* wl_out, void* workspace, void* stream) {
-  DP_ARG(pos && pin_offset_x && pin_offset_y && pin2node_map && flat_netpin && netpin_start &&
-             pin2net_map && (groups || num_groups == 0) && (big_nets || num_big == 0) &&
-             net_mask && pin_node_order && inv_gamma && grad_node_order && wl_out && workspace,
+                       const int32_t* flat_netpin, const int32_t* netpin_start,
+                       const int32_t* pin2net_map, const uint8_t* pin_mask,
+                       const int32_t* flat_node2pin_map, const int32_t* flat_node2pin_start_map,
+                       const int32_t* net_slot, void* workspace, void* node_offsets,
+                       int32_t* node_dst, int32_t* slot_node, int32_t* pin_meta, void* stream) {
+  DP_ARG(pin_offset_x && pin_offset_y && flat_netpin && netpin_start && pin2net_map &&
+             flat_node2pin_map && flat_node2pin_start_map && net_slot && workspace &&
+             node_offsets && node_dst && slot_node && pin_meta,
+         "dp_wa_pins_prepare: null pointer");
+  DP_ARG(num_pins >= 0 && num_pins < ((int64_t)1 << 31) && num_physical_nodes >= 0,
+         "dp_wa_pins_prepare: bad counts");
+  if (num_pins == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  int* inv = (int*)workspace;
+  hipLaunchKernelGGL(k_prep_net, dim3(grid_for(num_pins, 256)), dim3(256), 0, s,
+                     (const int*)flat_netpin, (const int*)netpin_start, (const int*)pin2net_map,
+                     pin_mask, (const int*)net_slot, num_pins, (int4*)pin_meta, inv);
+  DP_LAUNCHED("k_prep_net");
+  hipLaunchKernelGGL(k_prep_slot_node, dim3(grid_for(num_physical_nodes, 256)), dim3(256), 0, s,
+                     (const int*)flat_node2pin_start_map, num_physical_nodes, (int*)slot_node);
+  DP_LAUNCHED("k_prep_slot_node");
+  return DP_DISPATCH(dtype, "dp_wa_pins_prepare", [&] {
+    hipLaunchKernelGGL(k_prep_node<T>, dim3(grid_for(num_pins, 256)), dim3(256), 0, s,
+                       (const int*)flat_node2pin_map, (const int*)inv, (const T*)pin_offset_x,
+                       (const T*)pin_offset_y, num_pins, (int*)node_dst, (int4*)pin_meta,
+                       (typename Vec2<T>::type*)node_offsets);
+    DP_LAUNCHED("k_prep_node");
+    return DP_OK;
+  });
+}
+
+size_t dp_wa_pins_workspace_bytes(int64_t num_pins, int dtype) {
+  const size_t eb = dtype == DP_FLOAT64 ? 16 : 8;
+  return reduce_slot_bytes() + align256(eb * (size_t)(num_pins > 0 ? num_pins : 1));
+}
+
+int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                       const int32_t* slot_node, const void* node_offsets,
+                       const int32_t* node_dst, const int32_t* pin_meta,
+                       const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
+                       int64_t num_big, int64_t num_nets, const void* net_weights,
+                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
+                       void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+  DP_ARG(pos && slot_node && node_offsets && node_dst && pin_meta &&
+             (group_start || num_groups == 0) && (big_nets || num_big == 0) && net_mask &&
+             inv_gamma && grad_node_order && wl_out && workspace,
          "dp_wa_forward_pins: null pointer");
-  DP_ARG(num_physical_nodes <= num_nodes && num_groups >= 0 && num_big >= 0,
+  DP_ARG(num_pins >= 0 && num_pins < ((int64_t)1 << 31) && num_groups >= 0 && num_big >= 0 &&
+             num_nets >= 0,
          "dp_wa_forward_pins: bad counts");
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_WA_FORWARD, s);
   double* partials = (double*)workspace;
-  char* xy = (char*)workspace + reduce_slot_bytes();
-  const int g1 = grid_cap(num_groups, kPackedGrid);
-  const int g2 = grid_cap(num_big, kBigGrid);
+  char* pin_xy = (char*)workspace + reduce_slot_bytes();
+  const int g1 = grid_cap(num_groups, 1, kPackedGrid);
+  const int g2 = grid_cap(num_big, 4, kBigGrid);
   return DP_DISPATCH(dtype, "dp_wa_forward_pins", [&] {
     typedef typename Vec2<T>::type V;
     const T* P = (const T*)pos;
-    hipLaunchKernelGGL(k_interleave<T>, dim3(grid_for(num_physical_nodes, 256)), dim3(256), 0, s,
-                       P, P + num_nodes, num_physical_nodes, (V*)xy);
-    DP_LAUNCHED("k_interleave");
+    if (partial_wl && num_nets > 0) {  // nets without a live group entry keep 0
+      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(num_nets, 256)), dim3(256), 0, s,
+                         (T*)partial_wl, num_nets, T(0));
+      DP_LAUNCHED("partial_wl clear");
+    }
+    if (num_pins > 0) {
+      DP_PROBED(DP_KERNEL_PIN_PUSH, s,
+                hipLaunchKernelGGL(k_pin_push<T>, dim3((unsigned)ceil_div(num_pins, 256)),
+                                   dim3(256), 0, s, P, P + num_nodes, num_pins,
+                                   (const int*)slot_node, (const V*)node_offsets,
+                                   (const int*)node_dst, (V*)pin_xy));
+      DP_LAUNCHED("k_pin_push");
+    }
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
-              hipLaunchKernelGGL(k_wa_pins<T>, dim3(g1), dim3(256), 0, s, (const V*)xy,
-                                 (const T*)pin_offset_x, (const T*)pin_offset_y,
-                                 (const int*)pin2node_map, (const int*)flat_netpin,
-                                 (const int*)netpin_start, (const int*)pin2net_map,
-                                 (const int2*)groups, (int)num_groups, (const T*)net_weights,
-                                 net_mask, pin_mask, (const int*)pin_node_order,
-                                 (const T*)inv_gamma, (T*)partial_wl, (T*)grad_node_order,
-                                 partials));
+              hipLaunchKernelGGL(k_wa_pins<T>, dim3(g1), dim3(64), 0, s, (const V*)pin_xy,
+                                 (const int4*)pin_meta, (const int*)group_start, (int)num_groups,
+                                 (const T*)net_weights, net_mask, (const T*)inv_gamma,
+                                 (T*)partial_wl, (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
-    hipLaunchKernelGGL(k_wa_big<T>, dim3(g2), dim3(256), 0, s, (const V*)xy,
-                       (const T*)pin_offset_x, (const T*)pin_offset_y, (const int*)pin2node_map,
-                       (const int*)flat_netpin, (const int*)netpin_start, (const int*)big_nets,
-                       (int)num_big, (const T*)net_weights, net_mask, pin_mask,
-                       (const int*)pin_node_order, (const T*)inv_gamma, (T*)partial_wl,
+    hipLaunchKernelGGL(k_wa_big<T>, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
+                       (const int4*)pin_meta, (const int4*)big_nets, (int)num_big,
+                       (const T*)net_weights, net_mask, (const T*)inv_gamma, (T*)partial_wl,
                        (T*)grad_node_order, partials + g1);
     DP_LAUNCHED("k_wa_big");
     return finish_reduction(partials, g1 + g2, 1, nullptr, wl_out, dtype, s);
@@ -389,13 +526,16 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
 
 int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,
                            const int32_t* flat_node2pin_start_map, const void* add,
-                           int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
-                           void* stream) {
+                           int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
+                           void* grad_pos_out, void* stream) {
   DP_ARG(grad && grad_node_order && flat_node2pin_start_map && grad_pos_out,
          "dp_node_grad_from_pins: null pointer");
   DP_ARG(num_physical_nodes <= num_nodes, "dp_node_grad_from_pins: bad node counts");
+  DP_ARG(add_layout == DP_LAYOUT_SPLIT || add_layout == DP_LAYOUT_INTERLEAVED,
+         "dp_node_grad_from_pins: bad add_layout %d", add_layout);
   if (num_nodes == 0) return DP_OK;
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_NODE_GRAD, s);
   return DP_DISPATCH(dtype, "dp_node_grad_from_pins", [&] {
     typedef typename Vec2<T>::type V;
     T* out = (T*)grad_pos_out;
@@ -403,8 +543,8 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
               hipLaunchKernelGGL(k_node_grad<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
                                  dim3(256), 0, s, (const V*)grad_node_order,
                                  (const int*)flat_node2pin_start_map, (const T*)grad,
-                                 (const T*)add, num_physical_nodes, num_nodes, out,
-                                 out + num_nodes));
+                                 (const T*)add, add_layout == DP_LAYOUT_INTERLEAVED,
+                                 num_physical_nodes, num_nodes, out, out + num_nodes));
     DP_LAUNCHED("k_node_grad");
     return DP_OK;
   });

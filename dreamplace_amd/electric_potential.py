@@ -103,9 +103,12 @@ class ElectricPotential(ElectricOverflow):
             energy = torch.zeros((), dtype=density_map.dtype, device=density_map.device)
         return fx, fy, energy
 
-    def electric_force(self, pos, field_x, field_y, grad_pos):
+    def electric_force(self, pos, field_x, field_y, grad_pos, grad_layout="split"):
         """-(ratio * sum overlap*E) * grad for movables and fillers; 0 for
-        terminals (electric_potential.py:219-245)."""
+        terminals (electric_potential.py:219-245).  grad_layout "split" is the
+        reference's [2N] (all x, then all y); "interleaved" holds (x, y) of
+        node i at [2i, 2i+1]."""
+        layout = {"split": N.DP_LAYOUT_SPLIT, "interleaved": N.DP_LAYOUT_INTERLEAVED}[grad_layout]
         pos = pos.view(-1)
         grad_scale = grad_pos.reshape(()).to(pos.dtype).contiguous()
         out = torch.empty_like(pos)
@@ -115,7 +118,8 @@ class ElectricPotential(ElectricOverflow):
                N.ptr(self.ratio), float(self.xl), float(self.yl), float(self.bin_size_x),
                float(self.bin_size_y), int(self.num_movable_nodes), int(self.num_filler_nodes),
                self.num_bins_x, self.num_bins_y, self.algorithm_code, self.large_capacity,
-               N.ptr(self.workspace(pos.device)), N.ptr(out), N.stream_ptr(pos.device))
+               layout, N.ptr(self.workspace(pos.device)), N.ptr(out),
+               N.stream_ptr(pos.device))
         return out
 
     def forward(self, pos, mode="density"):

@@ -68,38 +68,85 @@ class DeviceDesign:
         raise AttributeError(k)
 
 
+def check_netlist(flat_netpin, netpin_start, pin2net, flat_node2pin, node2pin_start, num_pins,
+                  num_physical_nodes):
+    """Host check of the netlist maps the pin-parallel kernels index with:
+    both flat maps are permutations of [0, P), the CSR starts are monotone and
+    cover [0, P), and pin2net agrees with the net CSR.  Raises ValueError."""
+    P = int(num_pins)
+    for name, perm in (("flat_net2pin_map", flat_netpin), ("flat_node2pin_map", flat_node2pin)):
+        if len(perm) != P or not np.array_equal(np.sort(perm), np.arange(P, dtype=perm.dtype)):
+            raise ValueError("%s is not a permutation of [0, num_pins)" % name)
+    for name, st, n in (("flat_net2pin_start_map", netpin_start, None),
+                        ("flat_node2pin_start_map", node2pin_start, num_physical_nodes)):
+        st = np.asarray(st, dtype=np.int64)
+        if n is not None and len(st) < n + 1:
+            raise ValueError("%s has fewer than num_physical_nodes+1 entries" % name)
+        st = st if n is None else st[:n + 1]
+        if st[0] != 0 or st[-1] != P or np.any(np.diff(st) < 0):
+            raise ValueError("%s is not a CSR start array over [0, num_pins)" % name)
+    deg = np.diff(np.asarray(netpin_start, dtype=np.int64))
+    owner = np.repeat(np.arange(len(deg)), deg)
+    if not np.array_equal(np.asarray(pin2net)[np.asarray(flat_netpin)], owner):
+        raise ValueError("pin2net_map disagrees with flat_net2pin_map / flat_net2pin_start_map")
+
+
 class PinPlan:
-    """Static plan of the pin-parallel WA kernel for a netlist: net groups of
-    at most 64 pins (dp_wa_pack_nets), the nets above that, and each pin's
-    position in node (CSR) order."""
+    """Static plan of the pin-parallel WA kernels for a netlist: nets packed by
+    degree into groups of at most 64 pins and their pins' group order
+    (dp_wa_pack_nets), the nets above 64 pins, and the node-order pin offsets,
+    destinations and group-order metadata of dp_wa_pins_prepare."""
 
     def __init__(self, dd):
         import ctypes
         start = dd.flat_net2pin_start_map.cpu().numpy().astype(np.int32)
         E = len(start) - 1
-        groups = np.zeros(2 * max(E, 1), dtype=np.int32)
-        big = np.zeros(max(E, 1), dtype=np.int32)
-        ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
-        N.call("dp_wa_pack_nets", start.ctypes.data_as(ctypes.c_void_p), E,
-               groups.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ng),
-               big.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nb))
-        self.num_groups, self.num_big = ng.value, nb.value
-        dev = dd.device
-        self.groups = torch.from_numpy(groups[:2 * max(self.num_groups, 1)]).to(dev)
-        self.big = torch.from_numpy(big[:max(self.num_big, 1)]).to(dev)
         P = dd.num_pins
-        order = torch.empty(P, dtype=torch.int32, device=dev)
-        order[dd.flat_node2pin_map.long()] = torch.arange(P, dtype=torch.int32, device=dev)
-        self.pin_node_order = order
-        self.ws = torch.empty(N.query("dp_wa_pins_workspace_bytes", dd.num_physical_nodes,
-                                      N.dtype_code(dd.pos)), dtype=torch.uint8, device=dev)
+        check_netlist(dd.flat_net2pin_map.cpu().numpy(), start, dd.pin2net_map.cpu().numpy(),
+                      dd.flat_node2pin_map.cpu().numpy(), dd.flat_node2pin_start_map.cpu().numpy(),
+                      P, dd.num_physical_nodes)
+        group_start = np.zeros(E + 1, dtype=np.int32)
+        big = np.zeros(4 * max(E, 1), dtype=np.int32)
+        net_slot = np.zeros(max(E, 1), dtype=np.int32)
+        ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        N.call("dp_wa_pack_nets", vp(start), E, vp(group_start), ctypes.byref(ng), vp(big),
+               ctypes.byref(nb), vp(net_slot))
+        self.num_groups, self.num_big, self.num_nets = ng.value, nb.value, E
+        dev = dd.device
+        self.group_start = torch.from_numpy(group_start[:self.num_groups + 1]).to(dev)
+        self.big = torch.from_numpy(big[:4 * max(self.num_big, 1)]).to(dev)
+        net_slot_d = torch.from_numpy(net_slot).to(dev)
+        code = N.dtype_code(dd.pos)
+        self.node_offsets = torch.empty(2 * P, dtype=dd.dtype, device=dev)
+        self.node_dst = torch.empty(P, dtype=torch.int32, device=dev)
+        self.slot_node = torch.empty(P, dtype=torch.int32, device=dev)
+        self.pin_meta = torch.empty(4 * P, dtype=torch.int32, device=dev)
+        prep_ws = torch.empty(N.query("dp_wa_prepare_workspace_bytes", P), dtype=torch.uint8,
+                              device=dev)
+        pin_mask = dd.pin_mask_ignore_fixed_macros
+        N.call("dp_wa_pins_prepare", code, P, dd.num_physical_nodes, N.ptr(dd.pin_offset_x),
+               N.ptr(dd.pin_offset_y), N.ptr(dd.flat_net2pin_map), N.ptr(dd.flat_net2pin_start_map),
+               N.ptr(dd.pin2net_map), N.ptr(pin_mask), N.ptr(dd.flat_node2pin_map),
+               N.ptr(dd.flat_node2pin_start_map), N.ptr(net_slot_d), N.ptr(prep_ws),
+               N.ptr(self.node_offsets), N.ptr(self.node_dst), N.ptr(self.slot_node),
+               N.ptr(self.pin_meta), N.stream_ptr(dev))
+        del net_slot_d
+        del prep_ws  # stream-ordered: the caching allocator reuses it only after the kernels
+        self.ws = torch.empty(N.query("dp_wa_pins_workspace_bytes", P, code), dtype=torch.uint8,
+                              device=dev)
         self.grad_node_order = torch.empty(2 * P, dtype=dd.dtype, device=dev)
 
 
 class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
-                 energy_mode="potential", convention="cpu", algorithm="global", schedule=None):
+                 energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
+                 keep_partial_wl=False):
+        """keep_partial_wl: also store the per-net (pre-weight) WA wirelength in
+        `partial_wl` (what the net-weight adjoint needs); the pins schedule
+        skips that store otherwise."""
         self.dd = dd
+        self.keep_partial_wl = keep_partial_wl
         dt = dd.dtype
         dev = dd.device
         self.fused = fused
@@ -132,7 +179,10 @@ class PlacementObjective:
         self.field_y = torch.empty_like(self.field_x)
         self.potential = torch.empty_like(self.field_x) if energy_mode == "potential" else None
         self.energy = torch.empty((), dtype=pdt, device=dev)
-        self.density_grad = z(2 * n)
+        # the pins schedule keeps the density gradient interleaved ((x, y) per
+        # node: one scattered word per cell in the gather); others keep [2N]
+        self.grad_layout = N.DP_LAYOUT_INTERLEAVED if self.schedule == "pins" else N.DP_LAYOUT_SPLIT
+        self._density_grad = z(2 * n)
         self.wl_grad = z(2 * n)
         self.grad = z(2 * n)
         self.obj = z(())
@@ -140,6 +190,13 @@ class PlacementObjective:
         self.w = dd.net_weights.contiguous()
         self.net_mask = dd.net_mask_ignore_large_degrees
         self.pin_mask = dd.pin_mask_ignore_fixed_macros
+
+    @property
+    def density_grad(self):
+        """Density gradient of the last evaluation in the reference [2N] layout."""
+        if self.grad_layout == N.DP_LAYOUT_INTERLEAVED:
+            return self._density_grad.view(-1, 2).t().reshape(-1)
+        return self._density_grad
 
     def set_gamma(self, gamma):
         self.inv_gamma.fill_(1.0 / gamma)
@@ -180,8 +237,8 @@ class PlacementObjective:
                N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
                ep.num_bins_x, ep.num_bins_y, ep.algorithm_code, ep.large_capacity,
-               N.ptr(ep.workspace(pos.device)),
-               N.ptr(self.density_grad), N.stream_ptr(pos.device))
+               self.grad_layout, N.ptr(ep.workspace(pos.device)),
+               N.ptr(self._density_grad), N.stream_ptr(pos.device))
 
     def evaluate(self, pos):
         """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers."""
@@ -191,18 +248,17 @@ class PlacementObjective:
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
         if self.schedule == "pins":
             pp = self.pin_plan
-            N.call("dp_wa_forward_pins", code, N.ptr(pos), n, dd.num_physical_nodes,
-                   N.ptr(dd.pin_offset_x), N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map),
-                   N.ptr(dd.flat_net2pin_map), N.ptr(dd.flat_net2pin_start_map),
-                   N.ptr(dd.pin2net_map), N.ptr(pp.groups), pp.num_groups, N.ptr(pp.big),
-                   pp.num_big, N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.pin_mask),
-                   N.ptr(pp.pin_node_order), N.ptr(self.inv_gamma), N.ptr(self.partial_wl),
+            N.call("dp_wa_forward_pins", code, N.ptr(pos), n, P, N.ptr(pp.slot_node),
+                   N.ptr(pp.node_offsets), N.ptr(pp.node_dst), N.ptr(pp.pin_meta),
+                   N.ptr(pp.group_start), pp.num_groups, N.ptr(pp.big), pp.num_big, E,
+                   N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.inv_gamma),
+                   N.ptr(self.partial_wl) if self.keep_partial_wl else None,
                    N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)
             self._density_forward(pos)
             self._density_backward(pos)
             N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
-                   N.ptr(dd.flat_node2pin_start_map), N.ptr(self.density_grad),
-                   dd.num_physical_nodes, n, N.ptr(self.grad), s)
+                   N.ptr(dd.flat_node2pin_start_map), N.ptr(self._density_grad),
+                   self.grad_layout, dd.num_physical_nodes, n, N.ptr(self.grad), s)
         elif self.schedule == "fused":
             N.call("dp_wa_forward_from_nodes", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), P, N.ptr(dd.flat_net2pin_map),
@@ -214,7 +270,7 @@ class PlacementObjective:
             N.call("dp_wa_backward_to_nodes", code, N.ptr(self.one), N.ptr(self.grad_intermediate),
                    P, N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    N.ptr(dd.pin2net_map), N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.pin_mask),
-                   N.ptr(self.density_grad), dd.num_physical_nodes, n, N.ptr(self.grad), s)
+                   N.ptr(self._density_grad), dd.num_physical_nodes, n, N.ptr(self.grad), s)
         else:
             N.call("dp_pin_pos_forward", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, P,
@@ -231,7 +287,7 @@ class PlacementObjective:
             N.call("dp_pin_pos_backward", code, N.ptr(self.grad_intermediate), P,
                    N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    dd.num_physical_nodes, n, N.ptr(self.wl_grad), s)
-            torch.add(self.wl_grad, self.density_grad, out=self.grad)
+            torch.add(self.wl_grad, self._density_grad, out=self.grad)
         energy = self.energy if self.energy.dtype == pos.dtype else self.energy.to(pos.dtype)
         torch.add(self.wl, energy, alpha=self.lam_host, out=self.obj)
         return self.obj, self.grad

@@ -48,25 +48,38 @@ enum { DP_SPECTRAL_FP64 = 0, DP_SPECTRAL_FP32 = 1, DP_SPECTRAL_BF16X3 = 2 };
 /* Density scatter algorithm: per-cell global int64 atomics, or cells binned
  * into LDS-resident bin tiles first. */
 enum { DP_SCATTER_GLOBAL = 0, DP_SCATTER_TILED = 1 };
+/* Layout of a per-node gradient: DP_LAYOUT_SPLIT is the reference's [2N]
+ * (all x, then all y); DP_LAYOUT_INTERLEAVED stores (x, y) of node i at
+ * [2i, 2i+1], so one node is one 8- (FP32) or 16-byte (FP64) word. */
+enum { DP_LAYOUT_SPLIT = 0, DP_LAYOUT_INTERLEAVED = 1 };
 
 const char* dp_last_error(void);
 int dp_version(void);
 
 /* Kernel probe (no counterpart in the reference): when enabled, the library
  * records `start_event` immediately before and `stop_event` immediately after
- * every launch of the named kernel, on the stream it is launched on (a launch
- * overwrites the previous pair).  kernel_id < 0 disables.  Events are
- * hipEvent_t passed as void*.  Used by the benchmark to time one kernel live. */
+ * every launch of the named kernel (DP_KERNEL_*), or around all launches of
+ * one call of the named entry point (DP_OP_*), on the stream they are
+ * launched on (a launch overwrites the previous pair).  kernel_id < 0
+ * disables.  Events are hipEvent_t passed as void*.  Used by the benchmark to
+ * time one kernel or op live. */
 enum {
-  DP_KERNEL_DENSITY_SCATTER = 0,
-  DP_KERNEL_ELECTRIC_FORCE = 1,
-  DP_KERNEL_WA_FORWARD = 2,
+  DP_KERNEL_DENSITY_SCATTER = 0,  /* k_tile_scatter / k_tri_scatter */
+  DP_KERNEL_ELECTRIC_FORCE = 1,   /* k_tile_gather / k_force_small */
+  DP_KERNEL_WA_FORWARD = 2,       /* k_wa_pins / k_wa_fwd_nodes / k_wa_fwd */
   DP_KERNEL_WA_BACKWARD = 3,
   DP_KERNEL_PIN_POS_FORWARD = 4,
-  DP_KERNEL_PIN_POS_BACKWARD = 5,
+  DP_KERNEL_PIN_POS_BACKWARD = 5, /* k_node_grad / k_wa_pin_pos_bwd / k_pin_pos_bwd */
   DP_KERNEL_SPECTRAL_GEMM = 6,
   DP_KERNEL_DENSITY_CONVERT = 7,
-  DP_KERNEL_COUNT = 8
+  DP_KERNEL_DENSITY_BIN = 8,      /* k_bin_fill */
+  DP_KERNEL_PIN_PUSH = 9,         /* k_pin_push */
+  DP_OP_DENSITY_MAP = 16,         /* dp_density_map */
+  DP_OP_ELECTRIC_FORCE = 17,      /* dp_electric_force */
+  DP_OP_WA_FORWARD = 18,          /* dp_wa_forward_pins / dp_wa_forward_from_nodes / dp_wa_merged_forward */
+  DP_OP_NODE_GRAD = 19,           /* dp_node_grad_from_pins / dp_wa_backward_to_nodes */
+  DP_OP_POISSON = 20,             /* dp_poisson_solve */
+  DP_KERNEL_COUNT = 32
 };
 int dp_profile_kernel(int kernel_id, void* start_event, void* stop_event);
 
@@ -124,14 +137,17 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
  * (nullable: 1).  workspace as for dp_density_map (same large_capacity).
  * With DP_SCATTER_TILED the gather reuses the cell binning that the last
  * dp_density_map call (DP_SCATTER_TILED, same workspace) made: `pos` must be
- * the positions of that call. */
+ * the positions of that call.  grad_layout: DP_LAYOUT_SPLIT (the reference's
+ * layout) or DP_LAYOUT_INTERLEAVED (one word per node, as
+ * dp_node_grad_from_pins reads it). */
 int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
                       const void* field_map_y, const void* pos, int64_t num_nodes,
                       const void* node_size_x_clamped, const void* node_size_y_clamped,
                       const void* offset_x, const void* offset_y, const void* ratio, double xl,
                       double yl, double bin_size_x, double bin_size_y, int64_t num_movable_nodes,
                       int64_t num_filler_nodes, int num_bins_x, int num_bins_y, int algorithm,
-                      int64_t large_capacity, void* workspace, void* grad_out, void* stream);
+                      int64_t large_capacity, int grad_layout, void* workspace, void* grad_out,
+                      void* stream);
 
 /* ElectricOverflow metric (electric_overflow.py:260-278): out2[0] =
  * sum(max(map - target_area, 0)), out2[1] = max(map); FP64 accumulation,
@@ -254,37 +270,64 @@ int dp_wa_backward_to_nodes(int dtype, const void* grad, const void* grad_interm
 
 /* Pin-parallel form of the fused wirelength chain (no reference counterpart:
  * the reference runs one thread per net, weighted_average_wirelength_cuda_
- * merged_kernel.cu:16-85).  Host routine: packs whole nets, in order, into
- * groups of at most 64 pins (groups[2g], groups[2g+1] = first net, one past
- * the last) and lists nets of more than 64 pins in big_nets.  Capacities:
- * num_nets groups and num_nets big nets.  Plain host memory; no GPU work. */
-int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* groups,
-                    int64_t* num_groups, int32_t* big_nets, int64_t* num_big);
+ * merged_kernel.cu:16-85).  Host routine, plain host memory, no GPU work:
+ * packs the nets of 1..64 pins best-fit-decreasing by degree into groups of
+ * at most 64 pins and lays their pins out in "group order": group g owns
+ * slots [group_start[g], group_start[g+1]), a net's pins are contiguous in
+ * it, in flat_netpin order.  Nets of more than 64 pins follow, one per
+ * big_nets entry {net, first slot, degree, 0}.  net_slot[e] = first slot of
+ * net e.  Capacities: group_start num_nets+1, big_nets 4*num_nets, net_slot
+ * num_nets. */
+int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* group_start,
+                    int64_t* num_groups, int32_t* big_nets, int64_t* num_big, int32_t* net_slot);
 
-size_t dp_wa_pins_workspace_bytes(int64_t num_physical_nodes, int dtype);
+/* Static preparation of the pin-parallel form for a netlist (run once; the
+ * pin offsets, maps and pin mask are constant during placement).  Writes
+ *   node_offsets[2P]  (offset_x, offset_y) of the pin in node-order slot i
+ *                     (slot i of flat_node2pin_map), interleaved;
+ *   node_dst[P]       the pin's slot in group order;
+ *   slot_node[P]      the physical node owning node-order slot i;
+ *   pin_meta[4P]      per group-order slot s: {net | sign bit when pin_mask
+ *                     marks the pin (pin_mask nullable), node-order slot,
+ *                     rank in the net | degree << 8 (nets of <= 64 pins), 0}.
+ * flat_netpin and flat_node2pin_map must both be permutations of
+ * [0, num_pins) and flat_node2pin_start_map must cover them over the physical
+ * nodes, as PlaceDB builds them.  workspace:
+ * dp_wa_prepare_workspace_bytes(num_pins). */
+size_t dp_wa_prepare_workspace_bytes(int64_t num_pins);
+int dp_wa_pins_prepare(int dtype, int64_t num_pins, int64_t num_physical_nodes,
+                       const void* pin_offset_x, const void* pin_offset_y,
+                       const int32_t* flat_netpin, const int32_t* netpin_start,
+                       const int32_t* pin2net_map, const uint8_t* pin_mask,
+                       const int32_t* flat_node2pin_map, const int32_t* flat_node2pin_start_map,
+                       const int32_t* net_slot, void* workspace, void* node_offsets,
+                       int32_t* node_dst, int32_t* slot_node, int32_t* pin_meta, void* stream);
+
+size_t dp_wa_pins_workspace_bytes(int64_t num_pins, int dtype);
 
 /* The chain WA(PinPos(pos)) with one wavefront per net group and one lane per
- * pin.  Outputs partial_wl (nullable) and *wl_out as dp_wa_merged_forward, and
- * grad_node_order[2P]: pin p's derivative (x, y interleaved), times its net
- * weight, 0 for masked nets and pins, stored at index pin_node_order[p] (the
- * pin's position in flat_node2pin_map). */
-int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_physical_nodes,
-                       const void* pin_offset_x, const void* pin_offset_y,
-                       const int32_t* pin2node_map, const int32_t* flat_netpin,
-                       const int32_t* netpin_start, const int32_t* pin2net_map,
-                       const int32_t* groups, int64_t num_groups, const int32_t* big_nets,
-                       int64_t num_big, const void* net_weights, const uint8_t* net_mask,
-                       const uint8_t* pin_mask, const int32_t* pin_node_order,
-                       const void* inv_gamma, void* partial_wl, void* grad_node_order,
-                       void* wl_out, void* workspace, void* stream);
+ * pin.  Each pin slot in node order first pushes pos + offset to the pin's
+ * group-order slot; the WA kernel then reads pins contiguously.  Outputs
+ * partial_wl[num_nets] (nullable; 0 for masked nets) and *wl_out as
+ * dp_wa_merged_forward, and grad_node_order[2P]: each pin's derivative
+ * (x, y interleaved), times its net weight, 0 for masked nets and pins,
+ * stored at the pin's node-order slot. */
+int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                       const int32_t* slot_node, const void* node_offsets,
+                       const int32_t* node_dst, const int32_t* pin_meta,
+                       const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
+                       int64_t num_big, int64_t num_nets, const void* net_weights,
+                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
+                       void* grad_node_order, void* wl_out, void* workspace, void* stream);
 
 /* Backward of that chain plus the gradient combine: grad_pos[n] =
  * (*grad) * (sum of grad_node_order over node n's CSR range) + add[n]
- * (add nullable), read contiguously; fillers get add only. */
+ * (add nullable, in add_layout: DP_LAYOUT_SPLIT or DP_LAYOUT_INTERLEAVED),
+ * read contiguously; fillers get add only.  grad_pos_out is split. */
 int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,
                            const int32_t* flat_node2pin_start_map, const void* add,
-                           int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
-                           void* stream);
+                           int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
+                           void* grad_pos_out, void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
  * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64

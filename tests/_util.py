@@ -24,6 +24,59 @@ def density_params(oracle, d):
                                 d.num_filler_nodes, d.target_density, d.movable_macro_mask)
 
 
+def off_grid(d, seed=3):
+    """A copy of design `d` with some movable and filler cells moved past each
+    die edge: far outside (no bin at all), and just outside or straddling an
+    edge, where the truncated bin index still names an edge bin (a partial or
+    negative overlap, as the reference computes it)."""
+    rng = np.random.default_rng(seed)
+    d2 = synth.Design(d)
+    n = d.num_nodes
+    pos = d.pos.copy()
+    cells = np.concatenate([np.arange(d.num_movable_nodes), np.arange(n - d.num_filler_nodes, n)])
+    pick = rng.choice(cells, size=min(96, len(cells)), replace=False)
+    bx, by = d.bin_size_x, d.bin_size_y
+    for j, i in enumerate(pick):
+        w, h = d.node_size_x[i], d.node_size_y[i]
+        kind = j % 8
+        if kind == 0:
+            pos[i] = d.xl - 10 * bx - w                   # far left
+        elif kind == 1:
+            pos[i] = d.xl - w - 0.4 * bx                  # ends just left of the die
+        elif kind == 2:
+            pos[i] = d.xh + 5 * bx                        # far right
+        elif kind == 3:
+            pos[i] = d.xh - 0.5 * w                       # straddles the right edge
+        elif kind == 4:
+            pos[n + i] = d.yl - 10 * by - h               # far below
+        elif kind == 5:
+            pos[n + i] = d.yl - h - 0.3 * by              # ends just below the die
+        elif kind == 6:
+            pos[n + i] = d.yh + 7 * by                    # far above
+        else:
+            pos[n + i] = d.yh - 0.5 * h                   # straddles the top edge
+    d2.pos = pos.astype(d.pos.dtype)
+    return d2
+
+
+def relabel_pins(d, seed=4):
+    """A copy of design `d` whose pins carry new, random ids.  Each net's pin
+    sequence (flat_net2pin_map) and each node's (flat_node2pin_map) is kept,
+    only renamed, so every per-net and per-node sum runs in the same order."""
+    rng = np.random.default_rng(seed)
+    P = d.num_pins
+    perm = rng.permutation(P).astype(np.int32)  # new id of old pin p
+    d2 = synth.Design(d)
+    for k in ("pin_offset_x", "pin_offset_y", "pin2node_map", "pin2net_map",
+              "pin_mask_ignore_fixed_macros"):
+        v = np.empty_like(d[k])
+        v[perm] = d[k]
+        d2[k] = v
+    d2.flat_net2pin_map = perm[d.flat_net2pin_map]
+    d2.flat_node2pin_map = perm[d.flat_node2pin_map]
+    return d2
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

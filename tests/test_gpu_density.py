@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from _util import density_params, design, to_dev
+from _util import density_params, design, off_grid, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +118,51 @@ def test_electric_force(oracle, gpu, case, dtype, tol, algorithm):
     nm, nt = d.num_movable_nodes, d.num_terminals
     n = d.num_nodes
     assert np.all(got[nm:nm + nt] == 0) and np.all(got[n + nm:n + nm + nt] == 0)
+
+
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)], ids=["f64", "f32"])
+def test_off_grid_cells(oracle, gpu, dtype, tol, algorithm):
+    """Cells past the die edges (no bin, or a truncated index naming an edge
+    bin): bit-exact map, force as the oracle, zero force where no bin."""
+    print("case off_grid %s %s" % (np.dtype(dtype).name, algorithm), flush=True)
+    d = off_grid(design(0, dtype=dtype))
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype, algorithm=algorithm)
+    op.compute_initial_density_map(t["pos"])
+    raw = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
+    dmap = op.density_map(t["pos"], fixed_map_out=raw)
+    dp = density_params(oracle, d)
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    assert np.array_equal(raw.cpu().numpy(), oracle.density_fixed_sum(dp, d.pos))
+    assert np.array_equal(dmap.cpu().numpy(), oracle.density_map(dp, d.pos, init))
+    rng = np.random.default_rng(5)
+    fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    g = torch.tensor(1.5, dtype=t["pos"].dtype, device=gpu)
+    out = op.electric_force(t["pos"], torch.from_numpy(fx).to(gpu), torch.from_numpy(fy).to(gpu), g)
+    ref = oracle.electric_force(dp, d.pos, fx, fy, grad=1.5)
+    got = out.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= tol * np.abs(ref).max()
+    assert np.all(got[ref == 0] == 0)
+
+
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("case", CASES, ids=[c[2] for c in CASES])
+def test_electric_force_interleaved_layout(gpu, case, dtype, algorithm):
+    """The interleaved gradient is the split one, word for word."""
+    cfg, kw, name = case
+    print("case force layout %s %s %s" % (name, np.dtype(dtype).name, algorithm), flush=True)
+    d = design(cfg, dtype=dtype, **kw)
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype, algorithm=algorithm)
+    op.compute_initial_density_map(t["pos"])
+    op.density_map(t["pos"])
+    rng = np.random.default_rng(9)
+    fx = torch.from_numpy(rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)).to(gpu)
+    fy = torch.from_numpy(rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)).to(gpu)
+    g = torch.tensor(0.5, dtype=t["pos"].dtype, device=gpu)
+    split = op.electric_force(t["pos"], fx, fy, g)
+    inter = op.electric_force(t["pos"], fx, fy, g, grad_layout="interleaved")
+    assert torch.equal(inter.view(-1, 2).t().reshape(-1), split)

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import torch
 
-from _util import design, rel_l2
+from _util import design, rel_l2, relabel_pins
 
 pytestmark = pytest.mark.gpu
 
@@ -13,7 +13,7 @@ def _objective(d, dev, lam, fused, precision, dtype=None, algorithm="global", sc
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
     dd = DeviceDesign(d, dev, dtype=dtype)
     return dd, PlacementObjective(dd, d.wa_gamma, density_weight=lam, precision=precision, fused=fused,
-                                  algorithm=algorithm, schedule=schedule)
+                                  algorithm=algorithm, schedule=schedule, keep_partial_wl=True)
 
 
 @pytest.mark.parametrize("algorithm", ["global", "tiled"])
@@ -99,6 +99,25 @@ def test_engine_pins_schedule(oracle, gpu, cfg, kw, dtype, tol):
                                            d64.flat_net2pin_map, d64.flat_net2pin_start_map, d64.net_weights,
                                            d64.net_mask_ignore_large_degrees, d64.wa_gamma)[1]) <= tol
     assert rel_l2(g.double().cpu().numpy(), rg) <= tol
+
+
+@pytest.mark.parametrize("schedule", ["pins", "fused", "unfused"])
+def test_engine_pin_relabelling(gpu, schedule):
+    """Pin ids in no particular order (flat_net2pin_map not the identity):
+    renaming the pins while keeping each net's and node's pin sequence gives
+    the identical objective and gradient."""
+    print("case engine pin relabelling %s" % schedule, flush=True)
+    d = design(0, big_nets=20)
+    d2 = relabel_pins(d)
+    assert not np.array_equal(d2.flat_net2pin_map, np.arange(d.num_pins))
+    dd1, o1 = _objective(d, gpu, 0.3, schedule != "unfused", "fp64", algorithm="tiled",
+                         schedule=schedule)
+    dd2, o2 = _objective(d2, gpu, 0.3, schedule != "unfused", "fp64", algorithm="tiled",
+                         schedule=schedule)
+    f1, g1 = o1.evaluate(dd1.pos)
+    f2, g2 = o2.evaluate(dd2.pos)
+    assert torch.equal(g1, g2)
+    assert f1.item() == f2.item()
 
 
 def test_engine_pins_equals_unfused_fp64(gpu):

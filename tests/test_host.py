@@ -115,3 +115,69 @@ def test_fence_regions_not_supported():
                           1.0, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
                           d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
                           T(d.sorted_node_map), region_id=0)
+
+
+def _pack(start):
+    import ctypes
+    from dreamplace_amd import _native as N
+    E = len(start) - 1
+    start = np.ascontiguousarray(start, dtype=np.int32)
+    gs = np.zeros(E + 1, dtype=np.int32)
+    big = np.zeros(4 * max(E, 1), dtype=np.int32)
+    slot = np.zeros(max(E, 1), dtype=np.int32)
+    ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    N.call("dp_wa_pack_nets", vp(start), E, vp(gs), ctypes.byref(ng), vp(big), ctypes.byref(nb),
+           vp(slot))
+    return gs[:ng.value + 1], big[:4 * nb.value].reshape(-1, 4), slot[:E]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_wa_pack_nets_layout(seed):
+    """Host packing (no GPU work): groups of <= 64 pins made of whole nets,
+    every net's pins contiguous and every slot owned exactly once, nets of
+    more than 64 pins after the groups, and the packing nearly full."""
+    rng = np.random.default_rng(seed)
+    deg = synth._degrees(rng, 20000, 200)
+    deg[rng.choice(len(deg), 40, replace=False)] = rng.choice([0, 1, 64, 65, 130], 40)
+    start = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    gs, big, slot = _pack(start)
+    P = int(start[-1])
+    owner = np.full(P, -1)
+    for e, d in enumerate(deg):
+        if d > 0:
+            assert np.all(owner[slot[e]:slot[e] + d] == -1)
+            owner[slot[e]:slot[e] + d] = e
+    assert np.all(owner >= 0)                              # every slot owned exactly once
+    small = int(deg[(deg >= 1) & (deg <= 64)].sum())
+    assert gs[0] == 0 and gs[-1] == small and np.all(np.diff(gs) >= 1) and np.all(np.diff(gs) <= 64)
+    for g in range(len(gs) - 1):                           # groups hold whole nets
+        nets = set(owner[gs[g]:gs[g + 1]])
+        assert all(slot[e] >= gs[g] and slot[e] + deg[e] <= gs[g + 1] for e in nets)
+    assert sorted(big[:, 0]) == sorted(np.nonzero(deg > 64)[0])
+    assert np.all(big[:, 2] == deg[big[:, 0]]) and np.all(big[:, 1] == slot[big[:, 0]])
+    assert np.all(big[:, 1] >= small)
+    assert len(gs) - 1 <= math.ceil(small / 64 * 1.02) + 1    # best-fit decreasing packs tightly
+
+
+def test_netlist_check_rejects_bad_maps():
+    from dreamplace_amd.engine import check_netlist
+    d = synth.make_design(0)
+    args = [d.flat_net2pin_map, d.flat_net2pin_start_map, d.pin2net_map, d.flat_node2pin_map,
+            d.flat_node2pin_start_map, d.num_pins, d.num_physical_nodes]
+    check_netlist(*args)
+    bad = list(args)
+    bad[0] = d.flat_net2pin_map.copy()
+    bad[0][0] = bad[0][1]
+    with pytest.raises(ValueError):
+        check_netlist(*bad)
+    bad = list(args)
+    bad[2] = d.pin2net_map.copy()
+    bad[2][d.flat_net2pin_map[0]] += 1
+    with pytest.raises(ValueError):
+        check_netlist(*bad)
+    bad = list(args)
+    bad[4] = d.flat_node2pin_start_map.copy()
+    bad[4][-1] -= 1
+    with pytest.raises(ValueError):
+        check_netlist(*bad)

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 import torch
 
-from _util import density_params, design, rel_l2
+from _util import density_params, design, off_grid, rel_l2, relabel_pins
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -144,6 +144,33 @@ def test_electric_force_matches_reference(oracle, ref, dtype):
         t["pos"], t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"], t["bcx"], t["bcy"], d.xl, d.yl, d.xh,
         d.yh, d.bin_size_x, d.bin_size_y, d.num_movable_nodes, d.num_filler_nodes)
     mine = oracle.electric_force(dp, d.pos, fx, fy, grad=0.5)
+    assert np.array_equal(mine, -out.numpy())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_off_grid_cells_match_reference(oracle, ref, dtype):
+    """Cells past the die edges: no bin at all, or a truncated index naming an
+    edge bin (partial or negative overlap); map and force as the reference."""
+    d = off_grid(design(0, dtype=dtype))
+    dp = density_params(oracle, d)
+    t = _ref_density_inputs(d, dp, dtype)
+    ep = ref["electric_potential_cpp"]
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    m_ref = ep.density_map(t["pos"], t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"], t["bcx"], t["bcy"],
+                           torch.from_numpy(init), d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
+                           d.bin_size_y, d.num_movable_nodes, d.num_filler_nodes, 0, d.num_bins_x,
+                           d.num_bins_y, 3, 3, 3, 3, 1)
+    m = oracle.density_map(dp, d.pos, init)
+    assert np.array_equal(m.reshape(-1), m_ref.numpy().reshape(-1))
+    rng = np.random.default_rng(12)
+    fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    g = torch.tensor([1.25], dtype=torch.from_numpy(fx).dtype)
+    out = ep.electric_force(
+        g, d.num_bins_x, d.num_bins_y, 3, 3, 3, 3, torch.from_numpy(fx).view(-1), torch.from_numpy(fy).view(-1),
+        t["pos"], t["wxc"], t["wyc"], t["offx"], t["offy"], t["ratio"], t["bcx"], t["bcy"], d.xl, d.yl, d.xh,
+        d.yh, d.bin_size_x, d.bin_size_y, d.num_movable_nodes, d.num_filler_nodes)
+    mine = oracle.electric_force(dp, d.pos, fx, fy, grad=1.25)
     assert np.array_equal(mine, -out.numpy())
 
 
@@ -466,6 +493,18 @@ def test_pin_weight_sum_closed_form(oracle):
     np.add.at(expect, d.pin2node_map, w[d.pin2net_map])
     got = oracle.pin_weight_sum(w, d.flat_node2pin_map, d.flat_node2pin_start_map, d.pin2net_map, d.num_nodes)
     assert np.allclose(got, expect, rtol=1e-13)
+
+
+def test_objective_invariant_under_pin_relabelling(oracle):
+    """Pin ids carry no meaning beyond the net and node pin sequences: renaming
+    them gives the identical objective and gradient (the GPU schedules that
+    permute pins between net and node order are held to the same)."""
+    d = design(0, big_nets=5)
+    d2 = relabel_pins(d)
+    assert not np.array_equal(d2.flat_net2pin_map, d.flat_net2pin_map)
+    f1, g1, _ = oracle.objective_and_gradient(d, 0.3, d.wa_gamma)
+    f2, g2, _ = oracle.objective_and_gradient(d2, 0.3, d2.wa_gamma)
+    assert f1 == f2 and np.array_equal(g1, g2)
 
 
 def test_planted_sign_error_is_caught(oracle, monkeypatch):
