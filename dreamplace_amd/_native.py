@@ -36,11 +36,11 @@ _SIGNATURES = {
     "dp_fixed_point_scale": (_L, [_D, _D, _D, _D]),
     "dp_density_map_workspace_bytes": (_S, [_L, _L, _L, _L, _I, _I]),
     "dp_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L,
-                            _I, _I, _I, _I, _L, _P, _P, _P, _P]),
+                            _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
-                               _I, _I, _I, _L, _I, _P, _P, _P]),
+                               _I, _I, _I, _L, _I, _P, _P, _P, _P]),
     "dp_reduce_workspace_bytes": (_S, [_L]),
     "dp_density_overflow": (_I, [_I, _P, _L, _D, _P, _P, _P]),
     "dp_spectral_plan_bytes": (_S, [_I, _I, _I]),
@@ -118,6 +118,13 @@ def ptr(t):
     if t is None:
         return None
     return ctypes.c_void_p(t.data_ptr())
+
+
+def host_doubles(values):
+    """A host double array for an argument read during the call (None for None)."""
+    if values is None:
+        return None
+    return (ctypes.c_double * len(values))(*[float(v) for v in values])
 
 
 def require_gpu(*tensors, name="dreamplace_amd"):

@@ -223,7 +223,8 @@ static DensityWorkspace carve(void* ws, int64_t nb, int64_t qcap) {
 template <typename T>
 TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, const void* wyc,
                          const void* offx, const void* offy, const void* ratio, double xl,
-                         double yl, double bsx, double bsy, int nbx, int nby, int64_t scale) {
+                         double yl, double bsx, double bsy, int nbx, int nby, int64_t scale,
+                         const double* uniform_filler, int64_t fill_begin) {
   TriArgs<T> a;
   a.x = (const T*)pos;
   a.y = (const T*)pos + num_nodes;
@@ -242,15 +243,25 @@ TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, co
   a.nbx = nbx;
   a.nby = nby;
   a.scale = (T)scale;
+  // the shared filler values come from arrays of T, so (T) is exact
+  a.uni = uniform_filler != nullptr;
+  a.fill_begin = fill_begin;
+  a.uwx = a.uni ? (T)uniform_filler[0] : T(0);
+  a.uwy = a.uni ? (T)uniform_filler[1] : T(0);
+  a.uox = a.uni ? (T)uniform_filler[2] : T(0);
+  a.uoy = a.uni ? (T)uniform_filler[3] : T(0);
+  a.ur = a.uni ? (T)uniform_filler[4] : T(0);
   return a;
 }
 
 template TriArgs<float> make_tri_args<float>(const void*, int64_t, const void*, const void*,
                                              const void*, const void*, const void*, double,
-                                             double, double, double, int, int, int64_t);
+                                             double, double, double, int, int, int64_t,
+                                             const double*, int64_t);
 template TriArgs<double> make_tri_args<double>(const void*, int64_t, const void*, const void*,
                                                const void*, const void*, const void*, double,
-                                               double, double, double, int, int, int64_t);
+                                               double, double, double, int, int, int64_t,
+                                               const double*, int64_t);
 
 template <typename T>
 static int scatter_global(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
@@ -299,7 +310,8 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
                    double xh, double yh, double bin_size_x, double bin_size_y,
                    int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
                    int num_bins_y, int convention, int algorithm, int64_t large_capacity,
-                   void* workspace, void* density_map_out, int64_t* fixed_map_out, void* stream) {
+                   const double* uniform_filler, void* workspace, void* density_map_out,
+                   int64_t* fixed_map_out, void* stream) {
   DP_ARG(pos && node_size_x_clamped && node_size_y_clamped && offset_x && offset_y && ratio,
          "dp_density_map: null input array");
   DP_ARG(workspace, "dp_density_map: null workspace");
@@ -324,7 +336,8 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
   return DP_DISPATCH(dtype, "dp_density_map", [&] {
     TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
-                                    num_bins_x, num_bins_y, scale);
+                                    num_bins_x, num_bins_y, scale, uniform_filler,
+                                    num_nodes - num_filler_nodes);
     const T* init = (const T*)initial_density_map;
     if (convention == DP_CONVERT_GPU && init) {
       hipLaunchKernelGGL(k_quantize<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, init, nb,

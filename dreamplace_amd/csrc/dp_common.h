@@ -119,6 +119,15 @@ __device__ __forceinline__ V wave_max(V v) {
   return v;
 }
 
+// Orders LDS accesses among the lanes of one wavefront without a workgroup
+// barrier: a wavefront's LDS instructions execute in program order, so it is
+// enough that the compiler moves no LDS access across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Block sum of a double in a fixed order (deterministic): waves reduce with
 // the xor butterfly, then wave 0 adds the per-wave results in wave order.
 template <int BLOCK>

@@ -13,6 +13,11 @@ struct TriArgs {
   const T *x, *y, *wxc, *wyc, *offx, *offy, *ratio;
   T xl, yl, bsx, bsy, inv_x, inv_y, scale;
   int nbx, nby;
+  // uniform fillers: nodes >= fill_begin all have these clamped sizes,
+  // offsets and ratio, which are then not loaded per cell
+  int uni;
+  int64_t fill_begin;
+  T uwx, uwy, uox, uoy, ur;
 };
 
 // Overlap of [x, x+s) with bin k along one axis:
@@ -56,11 +61,19 @@ __device__ __forceinline__ RawCell<T> load_cell(const TriArgs<T>& a, int64_t i) 
   RawCell<T> c;
   c.x = a.x[i];
   c.y = a.y[i];
-  c.wx = a.wxc[i];
-  c.wy = a.wyc[i];
-  c.ox = a.offx[i];
-  c.oy = a.offy[i];
-  c.r = a.ratio[i];
+  if (a.uni && i >= a.fill_begin) {
+    c.wx = a.uwx;
+    c.wy = a.uwy;
+    c.ox = a.uox;
+    c.oy = a.uoy;
+    c.r = a.ur;
+  } else {
+    c.wx = a.wxc[i];
+    c.wy = a.wyc[i];
+    c.ox = a.offx[i];
+    c.oy = a.offy[i];
+    c.r = a.ratio[i];
+  }
   return c;
 }
 
@@ -105,7 +118,8 @@ __device__ __forceinline__ void put_grad(T* out, int64_t num_nodes, int64_t i, T
 template <typename T>
 TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, const void* wyc,
                          const void* offx, const void* offy, const void* ratio, double xl,
-                         double yl, double bsx, double bsy, int nbx, int nby, int64_t scale);
+                         double yl, double bsx, double bsy, int nbx, int nby, int64_t scale,
+                         const double* uniform_filler, int64_t fill_begin);
 
 // Tiled (LDS-binned) scatter and gather, density_tiled.hip.
 bool tiled_supported(int nbx, int nby);

@@ -118,8 +118,9 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                  double xl, double yl, double bin_size_x, double bin_size_y,
                                  int64_t num_movable_nodes, int64_t num_filler_nodes,
                                  int num_bins_x, int num_bins_y, int algorithm,
-                                 int64_t large_capacity, int grad_layout, void* workspace,
-                                 void* grad_out, void* stream) {
+                                 int64_t large_capacity, int grad_layout,
+                                 const double* uniform_filler, void* workspace, void* grad_out,
+                                 void* stream) {
   DP_ARG(field_map_x && field_map_y && pos && node_size_x_clamped && node_size_y_clamped &&
              offset_x && offset_y && ratio && grad_out && workspace,
          "dp_electric_force: null pointer");
@@ -146,7 +147,8 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
   return DP_DISPATCH(dtype, "dp_electric_force", [&] {
     TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
-                                    num_bins_x, num_bins_y, 1);
+                                    num_bins_x, num_bins_y, 1, uniform_filler,
+                                    num_nodes - num_filler_nodes);
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y))
       return gather_tiled<T>(a, num_nodes, num_movable_nodes, num_filler_nodes,
                              (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,

@@ -74,23 +74,60 @@ __global__ void k_plan_fill(T* plan, int M, int N) {
 }
 
 // ---------------------------------------------------------------------------
-// GEMM C[MxN] = alpha * A[MxK] * B[KxN] (row-major), batched over blockIdx.z.
-// Optional epilogue: per-block FP64 partial of sum(C .* R) (the energy).
+// GEMM C[MxN] = alpha * A[MxK] * B[KxN] (row-major), a batch of up to 3.
+// Optional epilogue: per-tile FP64 partial of sum(C .* R) (the energy).
 // ---------------------------------------------------------------------------
 struct GemmDesc {
   const void* A;
   const void* B;
   void* C;
   const void* R;      // nullable: energy reference map [M x N]
-  double* partials;   // per-block partials of sum(C*R) when R != null
+  double* partials;   // per-tile partials of sum(C*R) when R != null
 };
 struct GemmBatch {
   GemmDesc g[3];
   int M, N, K;
   double alpha;
+  int TY, TX, count;  // tile grid of one GEMM, GEMMs in the batch
 };
 
-constexpr int BT = 64;  // block tile (M and N)
+constexpr int BT = 64;          // block tile (M and N)
+constexpr int kXcds = 8;        // accelerator dies; workgroup ids go round-robin over them
+constexpr int kGroupRows = 4;   // tile rows per raster band
+
+struct TileIdx {
+  int g, ty, tx;
+  bool ok;
+};
+
+// The MFMA kernels run on a 1-D grid of kXcds * ceil(tiles / kXcds)
+// workgroups.  Workgroup b runs on die b % kXcds, so die x is given the
+// contiguous range [x*per, (x+1)*per) of a banded raster over all tiles:
+// its tiles form a compact block of rows and columns and the A and B slabs
+// they read stay in that die's L2.
+__device__ __forceinline__ TileIdx map_tile(const GemmBatch& bt) {
+  const int T = bt.TY * bt.TX, total = T * bt.count;
+  const int per = (total + kXcds - 1) / kXcds;
+  const int b = blockIdx.x;
+  const int q = (b % kXcds) * per + b / kXcds;
+  TileIdx r;
+  r.ok = q < total;
+  r.g = r.ty = r.tx = 0;
+  if (!r.ok) return r;
+  r.g = q / T;
+  const int t = q - r.g * T;
+  const int band = t / (kGroupRows * bt.TX);
+  const int row0 = band * kGroupRows;
+  const int rows = bt.TY - row0 < kGroupRows ? bt.TY - row0 : kGroupRows;
+  const int w = t - band * kGroupRows * bt.TX;
+  r.ty = row0 + w % rows;
+  r.tx = w / rows;
+  return r;
+}
+
+__device__ __forceinline__ const GemmDesc& desc_of(const GemmBatch& bt, int g) {
+  return g == 0 ? bt.g[0] : (g == 1 ? bt.g[1] : bt.g[2]);
+}
 
 // ---- FP32: v_mfma_f32_32x32x2_f32, K tile 32 --------------------------------
 // LDS rows hold 32 k values permuted as pos(k) = (k&1)*16 + (k>>1), so lane
@@ -99,29 +136,36 @@ constexpr int F32_BK = 32;
 constexpr int F32_LD = 36;  // padded row stride (floats)
 
 __global__ void __launch_bounds__(256) k_gemm_f32(GemmBatch bt) {
-  const GemmDesc& gd = bt.g[blockIdx.z];
+  const TileIdx ti = map_tile(bt);
+  if (!ti.ok) return;  // whole workgroup: before any barrier
+  const GemmDesc& gd = desc_of(bt, ti.g);
   const float* A = (const float*)gd.A;
   const float* B = (const float*)gd.B;
   float* C = (float*)gd.C;
-  const int M = bt.M, N = bt.N, K = bt.K;
-  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  const int N = bt.N, K = bt.K;
+  const int m0 = ti.ty * BT, n0 = ti.tx * BT;
   __shared__ __attribute__((aligned(16))) float As[BT * F32_LD];
   __shared__ __attribute__((aligned(16))) float Bs[BT * F32_LD];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int wm = wid >> 1, wn = wid & 1;
   const int h = lane >> 5, l32 = lane & 31;
   f32x16 acc = {};
-  // global load assignment
+  // global load assignment: A 8 consecutive k of one row, B 8 consecutive k
+  // of one column (each B load coalesced across lanes); the next K tile is
+  // loaded into registers while the current one feeds the MFMAs
   const int a_row = t >> 2, a_kc = (t & 3) * 8;
   const int b_col = t & 63, b_kc = (t >> 6) * 8;
-  for (int k0 = 0; k0 < K; k0 += F32_BK) {
-    // A: 8 consecutive k of one row
+  float4 a0, a1;
+  float bv[8];
+  auto load = [&](int k0) {
     const float4* ap = (const float4*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
-    float4 a0 = ap[0], a1 = ap[1];
-    // B: 8 consecutive k of one column (each load coalesced across lanes)
-    float bv[8];
+    a0 = ap[0];
+    a1 = ap[1];
 #pragma unroll
     for (int j = 0; j < 8; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += F32_BK) {
     __syncthreads();
     {
       float* r = As + a_row * F32_LD;
@@ -134,6 +178,7 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmBatch bt) {
       *(float4*)(q + 16 + f) = make_float4(bv[1], bv[3], bv[5], bv[7]);
     }
     __syncthreads();
+    if (k0 + F32_BK < K) load(k0 + F32_BK);
     const float* ar = As + (wm * 32 + l32) * F32_LD + h * 16;
     const float* br = Bs + (wn * 32 + l32) * F32_LD + h * 16;
 #pragma unroll
@@ -161,14 +206,13 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmBatch bt) {
   if (R) {
     __shared__ double smem[4];
     double bs = block_sum<256>(e, smem);
-    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+    if (t == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
   }
-  (void)M;
 }
 
-// ---- BF16x3: v_mfma_f32_32x32x16_bf16, K tile 32 ---------------------------
-constexpr int BF_BK = 32;
-constexpr int BF_LD = 40;  // padded row stride (bf16 elements): 80 bytes
+// ---- BF16x3: v_mfma_f32_32x32x16_bf16, K tile 64 ---------------------------
+constexpr int BF_BK = 64;
+constexpr int BF_LD = 72;  // padded row stride (bf16 elements): 144 bytes
 
 __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
 #pragma unroll
@@ -180,12 +224,14 @@ __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
 }
 
 __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
-  const GemmDesc& gd = bt.g[blockIdx.z];
+  const TileIdx ti = map_tile(bt);
+  if (!ti.ok) return;  // whole workgroup: before any barrier
+  const GemmDesc& gd = desc_of(bt, ti.g);
   const float* A = (const float*)gd.A;
   const float* B = (const float*)gd.B;
   float* C = (float*)gd.C;
   const int N = bt.N, K = bt.K;
-  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  const int m0 = ti.ty * BT, n0 = ti.tx * BT;
   __shared__ __attribute__((aligned(16))) __bf16 Ah[BT * BF_LD];
   __shared__ __attribute__((aligned(16))) __bf16 Al[BT * BF_LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[BT * BF_LD];
@@ -194,24 +240,42 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
   const int wm = wid >> 1, wn = wid & 1;
   const int h = lane >> 5, l32 = lane & 31;
   f32x16 acc = {};
-  const int a_row = t >> 2, a_kc = (t & 3) * 8;
-  const int b_col = t & 63, b_kc = (t >> 6) * 8;
-  for (int k0 = 0; k0 < K; k0 += BF_BK) {
+  // A: 16 consecutive k of one row; B: 16 consecutive k of one column.  The
+  // next K tile is loaded into registers while the current one is computed.
+  const int a_row = t >> 2, a_kc = (t & 3) * 16;
+  const int b_col = t & 63, b_kc = (t >> 6) * 16;
+  float av[16], bv[16];
+  auto load = [&](int k0) {
     const float4* ap = (const float4*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
-    float4 a0 = ap[0], a1 = ap[1];
-    float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-    float bv[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
-    bf16x8 ahi, alo, bhi, blo;
-    split8(av, ahi, alo);
-    split8(bv, bhi, blo);
+    for (int j = 0; j < 4; ++j) {
+      const float4 v = ap[j];
+      av[4 * j] = v.x;
+      av[4 * j + 1] = v.y;
+      av[4 * j + 2] = v.z;
+      av[4 * j + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += BF_BK) {
+    bf16x8 ahi0, alo0, ahi1, alo1, bhi0, blo0, bhi1, blo1;
+    split8(av, ahi0, alo0);
+    split8(av + 8, ahi1, alo1);
+    split8(bv, bhi0, blo0);
+    split8(bv + 8, bhi1, blo1);
     __syncthreads();
-    *(bf16x8*)(Ah + a_row * BF_LD + a_kc) = ahi;
-    *(bf16x8*)(Al + a_row * BF_LD + a_kc) = alo;
-    *(bf16x8*)(Bh + b_col * BF_LD + b_kc) = bhi;
-    *(bf16x8*)(Bl + b_col * BF_LD + b_kc) = blo;
+    *(bf16x8*)(Ah + a_row * BF_LD + a_kc) = ahi0;
+    *(bf16x8*)(Ah + a_row * BF_LD + a_kc + 8) = ahi1;
+    *(bf16x8*)(Al + a_row * BF_LD + a_kc) = alo0;
+    *(bf16x8*)(Al + a_row * BF_LD + a_kc + 8) = alo1;
+    *(bf16x8*)(Bh + b_col * BF_LD + b_kc) = bhi0;
+    *(bf16x8*)(Bh + b_col * BF_LD + b_kc + 8) = bhi1;
+    *(bf16x8*)(Bl + b_col * BF_LD + b_kc) = blo0;
+    *(bf16x8*)(Bl + b_col * BF_LD + b_kc + 8) = blo1;
     __syncthreads();
+    if (k0 + BF_BK < K) load(k0 + BF_BK);
 #pragma unroll
     for (int s = 0; s < BF_BK / 16; ++s) {
       const int ko = 16 * s + 8 * h;
@@ -240,7 +304,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
   if (R) {
     __shared__ double smem[4];
     double bs = block_sum<256>(e, smem);
-    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+    if (t == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
   }
 }
 
@@ -252,26 +316,37 @@ constexpr int F64_BK = 16;
 constexpr int F64_LD = 18;  // padded row stride (doubles)
 
 __global__ void __launch_bounds__(256) k_gemm_f64(GemmBatch bt) {
-  const GemmDesc& gd = bt.g[blockIdx.z];
+  const TileIdx ti = map_tile(bt);
+  if (!ti.ok) return;  // whole workgroup: before any barrier
+  const GemmDesc& gd = desc_of(bt, ti.g);
   const double* A = (const double*)gd.A;
   const double* B = (const double*)gd.B;
   double* C = (double*)gd.C;
   const int N = bt.N, K = bt.K;
-  const int m0 = blockIdx.y * BT, n0 = blockIdx.x * BT;
+  const int m0 = ti.ty * BT, n0 = ti.tx * BT;
   __shared__ __attribute__((aligned(16))) double As[BT * F64_LD];
   __shared__ __attribute__((aligned(16))) double Bs[BT * F64_LD];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int wm = wid >> 1, wn = wid & 1;
   const int q = lane >> 4, l16 = lane & 15;
   f64x4 acc[2][2] = {};
-  const int a_row = t >> 2, a_kc = (t & 3) * 4;  // 4 consecutive k of one row
+  // 4 consecutive k of one row (A) / one column (B); the next K tile is
+  // loaded into registers while the current one is computed
+  const int a_row = t >> 2, a_kc = (t & 3) * 4;
   const int b_col = t & 63, b_kc = (t >> 6) * 4;
-  for (int k0 = 0; k0 < K; k0 += F64_BK) {
-    const double* ap = A + (size_t)(m0 + a_row) * K + k0 + a_kc;
-    double av[4] = {ap[0], ap[1], ap[2], ap[3]};
-    double bv[4];
+  double av[4], bv[4];
+  auto load = [&](int k0) {
+    const double2* ap = (const double2*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
+    const double2 p0 = ap[0], p1 = ap[1];
+    av[0] = p0.x;
+    av[1] = p0.y;
+    av[2] = p1.x;
+    av[3] = p1.y;
 #pragma unroll
     for (int j = 0; j < 4; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += F64_BK) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -281,6 +356,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(GemmBatch bt) {
       Bs[b_col * F64_LD + (kb & 3) * 4 + (kb >> 2)] = bv[j];
     }
     __syncthreads();
+    if (k0 + F64_BK < K) load(k0 + F64_BK);
 #pragma unroll
     for (int s = 0; s < F64_BK / 4; ++s) {
       double a0 = As[(wm * 32 + l16) * F64_LD + q * 4 + s];
@@ -313,7 +389,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(GemmBatch bt) {
   if (R) {
     __shared__ double smem[4];
     double bs = block_sum<256>(e, smem);
-    if (t == 0) gd.partials[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+    if (t == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
   }
 }
 
@@ -358,8 +434,12 @@ static bool mfma_shape(int M, int N) { return M % BT == 0 && N % BT == 0; }
 static int launch_gemm(int precision, GemmBatch& bt, int count, hipStream_t s, int* nblocks) {
   dim3 block(256);
   if (mfma_shape(bt.M, bt.N) && bt.K % BT == 0) {
-    dim3 grid(bt.N / BT, bt.M / BT, count);
-    if (nblocks) *nblocks = (int)(grid.x * grid.y);
+    bt.TY = bt.M / BT;
+    bt.TX = bt.N / BT;
+    bt.count = count;
+    const int64_t tiles = (int64_t)bt.TY * bt.TX * count;
+    dim3 grid((unsigned)(kXcds * ceil_div(tiles, kXcds)));
+    if (nblocks) *nblocks = bt.TY * bt.TX;
     if (precision == DP_SPECTRAL_FP64)
       hipLaunchKernelGGL(k_gemm_f64, grid, block, 0, s, bt);
     else if (precision == DP_SPECTRAL_FP32)
@@ -541,6 +621,7 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
   DP_ARG(field_y, "dp_poisson_solve: null field_y");
   DP_ARG(!compute_potential || potential, "dp_poisson_solve: compute_potential needs potential");
   hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_POISSON, s);
   const size_t slab = ws_slab(M, N, precision);
   char* w = (char*)workspace;
   void* rho = w;               // map/(bx*by), kept for the energy

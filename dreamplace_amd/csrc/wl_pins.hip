@@ -38,7 +38,8 @@ __device__ __forceinline__ float big_value<float>() { return 3.402823466e+38f; }
 template <>
 __device__ __forceinline__ double big_value<double>() { return 1.7976931348623157e+308; }
 
-constexpr int kPackedGrid = 12288;  // cap of workgroups (one wavefront each) of the packed kernel
+constexpr int kWaves = 4;           // independent wavefronts per workgroup of the packed kernel
+constexpr int kPackedGrid = 8192;   // cap of workgroups of the packed kernel
 constexpr int kBigGrid = 1024;      // cap of workgroups of the big-net kernel (4 wavefronts each)
 constexpr int kMaskBit = (int)0x80000000u;
 
@@ -122,20 +123,28 @@ __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
   ((typename Vec2<T>::type*)gnode)[q] = gv;
 }
 
-// One wavefront per workgroup, one group per iteration of the grid-stride loop.
+// Four independent wavefronts per workgroup, each with its own LDS slice and
+// its own grid-stride loop over groups; they exchange data only within the
+// wavefront (wave_sync), so no wavefront waits for another until the final
+// workgroup sum.
 template <typename T>
-__global__ void __launch_bounds__(64) k_wa_pins(
+__global__ void __launch_bounds__(256) k_wa_pins(
     const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
     const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial,
     T* gnode, double* partials) {
-  __shared__ T s_x[64], s_y[64];
-  __shared__ T s_e[4][64];
-  __shared__ T s_c[8][64];
-  const int lane = threadIdx.x;
+  __shared__ T sh_x[kWaves][64], sh_y[kWaves][64];
+  __shared__ T sh_e[kWaves][4][64];
+  __shared__ T sh_c[kWaves][8][64];
+  __shared__ double s_red[kWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  T* s_x = sh_x[wv];
+  T* s_y = sh_y[wv];
+  T(*s_e)[64] = sh_e[wv];
+  T(*s_c)[64] = sh_c[wv];
   const T ig = *inv_gamma_p;
   double contrib = 0;
-  for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+  for (int g = blockIdx.x * kWaves + wv; g < ngroups; g += gridDim.x * kWaves) {
     const int pb = group_start[g], pe = group_start[g + 1];
     const int j = pb + lane;
     const bool on = j < pe;
@@ -156,7 +165,7 @@ __global__ void __launch_bounds__(64) k_wa_pins(
     }
     s_x[lane] = x;
     s_y[lane] = y;
-    __syncthreads();
+    wave_sync();
     const bool is_head = on && lane == head;
     if (is_head && live) {
       T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
@@ -172,7 +181,7 @@ __global__ void __launch_bounds__(64) k_wa_pins(
       s_c[2][lane] = ymax;
       s_c[3][lane] = ymin;
     }
-    __syncthreads();
+    wave_sync();
     T ex = 0, enx = 0, ey = 0, eny = 0;
     if (on && live) {
       ex = exp((x - s_c[0][head]) * ig);
@@ -184,7 +193,7 @@ __global__ void __launch_bounds__(64) k_wa_pins(
       s_e[2][lane] = ey;
       s_e[3][lane] = eny;
     }
-    __syncthreads();
+    wave_sync();
     if (is_head && live) {
       T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
       for (int k = head; k < head + deg; ++k) {
@@ -212,7 +221,7 @@ __global__ void __launch_bounds__(64) k_wa_pins(
       s_c[6][lane] = (T)((1.0 - bny * yns) / fns);
       s_c[7][lane] = bny;
     }
-    __syncthreads();
+    wave_sync();
     if (on) {
       T gx = 0, gy = 0;
       if (live && !pin_masked) {
@@ -226,10 +235,10 @@ __global__ void __launch_bounds__(64) k_wa_pins(
       }
       store_grad(gnode, dst, gx, gy);
     }
-    __syncthreads();  // LDS reuse by the next group
+    wave_sync();  // LDS reuse by the next group
   }
-  contrib = wave_sum(contrib);
-  if (lane == 0) partials[blockIdx.x] = contrib;
+  const double bs = block_sum<256>(contrib, s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
 // Nets of more than 64 pins: one wavefront per net; big[q] = {net, first
@@ -491,7 +500,7 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
   DP_PROBE_OP(DP_OP_WA_FORWARD, s);
   double* partials = (double*)workspace;
   char* pin_xy = (char*)workspace + reduce_slot_bytes();
-  const int g1 = grid_cap(num_groups, 1, kPackedGrid);
+  const int g1 = grid_cap(num_groups, kWaves, kPackedGrid);
   const int g2 = grid_cap(num_big, 4, kBigGrid);
   return DP_DISPATCH(dtype, "dp_wa_forward_pins", [&] {
     typedef typename Vec2<T>::type V;
@@ -510,7 +519,7 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
       DP_LAUNCHED("k_pin_push");
     }
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
-              hipLaunchKernelGGL(k_wa_pins<T>, dim3(g1), dim3(64), 0, s, (const V*)pin_xy,
+              hipLaunchKernelGGL(k_wa_pins<T>, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
                                  (const int4*)pin_meta, (const int*)group_start, (int)num_groups,
                                  (const T*)net_weights, net_mask, (const T*)inv_gamma,
                                  (T*)partial_wl, (T*)grad_node_order, partials));

@@ -40,7 +40,7 @@ class ElectricDensityMapFunction:
                 num_movable_impacted_bins_y, num_filler_impacted_bins_x,
                 num_filler_impacted_bins_y, deterministic_flag, sorted_node_map,
                 convention="cpu", algorithm="global", large_capacity=None, workspace=None,
-                fixed_map_out=None):
+                fixed_map_out=None, uniform_filler=None):
         name = "ElectricDensityMapFunction"
         N.require_gpu(pos, node_size_x_clamped, initial_density_map, name=name)
         pos = pos.view(pos.numel())
@@ -66,8 +66,8 @@ class ElectricDensityMapFunction:
                N.ptr(offset_y), N.ptr(ratio), N.ptr(init), _f(xl), _f(yl), _f(xh), _f(yh),
                _f(bin_size_x), _f(bin_size_y), int(num_movable_nodes), int(num_filler_nodes), nbx,
                nby, N.DP_CONVERT_CPU if convention == "cpu" else N.DP_CONVERT_GPU, alg,
-               int(large_capacity), N.ptr(workspace), N.ptr(out), N.ptr(fixed_map_out),
-               N.stream_ptr(pos.device))
+               int(large_capacity), N.host_doubles(uniform_filler), N.ptr(workspace), N.ptr(out),
+               N.ptr(fixed_map_out), N.stream_ptr(pos.device))
         if padding > 0:
             out.masked_fill_(padding_mask.bool(), _f(target_density) * _f(bin_size_x) * _f(bin_size_y))
         return out
@@ -101,7 +101,14 @@ class ElectricOverflow(nn.Module):
         self.deterministic_flag = deterministic_flag
         self.convention = convention
         self.algorithm = algorithm
+        # pass the fillers' shared size to the kernels when reset() finds one
+        # (set False to make the kernels read every filler's arrays)
+        self.use_uniform_filler = True
+        # the tiled gather reuses the cell binning of the last full density
+        # call; this counts those calls so a stale binning can be detected
+        self.binning_generation = 0
         self._ws = None
+        self._ws_overflow = None
         self._red_ws = None
         self.reset()
 
@@ -155,16 +162,41 @@ class ElectricOverflow(nn.Module):
         big = (self.node_size_x_clamped > 1.9 * self.bin_size_x) | \
               (self.node_size_y_clamped > 1.9 * self.bin_size_y)
         self.large_capacity = int(big.sum().item()) + 64
+        # PlaceDB gives every filler one size (dreamplace/PlaceDB.py:917-936):
+        # then the kernels take the fillers' clamped size, offset and ratio as
+        # five scalars instead of reading them per cell
+        self.uniform_filler = None
+        if nf > 0:
+            vals = []
+            for t in (self.node_size_x_clamped, self.node_size_y_clamped, self.offset_x,
+                      self.offset_y, self.ratio):
+                f = t[-nf:]
+                if not bool((f == f[0]).all().item()):
+                    vals = None
+                    break
+                vals.append(float(f[0].item()))
+            self.uniform_filler = vals
         self.initial_density_map = None
 
     @property
     def algorithm_code(self):
         return N.DP_SCATTER_TILED if self.algorithm == "tiled" else N.DP_SCATTER_GLOBAL
 
-    def workspace(self, device):
+    def uniform_filler_arg(self):
+        """The uniform_filler argument of dp_density_map / dp_electric_force."""
+        return N.host_doubles(self.uniform_filler if self.use_uniform_filler else None)
+
+    def workspace(self, device, overflow=False):
+        """The density workspace (its tiled region holds the binning the field
+        gather reuses); `overflow` gives a second one for the movables-only
+        overflow map, so that map never replaces that binning."""
         nbytes = N.query("dp_density_map_workspace_bytes", self.num_bins_x, self.num_bins_y,
                          self.node_size_x.numel(), self.large_capacity, self.algorithm_code,
                          N.dtype_code(self.node_size_x))
+        if overflow:
+            if self._ws_overflow is None or self._ws_overflow.numel() < nbytes:
+                self._ws_overflow = torch.empty(nbytes, dtype=torch.uint8, device=device)
+            return self._ws_overflow
         if self._ws is None or self._ws.numel() < nbytes:
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
         return self._ws
@@ -192,6 +224,9 @@ class ElectricOverflow(nn.Module):
 
     def density_map(self, pos, num_filler_nodes=None, fixed_map_out=None):
         nf = self.num_filler_nodes if num_filler_nodes is None else num_filler_nodes
+        full = nf == self.num_filler_nodes
+        if full:
+            self.binning_generation += 1
         return ElectricDensityMapFunction.forward(
             pos, self.node_size_x_clamped, self.node_size_y_clamped, self.offset_x, self.offset_y,
             self.ratio, self.bin_center_x, self.bin_center_y, self.initial_density_map,
@@ -201,8 +236,9 @@ class ElectricOverflow(nn.Module):
             self.num_movable_impacted_bins_y, self.num_filler_impacted_bins_x,
             self.num_filler_impacted_bins_y, self.deterministic_flag, self.sorted_node_map,
             convention=self.convention, algorithm=self.algorithm,
-            large_capacity=self.large_capacity, workspace=self.workspace(pos.device),
-            fixed_map_out=fixed_map_out)
+            large_capacity=self.large_capacity,
+            workspace=self.workspace(pos.device, overflow=not full), fixed_map_out=fixed_map_out,
+            uniform_filler=self.uniform_filler if self.use_uniform_filler else None)
 
     def overflow_from_map(self, density_map):
         """(sum max(D - t*A_bin, 0), max(D)/A_bin) with FP64 accumulation, as

@@ -38,6 +38,7 @@ class ElectricPotentialFunction(Function):
         ctx.op = op
         ctx.field_map_x = fx
         ctx.field_map_y = fy
+        ctx.binning = op.binning_generation
         ctx.save_for_backward(pos)
         op.last_density_map = density_map
         return energy
@@ -46,6 +47,10 @@ class ElectricPotentialFunction(Function):
     def backward(ctx, grad_pos):
         (pos,) = ctx.saved_tensors
         op = ctx.op
+        if op.algorithm == "tiled" and op.binning_generation != ctx.binning:
+            # another density call has re-binned the cells since this forward:
+            # bin these positions again for the gather
+            op.density_map(pos)
         grad = op.electric_force(pos, ctx.field_map_x, ctx.field_map_y, grad_pos)
         return grad, None
 
@@ -118,7 +123,7 @@ class ElectricPotential(ElectricOverflow):
                N.ptr(self.ratio), float(self.xl), float(self.yl), float(self.bin_size_x),
                float(self.bin_size_y), int(self.num_movable_nodes), int(self.num_filler_nodes),
                self.num_bins_x, self.num_bins_y, self.algorithm_code, self.large_capacity,
-               layout, N.ptr(self.workspace(pos.device)), N.ptr(out),
+               layout, self.uniform_filler_arg(), N.ptr(self.workspace(pos.device)), N.ptr(out),
                N.stream_ptr(pos.device))
         return out
 

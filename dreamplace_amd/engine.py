@@ -215,7 +215,8 @@ class PlacementObjective:
                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
                ep.num_bins_x, ep.num_bins_y,
                N.DP_CONVERT_CPU if ep.convention == "cpu" else N.DP_CONVERT_GPU,
-               ep.algorithm_code, ep.large_capacity, N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map), None,
+               ep.algorithm_code, ep.large_capacity, ep.uniform_filler_arg(),
+               N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map), None,
                N.stream_ptr(pos.device))
         dmap = self.density_map
         if dmap.dtype != self.plan.dtype:
@@ -237,7 +238,7 @@ class PlacementObjective:
                N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
                ep.num_bins_x, ep.num_bins_y, ep.algorithm_code, ep.large_capacity,
-               self.grad_layout, N.ptr(ep.workspace(pos.device)),
+               self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
                N.ptr(self._density_grad), N.stream_ptr(pos.device))
 
     def evaluate(self, pos):

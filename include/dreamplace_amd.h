@@ -107,14 +107,19 @@ size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, in
  * `sorted_node_map` is accepted for signature parity and not needed: the
  * integer sum is order-free.  large_capacity bounds the queue of cells
  * spanning more than 3x3 bins (any count is correct; a larger queue only
- * moves more of them off the per-thread path). */
+ * moves more of them off the per-thread path).  uniform_filler (nullable,
+ * host memory): {clamped size x, clamped size y, offset x, offset y, ratio}
+ * shared by every filler node, which PlaceDB's fillers are
+ * (dreamplace/PlaceDB.py:917-936); the filler entries of the five arrays are
+ * then not read.  The result is the same either way. */
 int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
                    const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
                    const void* ratio, const void* initial_density_map, double xl, double yl,
                    double xh, double yh, double bin_size_x, double bin_size_y,
                    int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
                    int num_bins_y, int convention, int algorithm, int64_t large_capacity,
-                   void* workspace, void* density_map_out, int64_t* fixed_map_out, void* stream);
+                   const double* uniform_filler, void* workspace, void* density_map_out,
+                   int64_t* fixed_map_out, void* stream);
 
 /* Replaces electric_potential_cuda.fixed_density_map
  * (electric_density_map_cuda.cpp:139-182, electric_density_map.cpp:155-213)
@@ -139,15 +144,15 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
  * dp_density_map call (DP_SCATTER_TILED, same workspace) made: `pos` must be
  * the positions of that call.  grad_layout: DP_LAYOUT_SPLIT (the reference's
  * layout) or DP_LAYOUT_INTERLEAVED (one word per node, as
- * dp_node_grad_from_pins reads it). */
+ * dp_node_grad_from_pins reads it).  uniform_filler as for dp_density_map. */
 int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
                       const void* field_map_y, const void* pos, int64_t num_nodes,
                       const void* node_size_x_clamped, const void* node_size_y_clamped,
                       const void* offset_x, const void* offset_y, const void* ratio, double xl,
                       double yl, double bin_size_x, double bin_size_y, int64_t num_movable_nodes,
                       int64_t num_filler_nodes, int num_bins_x, int num_bins_y, int algorithm,
-                      int64_t large_capacity, int grad_layout, void* workspace, void* grad_out,
-                      void* stream);
+                      int64_t large_capacity, int grad_layout, const double* uniform_filler,
+                      void* workspace, void* grad_out, void* stream);
 
 /* ElectricOverflow metric (electric_overflow.py:260-278): out2[0] =
  * sum(max(map - target_area, 0)), out2[1] = max(map); FP64 accumulation,

@@ -59,7 +59,7 @@ def test_argument_errors_return_codes(lib):
     fake = ctypes.c_void_p(0x1000)
     # null input arrays
     rc = lib.dp_density_map(0, None, 10, None, None, None, None, None, None, 0., 0., 1., 1., .1, .1,
-                            5, 0, 10, 10, 0, 0, 0, fake, fake, None, None)
+                            5, 0, 10, 10, 0, 0, 0, None, fake, fake, None, None)
     assert rc == -1 and b"null" in lib.dp_last_error()
     # unsupported dtype is reported before any launch
     rc = lib.dp_move_boundary(7, fake, 4, fake, fake, 0., 0., 1., 1., 2, 0, None)

@@ -166,3 +166,51 @@ def test_electric_force_interleaved_layout(gpu, case, dtype, algorithm):
     split = op.electric_force(t["pos"], fx, fy, g)
     inter = op.electric_force(t["pos"], fx, fy, g, grad_layout="interleaved")
     assert torch.equal(inter.view(-1, 2).t().reshape(-1), split)
+
+
+@pytest.mark.parametrize("algorithm", ["global", "tiled"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_uniform_filler_path_identical(gpu, dtype, algorithm):
+    """Taking the fillers' shared size as scalars gives the identical int64 map
+    and force as reading every filler's arrays."""
+    print("case uniform filler %s %s" % (np.dtype(dtype).name, algorithm), flush=True)
+    d = off_grid(design(1, dtype=dtype, scale=0.002))
+    t = to_dev(d, gpu)
+    op = _op(d, t, t["pos"].dtype, algorithm=algorithm)
+    assert op.uniform_filler is not None
+    op.compute_initial_density_map(t["pos"])
+    rng = np.random.default_rng(3)
+    fx = torch.from_numpy(rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)).to(gpu)
+    fy = torch.from_numpy(rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)).to(gpu)
+    g = torch.tensor(0.5, dtype=t["pos"].dtype, device=gpu)
+    out = []
+    for uni in (True, False):
+        op.use_uniform_filler = uni
+        raw = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
+        m = op.density_map(t["pos"], fixed_map_out=raw)
+        out.append((raw, m, op.electric_force(t["pos"], fx, fy, g)))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
+def test_backward_after_rebinning(gpu):
+    """The tiled gather reuses the forward's cell binning; a density call at
+    other positions between forward and backward must not change the
+    gradient, nor must an overflow-mode call."""
+    print("case backward after rebinning", flush=True)
+    d = design(0)
+    t = to_dev(d, gpu)
+    op = _op(d, t, torch.float64, algorithm="tiled")
+    pos = t["pos"].clone().requires_grad_(True)
+    op(pos).backward()
+    ref = pos.grad.clone()
+    other = t["pos"] + 0.37
+    for interleave in ("density", "overflow"):
+        pos.grad = None
+        e = op(pos)
+        if interleave == "density":
+            op(other)
+        else:
+            op(other, mode="overflow")
+        e.backward()
+        assert torch.equal(pos.grad, ref), interleave

@@ -92,6 +92,27 @@ def test_overflow_module_reset_matches_oracle(oracle):
         op.compute_initial_density_map(T(d.pos))
 
 
+def test_uniform_filler_detection():
+    """reset() hands the fillers' shared clamped size, offset and ratio to the
+    kernels only when every filler has them."""
+    from dreamplace_amd.electric_overflow import ElectricOverflow
+    d = synth.make_design(0)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+
+    def build(sx):
+        return ElectricOverflow(T(sx), T(d.node_size_y), T(d.bin_center_x), T(d.bin_center_y),
+                                d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
+                                d.bin_size_y, d.num_movable_nodes, d.num_terminals,
+                                d.num_filler_nodes, 0, True, T(d.sorted_node_map))
+    op = build(d.node_size_x)
+    nf = d.num_filler_nodes
+    assert op.uniform_filler == [float(op.node_size_x_clamped[-1]), float(op.node_size_y_clamped[-1]),
+                                 float(op.offset_x[-1]), float(op.offset_y[-1]), float(op.ratio[-1])]
+    sx = d.node_size_x.copy()
+    sx[-nf // 2] *= 1.5
+    assert build(sx).uniform_filler is None
+
+
 def test_ops_refuse_cpu_tensors():
     from dreamplace_amd.hpwl import hpwl_fp64
     from dreamplace_amd.pin_pos import PinPos
