@@ -42,6 +42,7 @@ constexpr int kMaxTiles = 2048;           // LDS histogram limit
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
 constexpr int kFillThreads = 1024;        // threads of a fill workgroup
 constexpr int kChunk = 8192;              // cells per scatter/gather work item
+constexpr int kGatherSplit = 2;           // the gather does each work item in this many parts
 
 // Cells per LDS-staged fill sub-batch: 28 (FP32) or 48 (FP64) bytes of LDS
 // per cell, ~115 KB (FP32) / ~100 KB (FP64) in all with the tile arrays.
@@ -142,15 +143,27 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * chunk;
   const int64_t e = b + chunk < total ? b + chunk : total;
-  for (int64_t t = b + threadIdx.x; t < e; t += blockDim.x) {
-    const int64_t i = node_of(t, n_mov, fill_begin);
-    const CellBins<T> c = cell_bins(a, i);
-    const int k = tile_key(c, nty);
-    if (k < 0) {
-      const int q = atomicAdd(ntail, 1);
-      rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i);
-    } else {
-      atomicAdd(&hist[k], 1);
+  constexpr int U = 4;  // cells per thread per pass, loaded before any is used
+  for (int64_t t0 = b + threadIdx.x; t0 < e; t0 += U * blockDim.x) {
+    RawCell<T> raw[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t t = t0 + u * blockDim.x;
+      if (t < e) raw[u] = load_cell(a, node_of(t, n_mov, fill_begin));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t t = t0 + u * blockDim.x;
+      if (t >= e) continue;
+      const int64_t i = node_of(t, n_mov, fill_begin);
+      const CellBins<T> c = bins_of(a, raw[u]);
+      const int k = tile_key(c, nty);
+      if (k < 0) {
+        const int q = atomicAdd(ntail, 1);
+        rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i);
+      } else {
+        atomicAdd(&hist[k], 1);
+      }
     }
   }
   __syncthreads();
@@ -421,10 +434,16 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriArgs<T> a, int nty, cons
                                                      int64_t num_nodes, int interleaved, T* out) {
   typedef typename Vec2<T>::type V;
   __shared__ V sf[kTileH * kTileH];
-  const int W = *nwork;
+  const int W = *nwork * kGatherSplit;
   const T g = grad ? *grad : T(1);
   for (int w = blockIdx.x; w < W; w += gridDim.x) {
-    const int4 it = work[w];
+    // part w % kGatherSplit of work item w / kGatherSplit
+    int4 it = work[w / kGatherSplit];
+    const int part = w % kGatherSplit, len = it.z - it.y;
+    const int lo = it.y + (int)((int64_t)len * part / kGatherSplit);
+    const int hi = it.y + (int)((int64_t)len * (part + 1) / kGatherSplit);
+    it.y = lo;
+    it.z = hi;
     const int tx0 = (it.x / nty) * kTile, ty0 = (it.x % nty) * kTile;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) {
       const int gx = tx0 + k / kTileH, gy = ty0 + k % kTileH;

@@ -115,6 +115,12 @@ __global__ void __launch_bounds__(256) k_pin_push(const T* __restrict__ x, const
   pin_xy[dst[i]] = v;
 }
 
+// A pin's four exponentials as one aligned LDS word.
+template <typename T>
+struct alignas(4 * sizeof(T)) Exp4 {
+  T a0, a1, a2, a3;
+};
+
 template <typename T>
 __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
   typename Vec2<T>::type gv;
@@ -133,14 +139,16 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
     const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial,
     T* gnode, double* partials) {
-  __shared__ T sh_x[kWaves][64], sh_y[kWaves][64];
-  __shared__ T sh_e[kWaves][4][64];
+  typedef typename Vec2<T>::type V;
+  // per-lane words read by the net's first lane: one LDS read per pin for
+  // (x, y) and one for its four exponentials
+  __shared__ V sh_xy[kWaves][64];
+  __shared__ Exp4<T> sh_e[kWaves][64];
   __shared__ T sh_c[kWaves][8][64];
   __shared__ double s_red[kWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  T* s_x = sh_x[wv];
-  T* s_y = sh_y[wv];
-  T(*s_e)[64] = sh_e[wv];
+  V* s_xy = sh_xy[wv];
+  Exp4<T>* s_e = sh_e[wv];
   T(*s_c)[64] = sh_c[wv];
   const T ig = *inv_gamma_p;
   double contrib = 0;
@@ -163,14 +171,19 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       x = v.x;
       y = v.y;
     }
-    s_x[lane] = x;
-    s_y[lane] = y;
+    {
+      V v;
+      v.x = x;
+      v.y = y;
+      s_xy[lane] = v;
+    }
     wave_sync();
     const bool is_head = on && lane == head;
     if (is_head && live) {
       T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
       for (int k = head; k < head + deg; ++k) {
-        const T xx = s_x[k], yy = s_y[k];
+        const V v = s_xy[k];
+        const T xx = v.x, yy = v.y;
         xmax = xx > xmax ? xx : xmax;
         xmin = xx < xmin ? xx : xmin;
         ymax = yy > ymax ? yy : ymax;
@@ -188,17 +201,21 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       enx = exp((s_c[1][head] - x) * ig);
       ey = exp((y - s_c[2][head]) * ig);
       eny = exp((s_c[3][head] - y) * ig);
-      s_e[0][lane] = ex;
-      s_e[1][lane] = enx;
-      s_e[2][lane] = ey;
-      s_e[3][lane] = eny;
+      Exp4<T> q;
+      q.a0 = ex;
+      q.a1 = enx;
+      q.a2 = ey;
+      q.a3 = eny;
+      s_e[lane] = q;
     }
     wave_sync();
     if (is_head && live) {
       T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
       for (int k = head; k < head + deg; ++k) {
-        const T xx = s_x[k], yy = s_y[k];
-        const T a0 = s_e[0][k], a1 = s_e[1][k], a2 = s_e[2][k], a3 = s_e[3][k];
+        const V v = s_xy[k];
+        const Exp4<T> q = s_e[k];
+        const T xx = v.x, yy = v.y;
+        const T a0 = q.a0, a1 = q.a1, a2 = q.a2, a3 = q.a3;
         xs += xx * a0;
         xns += xx * a1;
         es += a0;
