@@ -354,6 +354,42 @@ int dp_pin_weight_sum(int dtype, const void* net_weights, const int32_t* flat_no
                       const int32_t* nodepin_start, const int32_t* pin2net_map,
                       int64_t num_physical_nodes, int64_t num_nodes, void* out, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Optimizer (dreamplace/NesterovAcceleratedGradientOptimizer.py:65-240) and
+ * preconditioner (dreamplace/PlaceObj.py:69-132).  Scalars live on the device
+ * (one element of `dtype`) so that a step needs no host round trip.
+ * ------------------------------------------------------------------------- */
+
+size_t dp_optimizer_workspace_bytes(void);
+
+/* One Nesterov extrapolation (step_nobb/step_bb inner update, :120-124,211-212):
+ * a_kp1 = (1 + sqrt(4 a_k^2 + 1)) / 2, coef = (a_k - 1) / a_kp1,
+ * u_kp1 = v_k - alpha_k * g_k, v_kp1 = u_kp1 + coef * (u_kp1 - u_k), in
+ * `dtype` and in the reference's operation order.  a_kp1 (nullable) must not
+ * alias a_k. */
+int dp_nesterov_extrapolate(int dtype, int64_t n, const void* v_k, const void* g_k,
+                            const void* u_k, const void* alpha_k, const void* a_k, void* u_kp1,
+                            void* v_kp1, void* a_kp1, void* stream);
+
+/* out3 (device double[3]) = {sum((a1-a0)^2), sum((b1-b0)^2), sum((a1-a0)(b1-b0))}:
+ * differences in `dtype`, squares and sums in FP64, deterministic.  With
+ * a = positions and b = gradients these give the Lipschitz step
+ * sqrt(out3[0]/out3[1]) (:131) and the Barzilai-Borwein steps (:200-205). */
+int dp_step_norms(int dtype, int64_t n, const void* a1, const void* a0, const void* b1,
+                  const void* b0, void* workspace, double* out3, void* stream);
+
+/* *out (device double) = sum |g_i| in FP64 (the norms of PlaceObj.
+ * initialize_density_weight, dreamplace/PlaceObj.py:777-815). */
+int dp_l1_norm(int dtype, int64_t n, const void* g, void* workspace, double* out, void* stream);
+
+/* Replaces PreconditionOp.__call__ without fence regions (PlaceObj.py:69-82,
+ * 100-102): precond = max(pin_weight_sum + (alpha * density_weight) *
+ * node_areas, 1); grad[0:N] and grad[N:2N] are divided by it in place.
+ * density_weight is a device scalar of `dtype`. */
+int dp_precondition(int dtype, void* grad, int64_t num_nodes, const void* pin_weight_sum,
+                    const void* node_areas, const void* density_weight, double alpha,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -514,3 +514,61 @@ def test_planted_sign_error_is_caught(oracle, monkeypatch):
     monkeypatch.setattr(oracle, "idxst_idct", lambda x: -orig(x))
     with pytest.raises(AssertionError):
         test_poisson_single_mode(oracle, 3, 2)
+
+
+# ---------------------------------------------------------------------------
+# the placement driver (oracle/placer.py)
+# ---------------------------------------------------------------------------
+def _quadratic(seed=0, n=64):
+    a = np.random.default_rng(seed).uniform(0.5, 20.0, n)
+
+    def f(x):
+        return 0.5 * float(np.sum(a * x * x)), a * x
+    return a, f
+
+
+@pytest.mark.parametrize("use_bb", [False, True], ids=["nobb", "bb"])
+def test_nesterov_converges_on_quadratic(use_bb):
+    """f = 1/2 sum a_i x_i^2 (minimiser 0, Lipschitz constant L = max a): the
+    restated optimizer meets the accelerated-gradient bound
+    f(x_k) <= 2 L |x_0|^2 / (k+1)^2 and keeps descending."""
+    from oracle import placer as OP
+    a, f = _quadratic()
+    x = np.random.default_rng(1).standard_normal(len(a))
+    f0, r0 = f(x)[0], float(np.dot(x, x))
+    opt = OP.Nesterov(x, 0.01, f, lambda v: None, use_bb=use_bb)
+    for k in range(1, 61):
+        opt.step()
+        assert f(x)[0] <= 2 * a.max() * r0 / (k + 1) ** 2
+    assert f(x)[0] < 1e-6 * f0
+
+
+def test_nesterov_first_step_closed_form():
+    """First step_nobb: a_1 = (1+sqrt 5)/2, coef = 0, alpha from the initial
+    secant; on a quadratic with all a_i = c the secant step is exactly 1/c,
+    so v_1 = u_1 = x0 - (1/c) * c * x0 = 0 and the line search accepts it."""
+    from oracle import placer as OP
+    c = 4.0
+    x = np.linspace(-1, 1, 16)
+
+    def f(v):
+        return 0.5 * c * float(np.dot(v, v)), c * v
+    opt = OP.Nesterov(x, 0.05, f, lambda v: None)
+    opt.step()
+    assert np.allclose(x, 0.0, atol=1e-15)
+    assert opt.alpha_k == pytest.approx(1.0 / c)
+    assert opt.a_k == pytest.approx((1 + 5 ** 0.5) / 2)
+
+
+def test_placement_loop_decreases_hpwl(oracle):
+    """Six iterations of the restated loop on config 0: HPWL falls from the
+    spread-out start, the density weight starts at the gradient-norm ratio and
+    gamma follows the overflow schedule."""
+    from oracle import placer as OP
+    d = design(0)
+    pos, hist = OP.place(d, 6)
+    assert hist[-1]["hpwl"] < 0.5 * hist[0]["hpwl"]
+    base = OP.DEFAULTS["gamma"] * (d.bin_size_x + d.bin_size_y)
+    assert hist[0]["gamma"] == pytest.approx(10 * base)
+    for h0, h1 in zip(hist, hist[1:]):
+        assert h1["gamma"] == pytest.approx(base * 10 ** ((h0["overflow"] - 0.1) * 20 / 9 - 1))
