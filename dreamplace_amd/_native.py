@@ -77,6 +77,12 @@ _SIGNATURES = {
     "dp_step_norms": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P]),
     "dp_l1_norm": (_I, [_I, _L, _P, _P, _P, _P]),
     "dp_precondition": (_I, [_I, _P, _L, _P, _P, _P, _D, _P]),
+    # host-only Bookshelf reader
+    "dp_bookshelf_read": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "dp_bookshelf_sizes": (_I, [_P, _P]),
+    "dp_bookshelf_export": (_I, [_P] + [_P] * 21),
+    "dp_bookshelf_free": (None, [_P]),
+    "dp_rect_union_area": (_D, [_L, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -123,6 +129,15 @@ def ptr(t):
     if t is None:
         return None
     return ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_np(a):
+    """Host pointer of a C-contiguous numpy array (None for None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("host array must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def host_doubles(values):

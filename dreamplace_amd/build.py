@@ -2,7 +2,8 @@
 
     python -m dreamplace_amd.build [--force] [--jobs N]
 
-Each csrc/*.hip translation unit is compiled to an object under
+Each csrc/*.hip translation unit (and each host-only csrc/*.cpp, with g++) is
+compiled to an object under
 dreamplace_amd/_build/ and the objects are linked into
 dreamplace_amd/libdreamplace_amd.so.  Flags:
 
@@ -23,6 +24,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdreamplace_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall"]
 
 CFLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -43,14 +46,17 @@ def _compile(src, force):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     if not force and not _deps_newer(obj, src):
         return obj, True, ""
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    if src.endswith(".cpp"):  # host-only sources
+        cmd = [CXX] + CXXFLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return obj, r.returncode == 0, (r.stdout + r.stderr)
 
 
 def build(force=False, jobs=8, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     objs, ok, logs = [], True, []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for obj, good, log in ex.map(lambda s: _compile(s, force), srcs):

@@ -390,6 +390,53 @@ int dp_precondition(int dtype, void* grad, int64_t num_nodes, const void* pin_we
                     const void* node_areas, const void* density_weight, double alpha,
                     void* stream);
 
+/* ---- Bookshelf input / output (host only) ------------------------------- */
+/* Node orientations and pin directions as PlaceDB names them. */
+enum {
+  DP_ORIENT_N = 0, DP_ORIENT_S = 1, DP_ORIENT_W = 2, DP_ORIENT_E = 3,
+  DP_ORIENT_FN = 4, DP_ORIENT_FS = 5, DP_ORIENT_FW = 6, DP_ORIENT_FE = 7,
+  DP_ORIENT_UNKNOWN = 8
+};
+enum { DP_PIN_INPUT = 0, DP_PIN_OUTPUT = 1, DP_PIN_INOUT = 2 };
+/* Node status after reading: unplaced, placed, dummy-fixed (movable node
+ * taller than 5 rows), fixed. */
+enum { DP_NODE_UNPLACED = 0, DP_NODE_PLACED = 1, DP_NODE_DUMMY_FIXED = 2, DP_NODE_FIXED = 3 };
+
+/* Replaces PlaceIOFunction.read + PyPlaceDB construction for Bookshelf input
+ * (dreamplace/ops/place_io/place_io.py:17-30; PlaceDB.cpp:904-1290,
+ * 1726-1734, 1854-2090; PyPlaceDB.cpp:161-440): parses the .nodes, .nets,
+ * .wts, .scl and .pl files an .aux file names into a host-side handle, with
+ * the reference's node order (movable, fixed, IO pins), net order (by degree
+ * when sort_nets_by_degree) and pin order (by net).  Errors: DP_ERR_* with
+ * dp_last_error(). */
+int dp_bookshelf_read(const char* aux_path, int sort_nets_by_degree, void** handle);
+/* sizes[9] = {num_nodes, num_terminals (fixed, not IO), num_terminal_NIs,
+ * num_nets, num_pins, num_rows, bytes of the NUL-separated node names, bytes
+ * of the net names, num_movable_pins}. */
+int dp_bookshelf_sizes(void* handle, int64_t* sizes);
+/* Copies the design out (every pointer nullable; host memory):
+ * node_x/y (lower-left), node_size_x/y, node_orient (DP_ORIENT_*),
+ * node_status (DP_NODE_*), node_names; pin_offset_x/y (from the lower-left
+ * corner), pin_direct (DP_PIN_*), pin2node_map, pin2net_map;
+ * flat_net2pin_map + flat_net2pin_start_map[num_nets+1],
+ * flat_node2pin_map + flat_node2pin_start_map[num_nodes+1]; net_weights,
+ * net_ignore (fewer than 2 distinct nodes), net_names; rows[4*num_rows]
+ * (xl, yl, xh, yh); die[6] = {xl, yl, xh, yh, row_height, site_width}. */
+int dp_bookshelf_export(void* handle, double* node_x, double* node_y, double* node_size_x,
+                        double* node_size_y, int32_t* node_orient, int32_t* node_status,
+                        char* node_names, double* pin_offset_x, double* pin_offset_y,
+                        int32_t* pin_direct, int32_t* pin2node_map, int32_t* pin2net_map,
+                        int32_t* flat_net2pin_map, int32_t* flat_net2pin_start_map,
+                        int32_t* flat_node2pin_map, int32_t* flat_node2pin_start_map,
+                        double* net_weights, uint8_t* net_ignore, char* net_names, double* rows,
+                        double* die);
+void dp_bookshelf_free(void* handle);
+/* Area of the union of n rectangles clipped to clip[4] = {xl, yl, xh, yh}
+ * (nullable): the fixed-cell area of PyPlaceDB::computeAreaStatistics
+ * (PyPlaceDB.cpp:741-770), which the space area and filler count use. */
+double dp_rect_union_area(int64_t n, const double* xl, const double* yl, const double* xh,
+                          const double* yh, const double* clip);
+
 #ifdef __cplusplus
 }
 #endif
