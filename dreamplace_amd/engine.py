@@ -168,6 +168,9 @@ class PlacementObjective:
         self.inv_gamma = torch.tensor(1.0 / gamma, dtype=dt, device=dev)
         self.lam = torch.tensor(float(density_weight), dtype=dt, device=dev)
         self.lam_host = float(density_weight)
+        self.density_factor = 1.0
+        self.quad_coeff = None
+        self.lam_eff = torch.empty((), dtype=dt, device=dev)
         self.one = torch.ones((), dtype=dt, device=dev)
         self.wl = z(())
         self.partial_wl = z(E)
@@ -205,6 +208,13 @@ class PlacementObjective:
         self.lam_host = float(w)
         self.lam.fill_(float(w))
 
+    def set_density_penalty(self, factor=1.0, quad_coeff=None):
+        """PlaceObj.obj_fn's penalty forms (dreamplace/PlaceObj.py:293-320):
+        obj = WL + (factor * lambda) * D, or with the quadratic penalty
+        obj = WL + (factor * lambda) * D * (1 + c * D), c = quad_coeff."""
+        self.density_factor = float(factor)
+        self.quad_coeff = None if quad_coeff is None else float(quad_coeff)
+
     # -- stages ---------------------------------------------------------------
     def _density_forward(self, pos):
         ep = self.density
@@ -227,12 +237,30 @@ class PlacementObjective:
                           compute_energy=want_energy,
                           out=(self.field_x, self.field_y, self.potential, self.energy))
 
+    def _energy(self, pos):
+        e = self.energy
+        return e if e.dtype == pos.dtype else e.to(pos.dtype)
+
+    def _density_grad_weight(self, pos):
+        """d obj / d D on the device: factor * lambda, times (1 + c D) +
+        D c under the quadratic penalty (the product rule's two terms, in
+        autograd's order)."""
+        if self.quad_coeff is None and self.density_factor == 1.0:
+            return self.lam
+        g = torch.mul(self.lam, self.density_factor, out=self.lam_eff)
+        if self.quad_coeff is not None:
+            e = self._energy(pos)
+            c = self.quad_coeff
+            self.lam_eff.copy_(g * (1 + c * e) + (g * e) * c)
+        return self.lam_eff
+
     def _density_backward(self, pos):
         ep = self.density
         fx, fy = self.field_x, self.field_y
         if fx.dtype != pos.dtype:
             fx, fy = fx.to(pos.dtype), fy.to(pos.dtype)
-        N.call("dp_electric_force", N.dtype_code(pos), N.ptr(self.lam), N.ptr(fx), N.ptr(fy),
+        w = self._density_grad_weight(pos)
+        N.call("dp_electric_force", N.dtype_code(pos), N.ptr(w), N.ptr(fx), N.ptr(fy),
                N.ptr(pos), pos.numel() // 2, N.ptr(ep.node_size_x_clamped),
                N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x), N.ptr(ep.offset_y),
                N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
@@ -289,6 +317,8 @@ class PlacementObjective:
                    N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    dd.num_physical_nodes, n, N.ptr(self.wl_grad), s)
             torch.add(self.wl_grad, self._density_grad, out=self.grad)
-        energy = self.energy if self.energy.dtype == pos.dtype else self.energy.to(pos.dtype)
-        torch.add(self.wl, energy, alpha=self.lam_host, out=self.obj)
+        energy = self._energy(pos)
+        if self.quad_coeff is not None:
+            energy = energy * (1 + self.quad_coeff * energy)
+        torch.add(self.wl, energy, alpha=self.density_factor * self.lam_host, out=self.obj)
         return self.obj, self.grad

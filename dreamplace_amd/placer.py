@@ -1,18 +1,22 @@
 """Global placement driver: the default-parameter loop of NonLinearPlace
-(dreamplace/NonLinearPlace.py:377-530, 625-800) around the device objective.
+(dreamplace/NonLinearPlace.py:259-271, 377-800) around the device objective.
 
-Per iteration, as the reference with one Lsub and one Llambda step per Lgamma
-step:
+Per iteration, in the reference's order, with one Lsub and one Llambda step
+per Lgamma step:
+  divergence check on the last 50 (HPWL, overflow) pairs, with roll-back to
+  the best position and a stop after this iteration (NonLinearPlace.py:
+  534-570, 633-651);
   move_boundary; (first iteration) initialise the density weight
   (PlaceObj.initialize_density_weight, dreamplace/PlaceObj.py:767-815);
   metrics at the current positions (HPWL over net_mask_all, overflow and
   max density of the movables, FP64 reductions; EvalMetrics.py:116-128);
   one Nesterov step on obj_and_grad_fn = objective + preconditioner
-  (PlaceObj.py:69-132, 394-411); from the second iteration the HPWL-mode
+  (PlaceObj.py:69-132, 394-411); best-overflow bookkeeping (:519-526);
+  plateau handling: quadratic density penalty, doubled density factor and
+  entropy injection (:657-693); from the second iteration the HPWL-mode
   density weight update (PlaceObj.py:830-848); update_gamma (PlaceObj.py:
-  893-909); the Lgamma stop test (NonLinearPlace.py:286-316).
-Routability, timing, fence regions, divergence roll-back and entropy
-injection are not part of this loop.
+  893-909); the Lgamma stop tests (NonLinearPlace.py:286-326).
+Routability, timing, fence regions and macro halos are not part of this loop.
 """
 import math
 
@@ -25,7 +29,8 @@ from .nesterov import NesterovAcceleratedGradientOptimizer
 
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
-                use_bb=False, precondition_alpha=1.0)
+                use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
+                density_factor=1.0)
 
 
 def noise(d, ratio, seed):
@@ -41,6 +46,51 @@ def noise(d, ratio, seed):
     return z
 
 
+def injection_noise(num_nodes, seed, count):
+    """Standard normal draws for the `count`-th entropy injection (FP64)."""
+    return np.random.default_rng([seed, 7919, count]).standard_normal(2 * num_nodes)
+
+
+def check_plateau(x, window=10, threshold=0.001):
+    """NonLinearPlace.check_plateau (NonLinearPlace.py:527-532)."""
+    if len(x) < window:
+        return False
+    x = x[-window:]
+    return (np.max(x) - np.min(x)) / np.mean(x) < threshold
+
+
+def check_divergence(x, best, stop_overflow, window=50, threshold=0.05):
+    """NonLinearPlace.check_divergence (NonLinearPlace.py:534-570) without
+    routability: `x` is the list of (hpwl, overflow) pairs, `best` the
+    best-overflow metrics dict (or None)."""
+    if len(x) < window or best is None:
+        return False
+    x = np.array(x[-window:])
+    overflow_mean = np.mean(x[:, 1])
+    overflow_diff = np.maximum(0, np.sign(x[1:, 1] - x[:-1, 1])).astype(np.float32)
+    overflow_diff = np.sum(overflow_diff) / overflow_diff.shape[0]
+    overflow_range = np.max(x[:, 1]) - np.min(x[:, 1])
+    wl_mean = np.mean(x[:, 0])
+    wl_ratio = (wl_mean - best["hpwl"]) / best["hpwl"]
+    overflow_ratio = (overflow_mean - max(stop_overflow, best["overflow"])) / best["overflow"]
+    if wl_ratio > threshold * 1.2:
+        if overflow_ratio > threshold:
+            return True
+        if overflow_range / overflow_mean < threshold:
+            return True
+        return overflow_diff > 0.6
+    return False
+
+
+def density_weight_factor(cur_hpwl, prev_hpwl, flat_iteration, p):
+    """mu of the HPWL-mode density weight update (PlaceObj.py:830-841)."""
+    delta = cur_hpwl - prev_hpwl
+    if delta < 0:
+        return p["upper_pcof"] * max(0.9999 ** float(flat_iteration), 0.98)
+    return p["upper_pcof"] * min(max(p["upper_pcof"] ** (-delta / p["ref_hpwl"]), p["lower_pcof"]),
+                                 p["upper_pcof"])
+
+
 class GlobalPlacer:
     """`d`: a design dict (dreamplace_amd.synth layout).  Objective options
     (precision, algorithm, schedule) go to PlacementObjective."""
@@ -49,6 +99,7 @@ class GlobalPlacer:
         p = dict(DEFAULTS)
         p.update(params or {})
         self.p = p
+        self.seed = seed
         d = dict(d)
         pos0 = d["pos"].astype(np.float64)
         if p["noise_ratio"] > 0:
@@ -61,6 +112,9 @@ class GlobalPlacer:
         self.gamma = 10 * self.base_gamma
         self.objective = PlacementObjective(dd, self.gamma, density_weight=0.0, **objective_kw)
         self.lam = 0.0
+        self.quad_penalty = bool(p["quad_penalty"])
+        self.density_factor = float(p["density_factor"])
+        self.init_density = None
         n, dev, code = dd.num_nodes, dd.device, N.dtype_code(dd.pos)
         self.code = code
         self.areas = (dd.node_size_x * dd.node_size_y).contiguous()
@@ -80,6 +134,10 @@ class GlobalPlacer:
                                           d["node_size_y"][:d["num_movable_nodes"]]).astype(np.float64)))
         self.pos = dd.pos
         self.history = []
+        self.best = None
+        self.best_pos = None
+        self.rolled_back = False
+        self.perturbations = 0
 
     # -- pieces of the reference loop ----------------------------------------
     def constraint(self, x):
@@ -88,9 +146,25 @@ class GlobalPlacer:
                N.ptr(dd.node_size_y), float(dd.xl), float(dd.yl), float(dd.xh), float(dd.yh),
                dd.num_movable_nodes, dd.num_filler_nodes, N.stream_ptr(x.device))
 
+    def _penalty(self):
+        c = None
+        if self.quad_penalty:
+            # quad_penalty_coeff = coeff / 2 * (1 / init_density), 0 when the
+            # initial density is not positive (PlaceObj.py:306-311)
+            T = np.float32 if self.dd.dtype == torch.float32 else np.float64
+            inv = 1.0 / self.init_density if self.init_density > 0 else self.init_density
+            c = float(T(self.p["density_quad_coeff"] / 2) * T(inv))
+        self.objective.set_density_penalty(self.density_factor, c)
+
     def obj_and_grad(self, x):
         """PlaceObj.obj_and_grad_fn: objective, gradient, preconditioner."""
         f, g = self.objective.evaluate(x)
+        if self.init_density is None:
+            # PlaceObj.obj_fn records the density of its first evaluation
+            self.init_density = float(self.objective._energy(x).item())
+            if self.quad_penalty:
+                self._penalty()
+                f, g = self.objective.evaluate(x)
         N.call("dp_precondition", self.code, N.ptr(g), self.dd.num_nodes, N.ptr(self.pws),
                N.ptr(self.areas), N.ptr(self.objective.lam), float(self.p["precondition_alpha"]),
                N.stream_ptr(x.device))
@@ -99,6 +173,7 @@ class GlobalPlacer:
     def initialize_density_weight(self, x):
         """lambda = density_weight * |grad WL|_1 / |grad D|_1 (no preconditioner)."""
         ob = self.objective
+        ob.set_density_penalty(1.0, None)
         ob.set_density_weight(0.0)
         _, g = ob.evaluate(x)  # the wirelength gradient alone
         gw = g.clone()
@@ -113,6 +188,7 @@ class GlobalPlacer:
         T = np.float32 if self.dd.dtype == torch.float32 else np.float64
         self.lam = float(T(self.p["density_weight"] * wl_norm / d_norm))
         ob.set_density_weight(self.lam)
+        self._penalty()
 
     def metrics(self, x):
         dd, dev = self.dd, x.device
@@ -139,8 +215,29 @@ class GlobalPlacer:
         s = self.norm_out.tolist()
         return math.sqrt(s[0]) / math.sqrt(s[1])
 
+    def entropy_injection(self, x, shrink_factor, noise_intensity):
+        """entropy_injection, mode "random" (NonLinearPlace.py:572-605):
+        shrink every node towards the movables' centroid, add Gaussian noise,
+        then put the fixed nodes back."""
+        dd = self.dd
+        n, m0 = dd.num_nodes, dd.num_movable_nodes
+        m1 = n - dd.num_filler_nodes
+        xc = x[:m0].mean()
+        yc = x[n:n + m0].mean()
+        fx, fy = x[m0:m1].clone(), x[n + m0:n + m1].clone()
+        if shrink_factor != 1:
+            x[:n] = (x[:n] - xc) * shrink_factor + xc
+            x[n:] = (x[n:] - yc) * shrink_factor + yc
+        if noise_intensity > 0.01:
+            z = torch.from_numpy(injection_noise(n, self.seed, self.perturbations)).to(x.device, x.dtype)
+            x.add_(noise_intensity * z)
+        x[m0:m1] = fx
+        x[n + m0:n + m1] = fy
+
     # -- the loop -------------------------------------------------------------
-    def run(self, iterations):
+    def run(self, iterations, log_every=0):
+        """At most `iterations` steps; returns (positions, metrics before each
+        step).  `log_every` > 0 prints every that many iterations' metrics."""
         p, x = self.p, self.pos
         lr = self.estimate_learning_rate(x)
         self.optimizer = NesterovAcceleratedGradientOptimizer(
@@ -148,26 +245,109 @@ class GlobalPlacer:
             use_bb=p["use_bb"])
         hist = self.history
         T = np.float32 if self.dd.dtype == torch.float32 else np.float64
+        overflow_list, divergence_list = [1], []
+        last_perturb, min_perturb_interval = -50, 50
+        stop = False
         for it in range(iterations):
+            if (p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
+                    and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
+                                         threshold=overflow_list[-1])):
+                x.copy_(self.best_pos)
+                stop = True
+                self.rolled_back = True
             self.constraint(x)
             if self.lam == 0.0:
                 self.initialize_density_weight(x)
             cur = self.metrics(x)
             hist.append(cur)
+            if log_every and it % log_every == 0:
+                print("iteration %4d  HPWL %.6E  overflow %.4f  max density %.3f  lambda %.3E  gamma %.3E"
+                      % (it, cur["hpwl"], cur["overflow"], cur["max_density"], cur["density_weight"],
+                         cur["gamma"]), flush=True)
             self.optimizer.step()
-            if len(hist) > 1:
-                delta = cur["hpwl"] - hist[-2]["hpwl"]
-                if delta < 0:
-                    mu = p["upper_pcof"] * max(0.9999 ** float(it + 1), 0.98)
+            if self.best is None or self.best["overflow"] > cur["overflow"]:
+                self.best = cur
+                if self.best_pos is None:
+                    self.best_pos = x.clone()
                 else:
-                    mu = p["upper_pcof"] * min(max(p["upper_pcof"] ** (-delta / p["ref_hpwl"]),
-                                                   p["lower_pcof"]), p["upper_pcof"])
+                    self.best_pos.copy_(x)
+            overflow_list.append(cur["overflow"])
+            divergence_list.append([cur["hpwl"], cur["overflow"]])
+            if it - last_perturb > min_perturb_interval and check_plateau(overflow_list, 15, 0.001):
+                if overflow_list[-1] > 0.9:
+                    self.quad_penalty = True
+                    self.density_factor *= 2
+                    self._penalty()
+                    if overflow_list[-1] > 0.95:
+                        intensity = min(max(40 + (120 - 40) * (overflow_list[-1] - 0.95) * 10, 40), 90)
+                        self.entropy_injection(x, 0.996, intensity)
+                    last_perturb = it
+                    self.perturbations += 1
+            if len(hist) > 1:
+                mu = density_weight_factor(cur["hpwl"], hist[-2]["hpwl"], it + 1, p)
                 self.lam = float(T(self.lam) * T(mu))
                 self.objective.set_density_weight(self.lam)
             self.gamma = self.base_gamma * 10.0 ** ((cur["overflow"] - 0.1) * 20 / 9 - 1)
             self.objective.set_gamma(self.gamma)
-            if it > 100 and len(hist) > 1 and (
+            if len(hist) > 1 and it > 100 and (
                     (cur["overflow"] < p["stop_overflow"] and cur["hpwl"] > hist[-2]["hpwl"])
                     or cur["max_density"] < self.d["target_density"]):
                 break
+            if len(hist) > 50 and (cur["overflow"] > hist[-50]["overflow"]
+                                   and cur["hpwl"] > self.best["hpwl"] * 2):
+                break
+            if stop:
+                break
         return x, hist
+
+
+def main(argv=None):
+    """Run global placement on a synthetic design and print one JSON line
+    with the final metrics (at the last positions) and the timing."""
+    import argparse
+    import json
+    import time
+
+    from . import synth
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--config", type=int, default=0)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--iterations", type=int, default=1000)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--stop-overflow", type=float, default=None)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--history", default=None, help="write per-iteration metrics (JSON) here")
+    ap.add_argument("--log-every", type=int, default=100)
+    a = ap.parse_args(argv)
+    dt = np.float64 if a.precision == "fp64" else np.float32
+    t0 = time.time()
+    d = synth.make_design(a.config, seed=a.seed, dtype=dt, scale=a.scale)
+    t_gen = time.time() - t0
+    dev = torch.device("cuda", torch.cuda.current_device())
+    prm = {} if a.stop_overflow is None else dict(stop_overflow=a.stop_overflow)
+    gp = GlobalPlacer(d, dev, params=prm, precision=a.precision, algorithm="tiled", schedule="pins",
+                      seed=a.seed)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    x, hist = gp.run(a.iterations, log_every=a.log_every)
+    torch.cuda.synchronize()
+    t_run = time.time() - t1
+    gp.constraint(x)
+    fin = gp.metrics(x)
+    out = dict(config=a.config, scale=a.scale, precision=a.precision, dtype=str(np.dtype(dt)),
+               num_nodes=int(d["num_nodes"]), num_filler_nodes=int(d["num_filler_nodes"]),
+               num_pins=int(d["num_pins"]), bins=int(round((d["xh"] - d["xl"]) / d["bin_size_x"])),
+               iterations=len(hist), evaluations=int(gp.optimizer.param_groups[0]["obj_eval_count"]),
+               final_hpwl=fin["hpwl"], final_overflow=fin["overflow"],
+               final_max_density=fin["max_density"], best_overflow=gp.best["overflow"],
+               rolled_back=gp.rolled_back, perturbations=gp.perturbations,
+               seconds=t_run, ms_per_iteration=1e3 * t_run / max(len(hist), 1),
+               generate_seconds=t_gen)
+    if a.history:
+        with open(a.history, "w") as f:
+            json.dump(hist, f)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

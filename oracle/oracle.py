@@ -486,12 +486,15 @@ def pin_weight_sum(net_weights, flat_nodepin, nodepin_start, pin2net_map, num_no
     return out
 
 
-def objective_and_gradient(d, density_weight, gamma):
+def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_coeff=None):
     """obj = WA(pin_pos(pos)) + lambda*energy and its gradient for a design
     dict (dreamplace_amd.synth layout), composed from the functions above as
     PlaceObj.obj_fn / obj_and_grad_fn compose the ops
     (dreamplace/PlaceObj.py:293-320,394-411), FP64 spectral solve.
-    Returns (obj, grad[2N], parts)."""
+    With `density_factor` f and the quadratic penalty coefficient c
+    (PlaceObj.py:308-318): obj = WA + (f*lambda) * E*(1 + c*E), whose
+    derivative in E, g*(1 + c*E) + (g*E)*c with g = f*lambda, scales the field
+    gather.  Returns (obj, grad[2N], parts)."""
     dp = DensityParams(d["node_size_x"], d["node_size_y"], d["xl"], d["yl"], d["xh"], d["yh"],
                        d["bin_size_x"], d["bin_size_y"], d["num_movable_nodes"], d["num_terminals"],
                        d["num_filler_nodes"], d["target_density"], d["movable_macro_mask"])
@@ -503,13 +506,21 @@ def objective_and_gradient(d, density_weight, gamma):
     init = fixed_density_map(dp, pos, d["bin_center_x"], d["bin_center_y"])
     dmap = density_map(dp, pos, init)
     sol = poisson_solve(dmap, d["bin_size_x"], d["bin_size_y"])
-    dgrad = electric_force(dp, pos, sol["field_x"].astype(T), sol["field_y"].astype(T),
-                           grad=density_weight)
+    g = density_weight
+    energy = sol["energy"]
+    if quad_coeff is not None or density_factor != 1.0:
+        g = T.type(T.type(density_factor) * T.type(density_weight))
+        e = T.type(energy)
+        if quad_coeff is not None:
+            c = T.type(quad_coeff)
+            g = g * (T.type(1) + c * e) + (g * e) * c
+            energy = e * (T.type(1) + c * e)
+    dgrad = electric_force(dp, pos, sol["field_x"].astype(T), sol["field_y"].astype(T), grad=g)
     gpin = wa_merged_backward(1.0, gi, d["pin2net_map"], d["net_weights"],
                               d["net_mask_ignore_large_degrees"], d["pin_mask_ignore_fixed_macros"])
     wgrad = pin_pos_backward(gpin, d["flat_node2pin_map"], d["flat_node2pin_start_map"],
                              d["num_physical_nodes"], d["num_nodes"])
-    obj = wl + density_weight * sol["energy"]
+    obj = wl + (density_factor * density_weight) * energy
     parts = dict(sol)
     parts.update(wl=wl, density_map=dmap, density_grad=dgrad, wl_grad=wgrad)
     return obj, wgrad + dgrad, parts

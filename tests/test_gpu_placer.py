@@ -48,7 +48,7 @@ def test_placer_matches_oracle(gpu):
     gp = GlobalPlacer(d, gpu, dtype=torch.float64, precision="fp64", algorithm="tiled",
                       schedule="pins")
     x, hist = gp.run(8)
-    xr, hr = OP.place(d, 8)
+    xr, hr, _ = OP.place(d, 8)
     assert len(hist) == len(hr)
     for h, r in zip(hist, hr):
         for k in ("hpwl", "overflow", "max_density", "density_weight", "gamma"):
@@ -68,3 +68,39 @@ def test_placer_fp32_tracks_fp64(gpu):
     h32 = GlobalPlacer(d32, gpu, precision="bf16x3", algorithm="tiled", schedule="pins").run(30)[1]
     assert h32[-1]["hpwl"] == pytest.approx(h64[-1]["hpwl"], rel=1e-2)
     assert h32[-1]["hpwl"] < 0.5 * h32[0]["hpwl"]
+
+
+def test_placer_quadratic_penalty_matches_oracle(gpu):
+    """The loop with the quadratic density penalty and a doubled density
+    factor in force from the start (the plateau response) follows the
+    restated loop (FP64)."""
+    from dreamplace_amd.placer import GlobalPlacer
+    from oracle import placer as OP
+    print("case placer config0 fp64 quadratic penalty", flush=True)
+    d = design(0)
+    prm = dict(quad_penalty=True, density_factor=2.0)
+    gp = GlobalPlacer(d, gpu, params=prm, dtype=torch.float64, precision="fp64", algorithm="tiled",
+                      schedule="pins")
+    x, hist = gp.run(6)
+    xr, hr, info = OP.place(d, 6, params=prm)
+    for h, r in zip(hist, hr):
+        for k in ("hpwl", "overflow", "density_weight"):
+            assert h[k] == pytest.approx(r[k], rel=1e-9), k
+    assert rel_l2(x.cpu().numpy(), xr) <= 1e-9
+    assert gp.best["overflow"] == pytest.approx(info["best"]["overflow"], rel=1e-9)
+
+
+def test_entropy_injection_matches_oracle(gpu):
+    """Shrink + Gaussian noise + fixed-node restore on the device equals the
+    restated injection with the same draws."""
+    from dreamplace_amd.placer import GlobalPlacer
+    from oracle import placer as OP
+    print("case entropy injection", flush=True)
+    d = design(0)
+    gp = GlobalPlacer(d, gpu, dtype=torch.float64, precision="fp64", algorithm="tiled",
+                      schedule="pins")
+    x = gp.pos.clone()
+    ref = x.cpu().numpy().copy()
+    gp.entropy_injection(x, 0.996, 47.5)
+    OP.entropy_injection(gp.d, ref, 0.996, 47.5, OP.injection_noise(d.num_nodes, 0, 0))
+    assert np.allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-9)

@@ -202,3 +202,27 @@ def test_netlist_check_rejects_bad_maps():
     bad[4][-1] -= 1
     with pytest.raises(ValueError):
         check_netlist(*bad)
+
+
+def test_placer_control_logic_matches_oracle():
+    """The placer's plateau / divergence tests and density-weight factor agree
+    with the oracle's restatement on random histories."""
+    from dreamplace_amd import placer as P
+    from oracle import placer as OP
+    rng = np.random.default_rng(0)
+    for k in range(200):
+        n = int(rng.integers(10, 80))
+        base = rng.uniform(0.1, 0.5)
+        ov = list(base + rng.uniform(-0.05, 0.05, n) * rng.choice([0.0, 0.01, 1.0]))
+        wl = list(rng.uniform(90, 200, n))
+        pairs = list(zip(wl, ov))
+        best = dict(hpwl=float(rng.uniform(80, 150)), overflow=float(rng.uniform(0.1, 0.5)))
+        thr = float(rng.uniform(0.02, 0.4))
+        assert P.check_plateau([1] + ov, 15, 0.001) == OP.plateau([1] + ov, 15, 0.001)
+        assert bool(P.check_divergence(pairs, best, 0.1, 50, thr)) == OP.diverging(pairs, best, 0.1, 50, thr)
+    p = dict(P.DEFAULTS)
+    for cur, prev, it in [(90.0, 100.0, 3), (100.0, 90.0, 7), (1e6, 1.0, 2), (1.0, 1.0, 500)]:
+        mu = P.density_weight_factor(cur, prev, it, p)
+        assert p["upper_pcof"] * p["lower_pcof"] <= mu <= p["upper_pcof"] ** 2
+    assert P.density_weight_factor(1e9, 0.0, 1, p) == pytest.approx(p["upper_pcof"] * p["lower_pcof"])
+    assert P.density_weight_factor(0.0, 1.0, 1, p) == pytest.approx(p["upper_pcof"] * 0.9999)

@@ -17,6 +17,7 @@ import pytest
 import torch
 
 from _util import density_params, design, off_grid, rel_l2, relabel_pins
+from dreamplace_amd import synth
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -566,9 +567,84 @@ def test_placement_loop_decreases_hpwl(oracle):
     gamma follows the overflow schedule."""
     from oracle import placer as OP
     d = design(0)
-    pos, hist = OP.place(d, 6)
+    pos, hist, _ = OP.place(d, 6)
     assert hist[-1]["hpwl"] < 0.5 * hist[0]["hpwl"]
     base = OP.DEFAULTS["gamma"] * (d.bin_size_x + d.bin_size_y)
     assert hist[0]["gamma"] == pytest.approx(10 * base)
     for h0, h1 in zip(hist, hist[1:]):
         assert h1["gamma"] == pytest.approx(base * 10 ** ((h0["overflow"] - 0.1) * 20 / 9 - 1))
+
+
+def test_plateau_and_divergence_known_answers():
+    """check_plateau / check_divergence (NonLinearPlace.py:527-570) on
+    histories whose answer follows from the definitions."""
+    from oracle import placer as OP
+    assert OP.plateau([1.0] * 15, 15, 0.001)
+    assert not OP.plateau([1.0] * 14, 15, 0.001)          # window not full
+    assert not OP.plateau(list(np.linspace(1.0, 0.9, 15)), 15, 0.001)
+    best = dict(hpwl=100.0, overflow=0.2)
+    flat_wl = [(100.0, 0.3)] * 50
+    assert not OP.diverging(flat_wl, best, 0.1, 50, 0.05)  # wirelength has not grown
+    grown = [(150.0, 0.3)] * 50                            # overflow mean 50% above best
+    assert OP.diverging(grown, best, 0.1, 50, 0.05)
+    assert not OP.diverging(grown[:49], best, 0.1, 50, 0.05)
+    assert not OP.diverging(grown, None, 0.1, 50, 0.05)
+    level = [(150.0, 0.2)] * 50                            # overflow flat at the best
+    assert OP.diverging(level, best, 0.1, 50, 0.05)
+    # overflow at the best on average, spread 13% of the mean: only the
+    # fraction of rising steps decides (40 of 49 rise; 25 of 49 rise)
+    best3 = dict(hpwl=100.0, overflow=0.3)
+    saw = [(150.0, 0.28 + 0.01 * (i % 5)) for i in range(50)]
+    assert OP.diverging(saw, best3, 0.1, 50, 0.05)
+    alt = [(150.0, 0.28 + 0.04 * (i % 2)) for i in range(50)]
+    assert not OP.diverging(alt, best3, 0.1, 50, 0.05)
+    falling = [(150.0, 0.2 - 0.002 * i) for i in range(50)]  # steady descent, ratio < 0
+    assert not OP.diverging(falling, dict(hpwl=100.0, overflow=0.2), 0.1, 50, 0.05)
+
+
+def test_entropy_injection_invariants():
+    """Shrink about the movables' centroid keeps that centroid and scales
+    every offset by the factor; fixed nodes stay put; the noise is added as
+    given."""
+    from oracle import placer as OP
+    d = design(0)
+    n, m0 = d.num_nodes, d.num_movable_nodes
+    m1 = n - d.num_filler_nodes
+    pos = d.pos.copy()
+    xc = pos[:m0].mean()
+    OP.entropy_injection(d, pos, 0.5, 0.0, None)
+    assert pos[:m0].mean() == pytest.approx(xc, rel=1e-12)
+    assert np.allclose(pos[:m0] - xc, 0.5 * (d.pos[:m0] - xc), rtol=0, atol=1e-9)
+    assert np.array_equal(pos[m0:m1], d.pos[m0:m1]) and np.array_equal(pos[n + m0:n + m1], d.pos[n + m0:n + m1])
+    pos2 = d.pos.copy()
+    z = OP.injection_noise(n, 0, 0)
+    OP.entropy_injection(d, pos2, 1, 2.0, z)
+    mov = np.r_[0:m0, m1:n + m0, n + m1:2 * n]
+    assert np.allclose(pos2[mov] - d.pos[mov], 2.0 * z[mov], rtol=0, atol=1e-12)
+    assert np.array_equal(pos2[m0:m1], d.pos[m0:m1])
+
+
+def test_quadratic_penalty_derivative(oracle):
+    """With the quadratic penalty the density part of the objective is
+    Q(E) = f*lambda*E*(1 + c*E): along a direction u, the ratio of the
+    density gradients (penalised / plain, lambda = 1) equals the ratio of
+    finite differences of Q and E along u, f*lambda*(1 + c*(E(+t) + E(-t)))."""
+    d = design(0)
+    gamma, lam, f = d.wa_gamma, 0.7, 2.0
+    _, _, p0 = oracle.objective_and_gradient(d, 1.0, gamma)
+    e0 = p0["energy"]
+    c = 1000.0 / e0
+    _, _, pq = oracle.objective_and_gradient(d, lam, gamma, density_factor=f, quad_coeff=c)
+    u = np.zeros(2 * d.num_nodes)
+    u[:d.num_movable_nodes] = np.random.default_rng(5).standard_normal(d.num_movable_nodes)
+    t = 1e-3
+    es = []
+    for s in (t, -t):
+        d2 = synth.Design(d)
+        d2.pos = d.pos + s * u
+        es.append(oracle.objective_and_gradient(d2, 1.0, gamma)[2]["energy"])
+    q = lambda e: f * lam * e * (1 + c * e)
+    fd_ratio = (q(es[0]) - q(es[1])) / (es[0] - es[1])
+    grad_ratio = float(np.dot(pq["density_grad"], u)) / float(np.dot(p0["density_grad"], u))
+    assert grad_ratio == pytest.approx(fd_ratio, rel=1e-6)
+    assert grad_ratio == pytest.approx(f * lam * (1 + 2 * c * e0), rel=1e-6)
