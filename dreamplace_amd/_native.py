@@ -55,6 +55,7 @@ _SIGNATURES = {
     "dp_pin_pos_backward": (_I, [_I, _P, _L, _P, _P, _L, _L, _P, _P]),
     "dp_wa_workspace_bytes": (_S, [_L]),
     "dp_wa_merged_forward": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dp_lse_merged_forward": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_merged_backward": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_wa_net_weight_grad": (_I, [_I, _P, _P, _P, _L, _P, _P]),
     "dp_wa_forward_from_nodes": (_I, [_I, _P, _L, _P, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P,
@@ -68,6 +69,8 @@ _SIGNATURES = {
     "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
     "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
                                 _P, _P, _P, _P, _P]),
+    "dp_lse_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
+                                 _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),

@@ -110,6 +110,63 @@ __global__ void __launch_bounds__(256) k_wa_fwd(const T* __restrict__ px, const 
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
+// Merged log-sum-exp wirelength, one thread per net (both axes)
+// (logsumexp_wirelength_merged.cpp:14-85): pass 1 max/min, pass 2 the
+// max-shifted exponential sums, pass 3 d/dx = e+/S+ - e-/S- per pin.
+// WL = (log S+ + log S-)*gamma + max - min per axis, in the reference's order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lse_fwd(const T* __restrict__ px, const T* __restrict__ py,
+                                                 const int* __restrict__ flat_netpin,
+                                                 const int* __restrict__ netpin_start, int64_t E,
+                                                 const T* __restrict__ w,
+                                                 const uint8_t* __restrict__ mask,
+                                                 const T* __restrict__ gamma_p, T* partial, T* gix,
+                                                 T* giy, double* partials) {
+  __shared__ double smem[4];
+  double contrib = 0;
+  const T gam = *gamma_p;
+  const T ig = T(1) / gam;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (!mask[e]) {
+      if (partial) partial[e] = 0;
+      continue;
+    }
+    const int b = netpin_start[e], en = netpin_start[e + 1];
+    T xmax = -tmax_value<T>(), xmin = tmax_value<T>();
+    T ymax = -tmax_value<T>(), ymin = tmax_value<T>();
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      T xx = px[p], yy = py[p];
+      xmax = xx > xmax ? xx : xmax;
+      xmin = xx < xmin ? xx : xmin;
+      ymax = yy > ymax ? yy : ymax;
+      ymin = yy < ymin ? yy : ymin;
+    }
+    T es = 0, ens = 0, fs = 0, fns = 0;
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      T xx = px[p], yy = py[p];
+      es += exp((xx - xmax) * ig);
+      ens += exp((xmin - xx) * ig);
+      fs += exp((yy - ymax) * ig);
+      fns += exp((ymin - yy) * ig);
+    }
+    T wl = (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin;
+    if (partial) partial[e] = wl;
+    contrib += (double)(w ? wl * w[e] : wl);
+    const T rx = (T)(1.0 / es), rnx = (T)(1.0 / ens), ry = (T)(1.0 / fs), rny = (T)(1.0 / fns);
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      T xx = px[p], yy = py[p];
+      gix[p] = exp((xx - xmax) * ig) * rx - exp((xmin - xx) * ig) * rnx;
+      giy[p] = exp((yy - ymax) * ig) * ry - exp((ymin - yy) * ig) * rny;
+    }
+  }
+  double bs = block_sum<256>(contrib, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
 // g[p] = (gi[p]*grad)*w[net(p)] for unmasked nets; 0 where pin_mask
 // (weighted_average_wirelength_cuda_merged.cpp:135, integrateNetWeights,
 // weighted_average_wirelength.py:207-208).
@@ -378,6 +435,31 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
                                  (const T*)net_weights, net_mask, (const T*)inv_gamma,
                                  (T*)partial_wl, gi, gi + num_pins, partials));
     DP_LAUNCHED("k_wa_fwd");
+    return finish_reduction(partials, grid, 1, nullptr, wl_out, dtype, s);
+  });
+}
+
+int dp_lse_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
+                          const int32_t* flat_netpin, const int32_t* netpin_start,
+                          int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                          const void* gamma, void* partial_wl, void* grad_intermediate,
+                          void* wl_out, void* workspace, void* stream) {
+  DP_ARG(pin_pos && flat_netpin && netpin_start && net_mask && gamma && grad_intermediate &&
+             wl_out && workspace,
+         "dp_lse_merged_forward: null pointer");
+  hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_WA_FORWARD, s);
+  double* partials = (double*)workspace;
+  int grid = reduce_grid(num_nets);
+  return DP_DISPATCH(dtype, "dp_lse_merged_forward", [&] {
+    const T* pp = (const T*)pin_pos;
+    T* gi = (T*)grad_intermediate;
+    DP_PROBED(DP_KERNEL_WA_FORWARD, s,
+              hipLaunchKernelGGL(k_lse_fwd<T>, dim3(grid), dim3(256), 0, s, pp, pp + num_pins,
+                                 (const int*)flat_netpin, (const int*)netpin_start, num_nets,
+                                 (const T*)net_weights, net_mask, (const T*)gamma,
+                                 (T*)partial_wl, gi, gi + num_pins, partials));
+    DP_LAUNCHED("k_lse_fwd");
     return finish_reduction(partials, grid, 1, nullptr, wl_out, dtype, s);
   });
 }

@@ -133,11 +133,13 @@ __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
 // its own grid-stride loop over groups; they exchange data only within the
 // wavefront (wave_sync), so no wavefront waits for another until the final
 // workgroup sum.
-template <typename T>
+// kLse: the log-sum-exp model (logsumexp_wirelength_merged.cpp:14-85) on the
+// same skeleton; g_p then points at gamma instead of 1/gamma.
+template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_pins(
     const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
-    const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial,
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial,
     T* gnode, double* partials) {
   typedef typename Vec2<T>::type V;
   // per-lane words read by the net's first lane: one LDS read per pin for
@@ -150,7 +152,8 @@ __global__ void __launch_bounds__(256) k_wa_pins(
   V* s_xy = sh_xy[wv];
   Exp4<T>* s_e = sh_e[wv];
   T(*s_c)[64] = sh_c[wv];
-  const T ig = *inv_gamma_p;
+  const T gam = kLse ? *g_p : T(0);
+  const T ig = kLse ? T(1) / gam : *g_p;
   double contrib = 0;
   for (int g = blockIdx.x * kWaves + wv; g < ngroups; g += gridDim.x * kWaves) {
     const int pb = group_start[g], pe = group_start[g + 1];
@@ -179,8 +182,8 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     }
     wave_sync();
     const bool is_head = on && lane == head;
+    T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
     if (is_head && live) {
-      T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
       for (int k = head; k < head + deg; ++k) {
         const V v = s_xy[k];
         const T xx = v.x, yy = v.y;
@@ -209,7 +212,24 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       s_e[lane] = q;
     }
     wave_sync();
-    if (is_head && live) {
+    if (kLse && is_head && live) {
+      T es = 0, ens = 0, fs = 0, fns = 0;
+      for (int k = head; k < head + deg; ++k) {
+        const Exp4<T> q = s_e[k];
+        es += q.a0;
+        ens += q.a1;
+        fs += q.a2;
+        fns += q.a3;
+      }
+      const T wl = (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin;
+      if (partial) partial[e] = wl;
+      contrib += (double)(w ? wl * w[e] : wl);
+      s_c[0][lane] = (T)(1.0 / es);
+      s_c[1][lane] = (T)(1.0 / ens);
+      s_c[2][lane] = (T)(1.0 / fs);
+      s_c[3][lane] = (T)(1.0 / fns);
+    }
+    if (!kLse && is_head && live) {
       T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
       for (int k = head; k < head + deg; ++k) {
         const V v = s_xy[k];
@@ -242,8 +262,13 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     if (on) {
       T gx = 0, gy = 0;
       if (live && !pin_masked) {
-        gx = (s_c[0][head] + s_c[1][head] * x) * ex - (s_c[2][head] + s_c[3][head] * x) * enx;
-        gy = (s_c[4][head] + s_c[5][head] * y) * ey - (s_c[6][head] + s_c[7][head] * y) * eny;
+        if (kLse) {
+          gx = ex * s_c[0][head] - enx * s_c[1][head];
+          gy = ey * s_c[2][head] - eny * s_c[3][head];
+        } else {
+          gx = (s_c[0][head] + s_c[1][head] * x) * ex - (s_c[2][head] + s_c[3][head] * x) * enx;
+          gy = (s_c[4][head] + s_c[5][head] * y) * ey - (s_c[6][head] + s_c[7][head] * y) * eny;
+        }
         if (w) {
           const T wn = w[e];
           gx *= wn;
@@ -260,17 +285,18 @@ __global__ void __launch_bounds__(256) k_wa_pins(
 
 // Nets of more than 64 pins: one wavefront per net; big[q] = {net, first
 // group-order slot, degree, 0}.
-template <typename T>
+template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_big(
     const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
     const int4* __restrict__ big, int nbig, const T* __restrict__ w,
-    const uint8_t* __restrict__ net_mask, const T* __restrict__ inv_gamma_p, T* partial, T* gnode,
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial, T* gnode,
     double* partials) {
   __shared__ double s_red[4];
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
-  const T ig = *inv_gamma_p;
+  const T gam = kLse ? *g_p : T(0);
+  const T ig = kLse ? T(1) / gam : *g_p;
   double contrib = 0;
   for (int q = wave; q < nbig; q += nwaves) {
     const int4 bq = big[q];
@@ -311,22 +337,32 @@ __global__ void __launch_bounds__(256) k_wa_big(
     yns = wave_sum(yns);
     fs = wave_sum(fs);
     fns = wave_sum(fns);
-    const T wl = xs / es - xns / ens + ys / fs - yns / fns;
+    const T wl = kLse ? (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin
+                      : xs / es - xns / ens + ys / fs - yns / fns;
     if (lane == 0) {
       if (partial && live) partial[e] = wl;
       if (live) contrib += (double)(w ? wl * w[e] : wl);
     }
-    const T bx = ig / es, bnx = -ig / ens, by = ig / fs, bny = -ig / fns;
-    const T ax = (T)((1.0 - bx * xs) / es), anx = (T)((1.0 - bnx * xns) / ens);
-    const T ay = (T)((1.0 - by * ys) / fs), any = (T)((1.0 - bny * yns) / fns);
+    // WA: d/dx = (a + b x) e+ - (a- + b- x) e-; LSE: a = 1/S, b = 0
+    const T bx = kLse ? T(0) : ig / es, bnx = kLse ? T(0) : -ig / ens;
+    const T by = kLse ? T(0) : ig / fs, bny = kLse ? T(0) : -ig / fns;
+    const T ax = kLse ? (T)(1.0 / es) : (T)((1.0 - bx * xs) / es);
+    const T anx = kLse ? (T)(1.0 / ens) : (T)((1.0 - bnx * xns) / ens);
+    const T ay = kLse ? (T)(1.0 / fs) : (T)((1.0 - by * ys) / fs);
+    const T any = kLse ? (T)(1.0 / fns) : (T)((1.0 - bny * yns) / fns);
     for (int j = pb + lane; j < pe; j += 64) {
       const int4 m = meta[j];
       T gx = 0, gy = 0;
       if (live && m.x >= 0) {
         const typename Vec2<T>::type v = pin_xy[j];
         const T xx = v.x, yy = v.y;
-        gx = (ax + bx * xx) * exp((xx - xmax) * ig) - (anx + bnx * xx) * exp((xmin - xx) * ig);
-        gy = (ay + by * yy) * exp((yy - ymax) * ig) - (any + bny * yy) * exp((ymin - yy) * ig);
+        if (kLse) {
+          gx = exp((xx - xmax) * ig) * ax - exp((xmin - xx) * ig) * anx;
+          gy = exp((yy - ymax) * ig) * ay - exp((ymin - yy) * ig) * any;
+        } else {
+          gx = (ax + bx * xx) * exp((xx - xmax) * ig) - (anx + bnx * xx) * exp((xmin - xx) * ig);
+          gy = (ay + by * yy) * exp((yy - ymax) * ig) - (any + bny * yy) * exp((ymin - yy) * ig);
+        }
         if (w) {
           gx *= w[e];
           gy *= w[e];
@@ -499,27 +535,27 @@ size_t dp_wa_pins_workspace_bytes(int64_t num_pins, int dtype) {
   return reduce_slot_bytes() + align256(eb * (size_t)(num_pins > 0 ? num_pins : 1));
 }
 
-int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+static int forward_pins(bool lse, const char* name, int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
                        const int32_t* slot_node, const void* node_offsets,
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
-                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
+                       const uint8_t* net_mask, const void* g_p, void* partial_wl,
                        void* grad_node_order, void* wl_out, void* workspace, void* stream) {
   DP_ARG(pos && slot_node && node_offsets && node_dst && pin_meta &&
              (group_start || num_groups == 0) && (big_nets || num_big == 0) && net_mask &&
-             inv_gamma && grad_node_order && wl_out && workspace,
-         "dp_wa_forward_pins: null pointer");
+             g_p && grad_node_order && wl_out && workspace,
+         "%s: null pointer", name);
   DP_ARG(num_pins >= 0 && num_pins < ((int64_t)1 << 31) && num_groups >= 0 && num_big >= 0 &&
              num_nets >= 0,
-         "dp_wa_forward_pins: bad counts");
+         "%s: bad counts", name);
   hipStream_t s = S(stream);
   DP_PROBE_OP(DP_OP_WA_FORWARD, s);
   double* partials = (double*)workspace;
   char* pin_xy = (char*)workspace + reduce_slot_bytes();
   const int g1 = grid_cap(num_groups, kWaves, kPackedGrid);
   const int g2 = grid_cap(num_big, 4, kBigGrid);
-  return DP_DISPATCH(dtype, "dp_wa_forward_pins", [&] {
+  return DP_DISPATCH(dtype, name, [&] {
     typedef typename Vec2<T>::type V;
     const T* P = (const T*)pos;
     if (partial_wl && num_nets > 0) {  // nets without a live group entry keep 0
@@ -535,19 +571,47 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                                    (const int*)node_dst, (V*)pin_xy));
       DP_LAUNCHED("k_pin_push");
     }
+    auto packed = lse ? k_wa_pins<T, true> : k_wa_pins<T, false>;
+    auto bignet = lse ? k_wa_big<T, true> : k_wa_big<T, false>;
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
-              hipLaunchKernelGGL(k_wa_pins<T>, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
+              hipLaunchKernelGGL(packed, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
                                  (const int4*)pin_meta, (const int*)group_start, (int)num_groups,
-                                 (const T*)net_weights, net_mask, (const T*)inv_gamma,
+                                 (const T*)net_weights, net_mask, (const T*)g_p,
                                  (T*)partial_wl, (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
-    hipLaunchKernelGGL(k_wa_big<T>, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
+    hipLaunchKernelGGL(bignet, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
                        (const int4*)pin_meta, (const int4*)big_nets, (int)num_big,
-                       (const T*)net_weights, net_mask, (const T*)inv_gamma, (T*)partial_wl,
+                       (const T*)net_weights, net_mask, (const T*)g_p, (T*)partial_wl,
                        (T*)grad_node_order, partials + g1);
     DP_LAUNCHED("k_wa_big");
     return finish_reduction(partials, g1 + g2, 1, nullptr, wl_out, dtype, s);
   });
+}
+
+int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                       const int32_t* slot_node, const void* node_offsets,
+                       const int32_t* node_dst, const int32_t* pin_meta,
+                       const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
+                       int64_t num_big, int64_t num_nets, const void* net_weights,
+                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
+                       void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+  return forward_pins(false, "dp_wa_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
+                      node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
+                      num_nets, net_weights, net_mask, inv_gamma, partial_wl, grad_node_order,
+                      wl_out, workspace, stream);
+}
+
+int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                        const int32_t* slot_node, const void* node_offsets,
+                        const int32_t* node_dst, const int32_t* pin_meta,
+                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
+                        int64_t num_big, int64_t num_nets, const void* net_weights,
+                        const uint8_t* net_mask, const void* gamma, void* partial_wl,
+                        void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+  return forward_pins(true, "dp_lse_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
+                      node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
+                      num_nets, net_weights, net_mask, gamma, partial_wl, grad_node_order, wl_out,
+                      workspace, stream);
 }
 
 int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,

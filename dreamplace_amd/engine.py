@@ -141,10 +141,17 @@ class PinPlan:
 class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
                  energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
-                 keep_partial_wl=False):
-        """keep_partial_wl: also store the per-net (pre-weight) WA wirelength in
+                 keep_partial_wl=False, wirelength="weighted_average"):
+        """keep_partial_wl: also store the per-net (pre-weight) wirelength in
         `partial_wl` (what the net-weight adjoint needs); the pins schedule
-        skips that store otherwise."""
+        skips that store otherwise.  wirelength: "weighted_average" or
+        "logsumexp" (PlaceObj.py:467-503); the log-sum-exp model runs on the
+        "pins" and "unfused" schedules."""
+        if wirelength not in ("weighted_average", "logsumexp"):
+            raise ValueError("wirelength must be weighted_average or logsumexp, got %r" % (wirelength,))
+        self.lse = wirelength == "logsumexp"
+        if self.lse and (schedule or ("fused" if fused else "unfused")) == "fused":
+            raise NotImplementedError("the log-sum-exp wirelength runs on the pins or unfused schedule")
         self.dd = dd
         self.keep_partial_wl = keep_partial_wl
         dt = dd.dtype
@@ -166,6 +173,7 @@ class PlacementObjective:
         M, Nb = self.density.num_bins_x, self.density.num_bins_y
         z = lambda *s, dtype=dt: torch.empty(*s, dtype=dtype, device=dev)
         self.inv_gamma = torch.tensor(1.0 / gamma, dtype=dt, device=dev)
+        self.gamma_t = torch.tensor(float(gamma), dtype=dt, device=dev)
         self.lam = torch.tensor(float(density_weight), dtype=dt, device=dev)
         self.lam_host = float(density_weight)
         self.density_factor = 1.0
@@ -203,6 +211,7 @@ class PlacementObjective:
 
     def set_gamma(self, gamma):
         self.inv_gamma.fill_(1.0 / gamma)
+        self.gamma_t.fill_(float(gamma))
 
     def set_density_weight(self, w):
         self.lam_host = float(w)
@@ -277,10 +286,11 @@ class PlacementObjective:
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
         if self.schedule == "pins":
             pp = self.pin_plan
-            N.call("dp_wa_forward_pins", code, N.ptr(pos), n, P, N.ptr(pp.slot_node),
-                   N.ptr(pp.node_offsets), N.ptr(pp.node_dst), N.ptr(pp.pin_meta),
-                   N.ptr(pp.group_start), pp.num_groups, N.ptr(pp.big), pp.num_big, E,
-                   N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.inv_gamma),
+            N.call("dp_lse_forward_pins" if self.lse else "dp_wa_forward_pins", code, N.ptr(pos), n,
+                   P, N.ptr(pp.slot_node), N.ptr(pp.node_offsets), N.ptr(pp.node_dst),
+                   N.ptr(pp.pin_meta), N.ptr(pp.group_start), pp.num_groups, N.ptr(pp.big),
+                   pp.num_big, E, N.ptr(self.w), N.ptr(self.net_mask),
+                   N.ptr(self.gamma_t if self.lse else self.inv_gamma),
                    N.ptr(self.partial_wl) if self.keep_partial_wl else None,
                    N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)
             self._density_forward(pos)
@@ -304,10 +314,11 @@ class PlacementObjective:
             N.call("dp_pin_pos_forward", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, P,
                    N.ptr(self.pin_pos), s)
-            N.call("dp_wa_merged_forward", code, N.ptr(self.pin_pos), P, N.ptr(dd.flat_net2pin_map),
+            N.call("dp_lse_merged_forward" if self.lse else "dp_wa_merged_forward", code,
+                   N.ptr(self.pin_pos), P, N.ptr(dd.flat_net2pin_map),
                    N.ptr(dd.flat_net2pin_start_map), E, N.ptr(self.w), N.ptr(self.net_mask),
-                   N.ptr(self.inv_gamma), N.ptr(self.partial_wl), N.ptr(self.grad_intermediate),
-                   N.ptr(self.wl), N.ptr(self.wa_ws), s)
+                   N.ptr(self.gamma_t if self.lse else self.inv_gamma), N.ptr(self.partial_wl),
+                   N.ptr(self.grad_intermediate), N.ptr(self.wl), N.ptr(self.wa_ws), s)
             self._density_forward(pos)
             self._density_backward(pos)
             N.call("dp_wa_merged_backward", code, N.ptr(self.one), N.ptr(self.grad_intermediate), P,

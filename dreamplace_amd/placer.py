@@ -30,7 +30,7 @@ from .nesterov import NesterovAcceleratedGradientOptimizer
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
-                density_factor=1.0)
+                density_factor=1.0, early_stop=True)
 
 
 def noise(d, ratio, seed):
@@ -93,7 +93,10 @@ def density_weight_factor(cur_hpwl, prev_hpwl, flat_iteration, p):
 
 class GlobalPlacer:
     """`d`: a design dict (dreamplace_amd.synth layout).  Objective options
-    (precision, algorithm, schedule) go to PlacementObjective."""
+    (precision, algorithm, schedule) go to PlacementObjective.  `params`
+    override DEFAULTS; early_stop=False skips the stop tests and the
+    divergence roll-back, so a run takes exactly the iterations asked for
+    (to compare runs step for step)."""
 
     def __init__(self, d, device, params=None, dtype=None, seed=0, **objective_kw):
         p = dict(DEFAULTS)
@@ -249,7 +252,7 @@ class GlobalPlacer:
         last_perturb, min_perturb_interval = -50, 50
         stop = False
         for it in range(iterations):
-            if (p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
+            if (p["early_stop"] and p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
                     and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
                                          threshold=overflow_list[-1])):
                 x.copy_(self.best_pos)
@@ -289,6 +292,8 @@ class GlobalPlacer:
                 self.objective.set_density_weight(self.lam)
             self.gamma = self.base_gamma * 10.0 ** ((cur["overflow"] - 0.1) * 20 / 9 - 1)
             self.objective.set_gamma(self.gamma)
+            if not p["early_stop"]:
+                continue
             if len(hist) > 1 and it > 100 and (
                     (cur["overflow"] < p["stop_overflow"] and cur["hpwl"] > hist[-2]["hpwl"])
                     or cur["max_density"] < self.d["target_density"]):
@@ -315,6 +320,8 @@ def main(argv=None):
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--stop-overflow", type=float, default=None)
+    ap.add_argument("--fixed-iterations", action="store_true",
+                    help="no stop tests: run exactly --iterations steps")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--history", default=None, help="write per-iteration metrics (JSON) here")
     ap.add_argument("--log-every", type=int, default=100)
@@ -325,6 +332,8 @@ def main(argv=None):
     t_gen = time.time() - t0
     dev = torch.device("cuda", torch.cuda.current_device())
     prm = {} if a.stop_overflow is None else dict(stop_overflow=a.stop_overflow)
+    if a.fixed_iterations:
+        prm["early_stop"] = False
     gp = GlobalPlacer(d, dev, params=prm, precision=a.precision, algorithm="tiled", schedule="pins",
                       seed=a.seed)
     torch.cuda.synchronize()

@@ -231,6 +231,21 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
                          const void* inv_gamma, void* partial_wl, void* grad_intermediate,
                          void* wl_out, void* workspace, void* stream);
 
+/* Replaces logsumexp_wirelength_cuda_merged.forward
+ * (logsumexp_wirelength_cuda_merged.cpp; CPU twin
+ * logsumexp_wirelength_merged.cpp:99-146): per unmasked net and axis
+ * WL = (log S+ + log S-)*gamma + max - min, S+- the max-shifted exponential
+ * sums; grad_intermediate[2P] = e+/S+ - e-/S- per pin (untouched for masked
+ * nets: the caller zeroes it, as the reference allocates it with zeros).
+ * `gamma` is a device scalar of `dtype`.  The backward
+ * (logsumexp_wirelength_merged.cpp:160-205) is dp_wa_merged_backward: the
+ * same grad, net-weight and pin-mask step on grad_intermediate. */
+int dp_lse_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
+                          const int32_t* flat_netpin, const int32_t* netpin_start,
+                          int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                          const void* gamma, void* partial_wl, void* grad_intermediate,
+                          void* wl_out, void* workspace, void* stream);
+
 /* Replaces weighted_average_wirelength_cuda_merged.backward
  * (weighted_average_wirelength_cuda_merged.cpp:121-160,
  * integrate_net_weights_cuda_kernel.cu:17-48) plus the pin_mask fill of
@@ -324,6 +339,19 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        int64_t num_big, int64_t num_nets, const void* net_weights,
                        const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
                        void* grad_node_order, void* wl_out, void* workspace, void* stream);
+
+/* The log-sum-exp model on the same plan and skeleton: the chain
+ * LSE(PinPos(pos)) of logsumexp_wirelength_merged.cpp:14-85 (CUDA twin
+ * logsumexp_wirelength_cuda_merged_kernel.cu), with `gamma` (device scalar of
+ * `dtype`, not its inverse) as LogSumExpWirelength passes it.  Outputs as
+ * dp_wa_forward_pins; dp_node_grad_from_pins finishes the backward. */
+int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                        const int32_t* slot_node, const void* node_offsets,
+                        const int32_t* node_dst, const int32_t* pin_meta,
+                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
+                        int64_t num_big, int64_t num_nets, const void* net_weights,
+                        const uint8_t* net_mask, const void* gamma, void* partial_wl,
+                        void* grad_node_order, void* wl_out, void* workspace, void* stream);
 
 /* Backward of that chain plus the gradient combine: grad_pos[n] =
  * (*grad) * (sum of grad_node_order over node n's CSR range) + add[n]

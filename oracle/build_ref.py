@@ -38,6 +38,9 @@ MODULES = {
     "weighted_average_wirelength_cpp_merged": [
         "weighted_average_wirelength/src/weighted_average_wirelength_merged.cpp"
     ],
+    "logsumexp_wirelength_cpp_merged": [
+        "logsumexp_wirelength/src/logsumexp_wirelength_merged.cpp"
+    ],
     "pin_pos_cpp": ["pin_pos/src/pin_pos.cpp"],
     "hpwl_cpp": ["hpwl/src/hpwl.cpp"],
     "move_boundary_cpp": ["move_boundary/src/move_boundary.cpp"],

@@ -373,6 +373,57 @@ def wa_merged_forward(pin_pos, flat_netpin, netpin_start, net_weights, net_mask,
     return wl, partial, gi
 
 
+def lse_merged_forward(pin_pos, flat_netpin, netpin_start, net_weights, net_mask, gamma):
+    """Merged log-sum-exp wirelength forward
+    (logsumexp_wirelength_merged.cpp:14-85, 99-146), per unmasked net and axis:
+    WL = (log S+ + log S-)*gamma + max - min with S+ = sum exp((x-max)/gamma),
+    S- = sum exp((min-x)/gamma); d/dx_p = e+_p/S+ - e-_p/S-.  Returns
+    (wl = sum w_n*WL_n in FP64, partial_wl[E] pre-weight, grad_intermediate[2P]).
+    Sums run in pin order in T; exponentials and logs are numpy's in T.
+    Pinned by: oracle/_ref merged forward, the two-pin closed form
+    |d| + 2*gamma*log(1 + exp(-|d|/gamma)), the gamma->0 HPWL limit and
+    central finite differences."""
+    pin_pos = np.asarray(pin_pos)
+    T = pin_pos.dtype.type
+    P = pin_pos.size // 2
+    fnp = np.asarray(flat_netpin, dtype=np.int64)
+    nps = np.asarray(netpin_start, dtype=np.int64)
+    E = len(nps) - 1
+    mask = np.asarray(net_mask, dtype=bool)
+    g = T(gamma)
+    ig = T(1) / g
+    partial = np.zeros(E, dtype=T)
+    gi = np.zeros(2 * P, dtype=T)
+    deg = np.diff(nps)
+    live = mask & (deg > 0)
+    net_of = np.repeat(np.arange(E), deg)
+    for ax in (0, 1):
+        v = pin_pos[ax * P:(ax + 1) * P][fnp]
+        vmax = np.full(E, -np.finfo(T).max, dtype=T)
+        vmin = np.full(E, np.finfo(T).max, dtype=T)
+        np.maximum.at(vmax, net_of, v)
+        np.minimum.at(vmin, net_of, v)
+        ep = np.exp((v - vmax[net_of]) * ig)
+        en = np.exp((vmin[net_of] - v) * ig)
+        sp = np.zeros(E, dtype=T)
+        sn = np.zeros(E, dtype=T)
+        np.add.at(sp, net_of, ep)   # unbuffered, in pin order
+        np.add.at(sn, net_of, en)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            term = (np.log(sp) + np.log(sn)) * g
+            rp, rn = T(1) / sp, T(1) / sn
+        if ax == 0:
+            partial = np.where(live, (term + vmax) - vmin, T(0)).astype(T)
+        else:
+            partial = np.where(live, ((partial + term) + vmax) - vmin, T(0)).astype(T)
+        d = ep * rp[net_of] - en * rn[net_of]
+        on = live[net_of]
+        gi[ax * P + fnp[on]] = d[on]
+    w = np.asarray(net_weights, dtype=T)
+    wl = float(np.sum((partial * w).astype(np.float64))) if w.size else float(np.sum(partial.astype(np.float64)))
+    return wl, partial, gi
+
+
 def wa_merged_backward(grad, grad_intermediate, pin2net_map, net_weights, net_mask, pin_mask):
     """WA backward (weighted_average_wirelength.py:197-213,
     weighted_average_wirelength_merged.cpp:193-230, functional.h:10-34):

@@ -37,7 +37,8 @@ def ref():
         sys.path.insert(0, d)
     mods = {}
     for name in ("electric_potential_cpp", "dct2_fft2_cpp", "weighted_average_wirelength_cpp_merged",
-                 "pin_pos_cpp", "hpwl_cpp", "move_boundary_cpp", "pws_cpp"):
+                 "logsumexp_wirelength_cpp_merged", "pin_pos_cpp", "hpwl_cpp", "move_boundary_cpp",
+                 "pws_cpp"):
         try:
             mods[name] = importlib.import_module(name)
         except ImportError as e:  # pragma: no cover

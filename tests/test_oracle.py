@@ -648,3 +648,80 @@ def test_quadratic_penalty_derivative(oracle):
     grad_ratio = float(np.dot(pq["density_grad"], u)) / float(np.dot(p0["density_grad"], u))
     assert grad_ratio == pytest.approx(fd_ratio, rel=1e-6)
     assert grad_ratio == pytest.approx(f * lam * (1 + 2 * c * e0), rel=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# log-sum-exp wirelength (oracle.lse_merged_forward)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-5)], ids=["f64", "f32"])
+def test_lse_matches_reference(oracle, ref, dtype, tol):
+    d = design(0, dtype=dtype, big_nets=5)
+    pin = oracle.pin_pos_forward(d.pos, d.pin_offset_x, d.pin_offset_y, d.pin2node_map)
+    m = ref["logsumexp_wirelength_cpp_merged"]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    gam = torch.tensor(d.wa_gamma, dtype=T(pin).dtype)
+    wl_ref, gi_ref = m.forward(T(pin), T(d.flat_net2pin_map), T(d.flat_net2pin_start_map),
+                               T(d.pin2net_map), T(d.net_weights), T(d.net_mask_ignore_large_degrees), gam)
+    wl, partial, gi = oracle.lse_merged_forward(pin, d.flat_net2pin_map, d.flat_net2pin_start_map,
+                                                d.net_weights, d.net_mask_ignore_large_degrees, d.wa_gamma)
+    assert abs(wl - wl_ref.item()) <= tol * abs(wl)
+    assert np.max(np.abs(gi - gi_ref.numpy())) <= tol * np.abs(gi).max()
+    g_ref = m.backward(torch.tensor(1.0, dtype=gam.dtype), T(pin), gi_ref.clone(), T(d.flat_net2pin_map),
+                       T(d.flat_net2pin_start_map), T(d.pin2net_map), T(d.net_weights),
+                       T(d.net_mask_ignore_large_degrees), gam)
+    P = d.num_pins
+    g_ref[:P].masked_fill_(T(d.pin_mask_ignore_fixed_macros), 0)
+    g_ref[P:].masked_fill_(T(d.pin_mask_ignore_fixed_macros), 0)
+    # the LSE backward is the WA backward's weight-and-mask step on its own derivative
+    g = oracle.wa_merged_backward(1.0, gi, d.pin2net_map, d.net_weights, d.net_mask_ignore_large_degrees,
+                                  d.pin_mask_ignore_fixed_macros)
+    assert np.max(np.abs(g - g_ref.numpy())) <= tol * np.abs(g).max()
+
+
+def test_lse_two_pin_closed_form(oracle):
+    """One two-pin net per row: WL = |dx| + |dy| + 2*gamma*(log(1+exp(-|dx|/gamma))
+    + log(1+exp(-|dy|/gamma))), d/dx1 = tanh((x1-x2)/(2 gamma))."""
+    rng = np.random.default_rng(3)
+    E = 50
+    pos = rng.uniform(-20, 20, 4 * E)            # pins 2e, 2e+1; x then y
+    P = 2 * E
+    starts = np.arange(0, P + 1, 2)
+    gamma = 3.0
+    wl, partial, gi = oracle.lse_merged_forward(pos, np.arange(P), starts, np.ones(E), np.ones(E), gamma)
+    dx = pos[0:P:2] - pos[1:P:2]
+    dy = pos[P::2] - pos[P + 1::2]
+    f = lambda d: np.abs(d) + 2 * gamma * np.log1p(np.exp(-np.abs(d) / gamma))
+    assert np.allclose(partial, f(dx) + f(dy), rtol=1e-13)
+    assert np.allclose(gi[0:P:2], np.tanh(dx / (2 * gamma)), atol=1e-14)
+    assert np.allclose(gi[P::2], np.tanh(dy / (2 * gamma)), atol=1e-14)
+    assert np.allclose(gi[0:P:2], -gi[1:P:2], atol=1e-15)
+
+
+def test_lse_limits_and_gradient(oracle):
+    """gamma -> 0 gives HPWL; HPWL <= LSE <= HPWL + 2*gamma*log(deg) per axis;
+    the derivative matches central finite differences."""
+    d = design(0)
+    pin = oracle.pin_pos_forward(d.pos, d.pin_offset_x, d.pin_offset_y, d.pin2node_map)
+    ones = np.ones(d.num_nets, dtype=np.uint8)
+    h = oracle.hpwl(pin, d.flat_net2pin_map, d.flat_net2pin_start_map, d.net_weights, ones)
+    wl0, _, _ = oracle.lse_merged_forward(pin, d.flat_net2pin_map, d.flat_net2pin_start_map, d.net_weights,
+                                          ones, 1e-6)
+    assert wl0 == pytest.approx(h, rel=1e-9)
+    gamma = 2.0
+    wl, partial, gi = oracle.lse_merged_forward(pin, d.flat_net2pin_map, d.flat_net2pin_start_map,
+                                                d.net_weights, ones, gamma)
+    deg = np.diff(d.flat_net2pin_start_map)
+    assert h <= wl <= h + float(np.sum(d.net_weights * 4 * gamma * np.log(deg))) + 1e-6
+    rng = np.random.default_rng(4)
+    idx = rng.choice(2 * d.num_pins, 40, replace=False)
+    hh = 1e-4
+    for i in idx:
+        e = d.pin2net_map[i % d.num_pins]   # difference the owning net's own wirelength
+        a, b = pin.copy(), pin.copy()
+        a[i] += hh
+        b[i] -= hh
+        fa = oracle.lse_merged_forward(a, d.flat_net2pin_map, d.flat_net2pin_start_map, np.ones(d.num_nets),
+                                       ones, gamma)[1][e]
+        fb = oracle.lse_merged_forward(b, d.flat_net2pin_map, d.flat_net2pin_start_map, np.ones(d.num_nets),
+                                       ones, gamma)[1][e]
+        assert (fa - fb) / (2 * hh) == pytest.approx(gi[i], abs=1e-7)
