@@ -72,6 +72,7 @@ _SIGNATURES = {
     "dp_lse_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
                                  _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
+    "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),

@@ -24,6 +24,14 @@
 // are written (in runs) to tile order and the per-cell gradients are written
 // back to node order, one 8/16-byte store per cell in the interleaved layout.
 //
+// Uniform fillers (all of one size, TriArgs::uni) form a class of their own:
+// the count and fill passes give them their own workgroups and their own
+// record run per tile (after the other cells' run), and their records hold
+// only the stretched origin and the node index.  Work items are per (tile,
+// class), so a scatter or gather work item of fillers loads 8 (scatter) or
+// 12 (gather) bytes per FP32 record instead of 20 or 24 and takes the size
+// and ratio from the uniform constants.
+//
 // Cells spanning more than 3x3 bins, or no bin at all, get their records at
 // the tail of the record arrays (slot total-1-q, q from a global counter) and
 // are done one workgroup (scatter) or one wavefront (gather) per cell.  Tile
@@ -64,7 +72,51 @@ struct Recs {
     r[j] = r_;
     idx[j] = i;
   }
+  // a uniform filler: size and ratio are the uniform constants
+  __device__ __forceinline__ void put_xy(int64_t j, T x_, T y_, int i) const {
+    x[j] = x_;
+    y[j] = y_;
+    idx[j] = i;
+  }
 };
+
+// Which cells of the (movable, filler) index space a count/fill workgroup
+// takes: class 0 (all cells, or all but uniform fillers) in groups
+// [0, groups0), class 1 (uniform fillers) in the rest.
+struct ClassSplit {
+  int64_t n0, total, chunk0, chunk1;
+  int groups0;
+};
+
+__device__ __forceinline__ void chunk_of(const ClassSplit& cs, int wg, int64_t& b, int64_t& e,
+                                         int& cls) {
+  if (wg < cs.groups0) {
+    b = (int64_t)wg * cs.chunk0;
+    e = b + cs.chunk0 < cs.n0 ? b + cs.chunk0 : cs.n0;
+    cls = 0;
+  } else {
+    b = cs.n0 + (int64_t)(wg - cs.groups0) * cs.chunk1;
+    e = b + cs.chunk1 < cs.total ? b + cs.chunk1 : cs.total;
+    cls = 1;
+  }
+}
+
+static ClassSplit class_split(int64_t n0, int64_t total, int groups) {
+  ClassSplit cs;
+  cs.n0 = n0;
+  cs.total = total;
+  const int64_t n1 = total - n0;
+  int g0 = groups;
+  if (n1 > 0) {
+    g0 = n0 == 0 ? 0 : (int)((double)groups * (double)n0 / (double)total + 0.5);
+    if (n0 > 0 && g0 < 1) g0 = 1;
+    if (g0 > groups - 1) g0 = groups - 1;
+  }
+  cs.groups0 = g0;
+  cs.chunk0 = g0 > 0 ? ceil_div(n0, g0) : 0;
+  cs.chunk1 = groups - g0 > 0 ? ceil_div(n1, groups - g0) : 0;
+  return cs;
+}
 
 struct TiledLayout {
   int ntx, nty, ntiles;
@@ -77,13 +129,13 @@ static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) 
   L.ntx = (int)ceil_div(nbx, kTile);
   L.nty = (int)ceil_div(nby, kTile);
   L.ntiles = L.ntx * L.nty;
-  L.max_work = ceil_div(ncells, kChunk) + L.ntiles + 1;
+  L.max_work = ceil_div(ncells, kChunk) + 2 * (int64_t)L.ntiles + 1;
   const size_t n = (size_t)(ncells > 0 ? ncells : 1);
   size_t o = 0;
-  L.off_count = o;
-  o += align256(sizeof(int) * (size_t)L.ntiles);
-  L.off_start = o;
-  o += align256(sizeof(int) * (size_t)(L.ntiles + 1));
+  L.off_count = o;  // cells per (class, tile)
+  o += align256(sizeof(int) * 2 * (size_t)L.ntiles);
+  L.off_start = o;  // first record of (class, tile)
+  o += align256(sizeof(int) * 2 * (size_t)L.ntiles);
   L.off_wgbase = o;
   o += align256(sizeof(int) * (size_t)kBinGroups * L.ntiles);
   L.off_work = o;
@@ -135,14 +187,15 @@ __device__ __forceinline__ int tile_key(const CellBins<T>& c, int nty) {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, int64_t fill_begin,
-                                                   int64_t total, int64_t chunk, int nty, int ntiles,
+                                                   int64_t total, ClassSplit cs, int nty, int ntiles,
                                                    int* tile_count, int* wg_base, int* ntail,
                                                    Recs<T> rec) {
   __shared__ int hist[kMaxTiles];
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * chunk;
-  const int64_t e = b + chunk < total ? b + chunk : total;
+  int64_t b, e;
+  int cls;
+  chunk_of(cs, blockIdx.x, b, e, cls);
   constexpr int U = 4;  // cells per thread per pass, loaded before any is used
   for (int64_t t0 = b + threadIdx.x; t0 < e; t0 += U * blockDim.x) {
     RawCell<T> raw[U];
@@ -168,14 +221,17 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
   }
   __syncthreads();
   int* my = wg_base + (size_t)blockIdx.x * ntiles;
+  int* cnt = tile_count + (size_t)cls * ntiles;
   for (int k = threadIdx.x; k < ntiles; k += blockDim.x) {
     const int h = hist[k];
-    my[k] = h ? atomicAdd(&tile_count[k], h) : 0;
+    my[k] = h ? atomicAdd(&cnt[k], h) : 0;
   }
 }
 
-// One workgroup of 1024 threads: tile_start = exclusive scan of tile_count;
-// work items (tile, begin, end) of at most kChunk cells per tile.
+// One workgroup of 1024 threads: a tile's records are its class-0 run, then
+// its class-1 run; tile_start[c * ntiles + k] = first record of (class c,
+// tile k) (exclusive scan over tiles of both counts); work items (tile,
+// begin, end, class) of at most kChunk cells.
 __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int ntiles,
                                                    int* tile_start, int4* work, int* nwork) {
   __shared__ int s_cells[1024];
@@ -184,9 +240,9 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
   const int k0 = threadIdx.x * per;
   int cells = 0, items = 0;
   for (int k = k0; k < k0 + per && k < ntiles; ++k) {
-    const int c = tile_count[k];
-    cells += c;
-    items += (c + kChunk - 1) / kChunk;
+    const int c0 = tile_count[k], c1 = tile_count[ntiles + k];
+    cells += c0 + c1;
+    items += (c0 + kChunk - 1) / kChunk + (c1 + kChunk - 1) / kChunk;
   }
   s_cells[threadIdx.x] = cells;
   s_items[threadIdx.x] = items;
@@ -203,18 +259,17 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
   int cbase = s_cells[threadIdx.x] - cells;
   int ibase = s_items[threadIdx.x] - items;
   for (int k = k0; k < k0 + per && k < ntiles; ++k) {
-    const int c = tile_count[k];
-    tile_start[k] = cbase;
-    for (int j = 0; j < c; j += kChunk) {
-      const int end = j + kChunk < c ? j + kChunk : c;
-      work[ibase++] = make_int4(k, cbase + j, cbase + end, 0);
+    for (int cls = 0; cls < 2; ++cls) {
+      const int c = tile_count[cls * ntiles + k];
+      tile_start[cls * ntiles + k] = cbase;
+      for (int j = 0; j < c; j += kChunk) {
+        const int end = j + kChunk < c ? j + kChunk : c;
+        work[ibase++] = make_int4(k, cbase + j, cbase + end, cls);
+      }
+      cbase += c;
     }
-    cbase += c;
   }
-  if (threadIdx.x == 1023) {
-    tile_start[ntiles] = s_cells[1023];
-    *nwork = s_items[1023];
-  }
+  if (threadIdx.x == 1023) *nwork = s_items[1023];
 }
 
 // Exclusive scan of in[0, n) into out[0, n) by a workgroup of kFillThreads
@@ -255,7 +310,7 @@ __device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, in
 template <typename T>
 __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t n_mov,
                                                            int64_t fill_begin, int64_t total,
-                                                           int64_t chunk, int nty, int ntiles,
+                                                           ClassSplit cs, int nty, int ntiles,
                                                            const int* tile_start,
                                                            const int* wg_base, Recs<T> rec) {
   constexpr int S = FillSub<T>::value;
@@ -267,10 +322,12 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
   __shared__ T s_x[S], s_y[S], s_sx[S], s_sy[S], s_r[S];
   __shared__ int s_idx[S];
   __shared__ unsigned short s_key[S];
+  int64_t b, e;
+  int cls;
+  chunk_of(cs, blockIdx.x, b, e, cls);
   const int* my = wg_base + (size_t)blockIdx.x * ntiles;
-  for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] = tile_start[k] + my[k];
-  const int64_t b = (int64_t)blockIdx.x * chunk;
-  const int64_t e = b + chunk < total ? b + chunk : total;
+  const int* start = tile_start + (size_t)cls * ntiles;
+  for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] = start[k] + my[k];
   // the next sub-batch's cells are loaded while this one is sorted in LDS
   RawCell<T> nxt[PER];
 #pragma unroll
@@ -323,7 +380,10 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
     for (int p = threadIdx.x; p < nsmall; p += kFillThreads) {
       const int k = s_key[p];
       const int64_t g = (int64_t)s_base[k] + (p - s_off[k]);
-      rec.put(g, s_x[p], s_y[p], s_sx[p], s_sy[p], s_r[p], s_idx[p]);
+      if (cls)
+        rec.put_xy(g, s_x[p], s_y[p], s_idx[p]);
+      else
+        rec.put(g, s_x[p], s_y[p], s_sx[p], s_sy[p], s_r[p], s_idx[p]);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] += s_hist[k];
@@ -354,9 +414,11 @@ struct RecBatch {
   bool ok[kBatch];
 };
 
+// uniform: a work item of uniform fillers (origin and index only stored).
 template <typename T>
-__device__ __forceinline__ void load_batch(const Recs<T>& rec, int j0, int stride, int end,
-                                           bool with_idx, RecBatch<T>& b) {
+__device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& rec, int j0,
+                                           int stride, int end, bool with_idx, bool uniform,
+                                           RecBatch<T>& b) {
 #pragma unroll
   for (int u = 0; u < kBatch; ++u) {
     const int j = j0 + u * stride;
@@ -364,9 +426,15 @@ __device__ __forceinline__ void load_batch(const Recs<T>& rec, int j0, int strid
     if (b.ok[u]) {
       b.x[u] = rec.x[j];
       b.y[u] = rec.y[j];
-      b.sx[u] = rec.sx[j];
-      b.sy[u] = rec.sy[j];
-      b.r[u] = rec.r[j];
+      if (uniform) {
+        b.sx[u] = a.uwx;
+        b.sy[u] = a.uwy;
+        b.r[u] = a.ur;
+      } else {
+        b.sx[u] = rec.sx[j];
+        b.sy[u] = rec.sy[j];
+        b.r[u] = rec.r[j];
+      }
       if (with_idx) b.idx[u] = rec.idx[j];
     }
   }
@@ -385,7 +453,7 @@ __global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, con
     __syncthreads();
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
-      load_batch(rec, j0, blockDim.x, it.z, false, b);
+      load_batch(a, rec, j0, blockDim.x, it.z, false, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (!b.ok[u]) continue;
@@ -457,7 +525,7 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriArgs<T> a, int nty, cons
     __syncthreads();
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
-      load_batch(rec, j0, blockDim.x, it.z, true, b);
+      load_batch(a, rec, j0, blockDim.x, it.z, true, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (!b.ok[u]) continue;
@@ -571,21 +639,22 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
   int* nwork = (int*)(tiled_ws + L.off_nwork);
   int* ntail = nwork + 1;
   Recs<T> rec = recs_of<T>(tiled_ws, L);
-  hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(L.ntiles, 256)), dim3(256), 0, s, tile_count,
-                     (int64_t)L.ntiles, 0);
+  hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(2 * L.ntiles, 256)), dim3(256), 0, s, tile_count,
+                     2 * (int64_t)L.ntiles, 0);
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, nwork, (int64_t)2, 0);
   DP_LAUNCHED("tiled init");
   if (total > 0) {
-    const int64_t chunk = ceil_div(total, kBinGroups);
+    // uniform fillers are class 1 (their own workgroups and record runs)
+    const ClassSplit cs = class_split(a.uni ? n_mov : total, total, kBinGroups);
     hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256), 0, s, a, n_mov, fill_begin,
-                       total, chunk, L.nty, L.ntiles, tile_count, wg_base, ntail, rec);
+                       total, cs, L.nty, L.ntiles, tile_count, wg_base, ntail, rec);
     DP_LAUNCHED("k_bin_count");
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, (const int*)tile_count, L.ntiles,
                        tile_start, work, nwork);
     DP_LAUNCHED("k_bin_scan");
     DP_PROBED(DP_KERNEL_DENSITY_BIN, s,
               hipLaunchKernelGGL(k_bin_fill<T>, dim3(kBinGroups), dim3(kFillThreads), 0, s, a,
-                                 n_mov, fill_begin, total, chunk, L.nty, L.ntiles,
+                                 n_mov, fill_begin, total, cs, L.nty, L.ntiles,
                                  (const int*)tile_start, (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,

@@ -43,14 +43,20 @@ constexpr int kPackedGrid = 8192;   // cap of workgroups of the packed kernel
 constexpr int kBigGrid = 1024;      // cap of workgroups of the big-net kernel (4 wavefronts each)
 constexpr int kMaskBit = (int)0x80000000u;
 
+// Per-pin metadata of the packed kernel, 12 bytes.
+struct PinMeta {
+  int x, y, z;
+};
+
 // ---- static preparation ----------------------------------------------------
 // group-order slot s of pin r of net e (s = net_slot[e] + r):
-//   meta[s] = {e | (pin masked ? sign bit : 0), node-order slot, r | deg << 8, 0}
-// and inv[pin] = s.  (.y is filled by k_prep_node.)
+//   meta[s] = {e | (pin masked ? sign bit : 0), node-order slot, r | deg << 8}
+// (12 bytes; deg only for nets of <= 64 pins) and inv[pin] = s.  (.y is
+// filled by k_prep_node.)
 __global__ void k_prep_net(const int* __restrict__ flat_netpin,
                            const int* __restrict__ netpin_start, const int* __restrict__ pin2net,
                            const uint8_t* __restrict__ pin_mask, const int* __restrict__ net_slot,
-                           int64_t num_pins, int4* meta, int* inv) {
+                           int64_t num_pins, PinMeta* meta, int* inv) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < num_pins;
        j += (int64_t)gridDim.x * blockDim.x) {
     const int p = flat_netpin[j];
@@ -58,11 +64,10 @@ __global__ void k_prep_net(const int* __restrict__ flat_netpin,
     const int b = netpin_start[e];
     const int r = (int)j - b, deg = netpin_start[e + 1] - b;
     const int s = net_slot[e] + r;
-    int4 m;
+    PinMeta m;
     m.x = e | (pin_mask && pin_mask[p] ? kMaskBit : 0);
     m.y = 0;
     m.z = deg <= 64 ? (r | (deg << 8)) : 0;
-    m.w = 0;
     meta[s] = m;
     inv[p] = s;
   }
@@ -73,7 +78,7 @@ __global__ void k_prep_net(const int* __restrict__ flat_netpin,
 template <typename T>
 __global__ void k_prep_node(const int* __restrict__ flat_node2pin, const int* __restrict__ inv,
                             const T* __restrict__ offx, const T* __restrict__ offy,
-                            int64_t num_pins, int* dst, int4* meta,
+                            int64_t num_pins, int* dst, PinMeta* meta,
                             typename Vec2<T>::type* off) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < num_pins;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -137,7 +142,7 @@ __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
 // same skeleton; g_p then points at gamma instead of 1/gamma.
 template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_pins(
-    const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
+    const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
     const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial,
     T* gnode, double* partials) {
@@ -163,7 +168,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     bool live = false, pin_masked = false;
     T x = 0, y = 0;
     if (on) {
-      const int4 m = meta[j];
+      const PinMeta m = meta[j];
       e = m.x & ~kMaskBit;
       pin_masked = m.x < 0;
       dst = m.y;
@@ -287,7 +292,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
 // group-order slot, degree, 0}.
 template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_big(
-    const typename Vec2<T>::type* __restrict__ pin_xy, const int4* __restrict__ meta,
+    const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int4* __restrict__ big, int nbig, const T* __restrict__ w,
     const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial, T* gnode,
     double* partials) {
@@ -351,7 +356,7 @@ __global__ void __launch_bounds__(256) k_wa_big(
     const T ay = kLse ? (T)(1.0 / fs) : (T)((1.0 - by * ys) / fs);
     const T any = kLse ? (T)(1.0 / fns) : (T)((1.0 - bny * yns) / fns);
     for (int j = pb + lane; j < pe; j += 64) {
-      const int4 m = meta[j];
+      const PinMeta m = meta[j];
       T gx = 0, gy = 0;
       if (live && m.x >= 0) {
         const typename Vec2<T>::type v = pin_xy[j];
@@ -378,11 +383,14 @@ __global__ void __launch_bounds__(256) k_wa_big(
 // ---- backward --------------------------------------------------------------
 // grad[n] = (*grad_scale) * sum of gnode over node n's CSR range + add[n]
 // (add nullable, split or interleaved).  Fillers (n >= nphys) get add[n] only.
-template <typename T>
+// kPre: then divided by max(pws[n] + (alpha * *dw) * area[n], 1), the
+// preconditioner of PlaceObj.py:69-102 applied to the finished gradient.
+template <typename T, bool kPre>
 __global__ void k_node_grad(const typename Vec2<T>::type* __restrict__ gnode,
                             const int* __restrict__ node2pin_start, const T* __restrict__ grad,
                             const T* __restrict__ add, int add_interleaved, int64_t nphys,
-                            int64_t nnodes, T* gx_out, T* gy_out) {
+                            int64_t nnodes, T* gx_out, T* gy_out, const T* __restrict__ pws,
+                            const T* __restrict__ area, const T* __restrict__ dw, T alpha) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
   T sx = 0, sy = 0;
@@ -403,6 +411,13 @@ __global__ void k_node_grad(const typename Vec2<T>::type* __restrict__ gnode,
       sx += add[n];
       sy += add[nnodes + n];
     }
+  }
+  if (kPre) {
+    const T c = alpha * *dw;
+    T p = pws[n] + c * area[n];
+    p = p < T(1) ? T(1) : p;
+    sx = sx / p;
+    sy = sy / p;
   }
   gx_out[n] = sx;
   gy_out[n] = sy;
@@ -427,16 +442,10 @@ int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* grou
   DP_ARG(num_nets >= 0 && num_nets < ((int64_t)1 << 31), "dp_wa_pack_nets: bad num_nets");
   // nets by degree (1..64), each bucket in netlist order
   std::vector<int64_t> count(65, 0);
-  int64_t nbig = 0, big_pins = 0;
   for (int64_t e = 0; e < num_nets; ++e) {
     const int64_t d = (int64_t)netpin_start[e + 1] - netpin_start[e];
     DP_ARG(d >= 0, "dp_wa_pack_nets: netpin_start not monotone at net %lld", (long long)e);
-    if (d > 64) {
-      ++nbig;
-      big_pins += d;
-    } else {
-      ++count[d];
-    }
+    if (d <= 64) ++count[d];
   }
   std::vector<int64_t> head(66, 0);
   for (int d = 1; d <= 64; ++d) head[d + 1] = head[d] + count[d];
@@ -515,7 +524,7 @@ int dp_wa_pins_prepare(int dtype, int64_t num_pins, int64_t num_physical_nodes,
   int* inv = (int*)workspace;
   hipLaunchKernelGGL(k_prep_net, dim3(grid_for(num_pins, 256)), dim3(256), 0, s,
                      (const int*)flat_netpin, (const int*)netpin_start, (const int*)pin2net_map,
-                     pin_mask, (const int*)net_slot, num_pins, (int4*)pin_meta, inv);
+                     pin_mask, (const int*)net_slot, num_pins, (PinMeta*)pin_meta, inv);
   DP_LAUNCHED("k_prep_net");
   hipLaunchKernelGGL(k_prep_slot_node, dim3(grid_for(num_physical_nodes, 256)), dim3(256), 0, s,
                      (const int*)flat_node2pin_start_map, num_physical_nodes, (int*)slot_node);
@@ -523,7 +532,7 @@ int dp_wa_pins_prepare(int dtype, int64_t num_pins, int64_t num_physical_nodes,
   return DP_DISPATCH(dtype, "dp_wa_pins_prepare", [&] {
     hipLaunchKernelGGL(k_prep_node<T>, dim3(grid_for(num_pins, 256)), dim3(256), 0, s,
                        (const int*)flat_node2pin_map, (const int*)inv, (const T*)pin_offset_x,
-                       (const T*)pin_offset_y, num_pins, (int*)node_dst, (int4*)pin_meta,
+                       (const T*)pin_offset_y, num_pins, (int*)node_dst, (PinMeta*)pin_meta,
                        (typename Vec2<T>::type*)node_offsets);
     DP_LAUNCHED("k_prep_node");
     return DP_OK;
@@ -575,12 +584,12 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
     auto bignet = lse ? k_wa_big<T, true> : k_wa_big<T, false>;
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
               hipLaunchKernelGGL(packed, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
-                                 (const int4*)pin_meta, (const int*)group_start, (int)num_groups,
+                                 (const PinMeta*)pin_meta, (const int*)group_start, (int)num_groups,
                                  (const T*)net_weights, net_mask, (const T*)g_p,
                                  (T*)partial_wl, (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
     hipLaunchKernelGGL(bignet, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
-                       (const int4*)pin_meta, (const int4*)big_nets, (int)num_big,
+                       (const PinMeta*)pin_meta, (const int4*)big_nets, (int)num_big,
                        (const T*)net_weights, net_mask, (const T*)g_p, (T*)partial_wl,
                        (T*)grad_node_order, partials + g1);
     DP_LAUNCHED("k_wa_big");
@@ -614,30 +623,55 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                       workspace, stream);
 }
 
+static int node_grad(const char* name, int dtype, const void* grad, const void* grad_node_order,
+                     const int32_t* flat_node2pin_start_map, const void* add, int add_layout,
+                     int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
+                     const void* pws, const void* area, const void* dw, double alpha,
+                     void* stream) {
+  DP_ARG(grad && grad_node_order && flat_node2pin_start_map && grad_pos_out,
+         "%s: null pointer", name);
+  DP_ARG(num_physical_nodes <= num_nodes, "%s: bad node counts", name);
+  DP_ARG(add_layout == DP_LAYOUT_SPLIT || add_layout == DP_LAYOUT_INTERLEAVED,
+         "%s: bad add_layout %d", name, add_layout);
+  if (num_nodes == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  DP_PROBE_OP(DP_OP_NODE_GRAD, s);
+  return DP_DISPATCH(dtype, name, [&] {
+    typedef typename Vec2<T>::type V;
+    T* out = (T*)grad_pos_out;
+    auto kern = pws ? k_node_grad<T, true> : k_node_grad<T, false>;
+    DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
+              hipLaunchKernelGGL(kern, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0, s,
+                                 (const V*)grad_node_order, (const int*)flat_node2pin_start_map,
+                                 (const T*)grad, (const T*)add,
+                                 add_layout == DP_LAYOUT_INTERLEAVED, num_physical_nodes,
+                                 num_nodes, out, out + num_nodes, (const T*)pws, (const T*)area,
+                                 (const T*)dw, (T)alpha));
+    DP_LAUNCHED("k_node_grad");
+    return DP_OK;
+  });
+}
+
 int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,
                            const int32_t* flat_node2pin_start_map, const void* add,
                            int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                            void* grad_pos_out, void* stream) {
-  DP_ARG(grad && grad_node_order && flat_node2pin_start_map && grad_pos_out,
-         "dp_node_grad_from_pins: null pointer");
-  DP_ARG(num_physical_nodes <= num_nodes, "dp_node_grad_from_pins: bad node counts");
-  DP_ARG(add_layout == DP_LAYOUT_SPLIT || add_layout == DP_LAYOUT_INTERLEAVED,
-         "dp_node_grad_from_pins: bad add_layout %d", add_layout);
-  if (num_nodes == 0) return DP_OK;
-  hipStream_t s = S(stream);
-  DP_PROBE_OP(DP_OP_NODE_GRAD, s);
-  return DP_DISPATCH(dtype, "dp_node_grad_from_pins", [&] {
-    typedef typename Vec2<T>::type V;
-    T* out = (T*)grad_pos_out;
-    DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
-              hipLaunchKernelGGL(k_node_grad<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
-                                 dim3(256), 0, s, (const V*)grad_node_order,
-                                 (const int*)flat_node2pin_start_map, (const T*)grad,
-                                 (const T*)add, add_layout == DP_LAYOUT_INTERLEAVED,
-                                 num_physical_nodes, num_nodes, out, out + num_nodes));
-    DP_LAUNCHED("k_node_grad");
-    return DP_OK;
-  });
+  return node_grad("dp_node_grad_from_pins", dtype, grad, grad_node_order, flat_node2pin_start_map,
+                   add, add_layout, num_physical_nodes, num_nodes, grad_pos_out, nullptr, nullptr,
+                   nullptr, 0.0, stream);
+}
+
+int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
+                                const int32_t* flat_node2pin_start_map, const void* add,
+                                int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
+                                const void* pin_weight_sum, const void* node_areas,
+                                const void* density_weight, double alpha, void* grad_pos_out,
+                                void* stream) {
+  DP_ARG(pin_weight_sum && node_areas && density_weight,
+         "dp_node_grad_preconditioned: null pointer");
+  return node_grad("dp_node_grad_preconditioned", dtype, grad, grad_node_order,
+                   flat_node2pin_start_map, add, add_layout, num_physical_nodes, num_nodes,
+                   grad_pos_out, pin_weight_sum, node_areas, density_weight, alpha, stream);
 }
 
 }  // extern "C"

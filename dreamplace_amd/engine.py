@@ -121,7 +121,7 @@ class PinPlan:
         self.node_offsets = torch.empty(2 * P, dtype=dd.dtype, device=dev)
         self.node_dst = torch.empty(P, dtype=torch.int32, device=dev)
         self.slot_node = torch.empty(P, dtype=torch.int32, device=dev)
-        self.pin_meta = torch.empty(4 * P, dtype=torch.int32, device=dev)
+        self.pin_meta = torch.empty(3 * P, dtype=torch.int32, device=dev)
         prep_ws = torch.empty(N.query("dp_wa_prepare_workspace_bytes", P), dtype=torch.uint8,
                               device=dev)
         pin_mask = dd.pin_mask_ignore_fixed_macros
@@ -278,8 +278,11 @@ class PlacementObjective:
                self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
                N.ptr(self._density_grad), N.stream_ptr(pos.device))
 
-    def evaluate(self, pos):
-        """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers."""
+    def evaluate(self, pos, precondition=None):
+        """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers.
+        precondition = (pin_weight_sum, node_areas, alpha) also applies
+        PlaceObj's preconditioner to the gradient (with the current density
+        weight), fused into the node-gradient pass on the pins schedule."""
         dd = self.dd
         code = N.dtype_code(pos)
         s = N.stream_ptr(pos.device)
@@ -295,9 +298,17 @@ class PlacementObjective:
                    N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)
             self._density_forward(pos)
             self._density_backward(pos)
-            N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
-                   N.ptr(dd.flat_node2pin_start_map), N.ptr(self._density_grad),
-                   self.grad_layout, dd.num_physical_nodes, n, N.ptr(self.grad), s)
+            if precondition is not None:
+                pws, areas, alpha = precondition
+                N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),
+                       N.ptr(pp.grad_node_order), N.ptr(dd.flat_node2pin_start_map),
+                       N.ptr(self._density_grad), self.grad_layout, dd.num_physical_nodes, n,
+                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(self.grad), s)
+                precondition = None  # done
+            else:
+                N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
+                       N.ptr(dd.flat_node2pin_start_map), N.ptr(self._density_grad),
+                       self.grad_layout, dd.num_physical_nodes, n, N.ptr(self.grad), s)
         elif self.schedule == "fused":
             N.call("dp_wa_forward_from_nodes", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), P, N.ptr(dd.flat_net2pin_map),
@@ -328,6 +339,10 @@ class PlacementObjective:
                    N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    dd.num_physical_nodes, n, N.ptr(self.wl_grad), s)
             torch.add(self.wl_grad, self._density_grad, out=self.grad)
+        if precondition is not None:
+            pws, areas, alpha = precondition
+            N.call("dp_precondition", code, N.ptr(self.grad), n, N.ptr(pws), N.ptr(areas),
+                   N.ptr(self.lam), float(alpha), s)
         energy = self._energy(pos)
         if self.quad_coeff is not None:
             energy = energy * (1 + self.quad_coeff * energy)

@@ -110,15 +110,25 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
             if done:
                 break
         self.backtracks.append(cnt)
-        v_k_1.copy_(v_k.data)
-        g_k_1.copy_(g_k)
         group["obj_k_1"][0].copy_(group["obj_k"][0])
-        u_k.copy_(self._u_kp1)
-        v_k.data.copy_(v_kp1)
-        g_k.copy_(g_kp1)
         group["obj_k"][0].copy_(f_kp1)
         a_k.copy_(self._a_kp1)
+        # v_k_1 <- v_k, v_k <- v_kp1, g_k_1 <- g_k, g_k <- g_kp1, u_k <- u_kp1
+        # (lines 151-162) by rotating storage: only the objective's reused
+        # gradient buffer is copied.  The parameter keeps its identity and
+        # takes v_kp1's storage.
+        self._rotate_positions(group, v_k, v_kp1)
+        group["g_k_1"][0], group["g_k"][0] = g_k, g_k_1
+        g_k_1.copy_(g_kp1)
+        group["u_k"][0], self._u_kp1 = self._u_kp1, u_k
         return loss
+
+    @staticmethod
+    def _rotate_positions(group, v_k, v_kp1):
+        old = v_k.data
+        v_k.data = v_kp1
+        group["v_kp1"][0] = group["v_k_1"][0]
+        group["v_k_1"][0] = old
 
     def step_bb(self, closure=None):
         """Lines 168-240: Barzilai-Borwein short step, else min(Lipschitz,
@@ -161,9 +171,8 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         self._extrapolate(v_k.data, g_k, u_k, step, a_k, v_kp1)
         self._constrain(v_kp1)
         group["obj_eval_count"] += 1
-        v_k_1.copy_(v_k.data)
         alpha_k.copy_(step)
-        u_k.copy_(self._u_kp1)
-        v_k.data.copy_(v_kp1)
         a_k.copy_(self._a_kp1)
+        self._rotate_positions(group, v_k, v_kp1)
+        group["u_k"][0], self._u_kp1 = self._u_kp1, u_k
         return loss

@@ -128,7 +128,7 @@ class GlobalPlacer:
         self.pin_pos = torch.empty(2 * dd.num_pins, dtype=dd.dtype, device=dev)
         self.hpwl_ws = torch.empty(N.query("dp_wa_workspace_bytes", dd.num_nets), dtype=torch.uint8,
                                    device=dev)
-        self.hpwl_out = torch.empty((), dtype=torch.float64, device=dev)
+        self.metric_buf = torch.empty(3, dtype=torch.float64, device=dev)
         self.opt_ws = torch.empty(N.query("dp_optimizer_workspace_bytes"), dtype=torch.uint8,
                                   device=dev)
         self.l1_out = torch.empty(2, dtype=torch.float64, device=dev)
@@ -161,16 +161,14 @@ class GlobalPlacer:
 
     def obj_and_grad(self, x):
         """PlaceObj.obj_and_grad_fn: objective, gradient, preconditioner."""
-        f, g = self.objective.evaluate(x)
+        pre = (self.pws, self.areas, self.p["precondition_alpha"])
+        f, g = self.objective.evaluate(x, precondition=pre)
         if self.init_density is None:
             # PlaceObj.obj_fn records the density of its first evaluation
             self.init_density = float(self.objective._energy(x).item())
             if self.quad_penalty:
                 self._penalty()
-                f, g = self.objective.evaluate(x)
-        N.call("dp_precondition", self.code, N.ptr(g), self.dd.num_nodes, N.ptr(self.pws),
-               N.ptr(self.areas), N.ptr(self.objective.lam), float(self.p["precondition_alpha"]),
-               N.stream_ptr(x.device))
+                f, g = self.objective.evaluate(x, precondition=pre)
         return f, g
 
     def initialize_density_weight(self, x):
@@ -198,14 +196,21 @@ class GlobalPlacer:
         N.call("dp_pin_pos_forward", self.code, N.ptr(x), dd.num_nodes, N.ptr(dd.pin_offset_x),
                N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, dd.num_pins,
                N.ptr(self.pin_pos), N.stream_ptr(dev))
+        buf = self.metric_buf  # {HPWL, overflow cost, max bin density}, FP64, one read-back
         N.call("dp_hpwl", self.code, N.ptr(self.pin_pos), dd.num_pins, N.ptr(dd.flat_net2pin_map),
                N.ptr(dd.flat_net2pin_start_map), dd.num_nets, N.ptr(dd.net_weights),
-               N.ptr(dd.net_mask_all), N.ptr(self.hpwl_ws), N.ptr(self.hpwl_out), N.stream_ptr(dev))
+               N.ptr(dd.net_mask_all), N.ptr(self.hpwl_ws), N.ptr(buf[0:1]), N.stream_ptr(dev))
         ep = self.objective.density
         m = ep.density_map(x, num_filler_nodes=0)
-        cost, mx = ep.overflow_from_map(m)
-        return dict(hpwl=float(self.hpwl_out.item()), overflow=float(cost.item()) / self.movable_area,
-                    max_density=float(mx.item()), density_weight=self.lam, gamma=self.gamma)
+        bin_area = ep.bin_size_x * ep.bin_size_y
+        N.call("dp_density_overflow", N.dtype_code(m), N.ptr(m), m.numel(),
+               float(ep.target_density) * bin_area, N.ptr(ep.reduce_workspace(dev)), N.ptr(buf[1:3]),
+               N.stream_ptr(dev))
+        h, cost, mx = buf.tolist()
+        # ElectricOverflow returns the cost and the peak in the map's dtype
+        T = np.float32 if m.dtype == torch.float32 else np.float64
+        return dict(hpwl=h, overflow=float(T(cost)) / self.movable_area,
+                    max_density=float(T(mx / bin_area)), density_weight=self.lam, gamma=self.gamma)
 
     def estimate_learning_rate(self, x):
         """PlaceObj.estimate_initial_learning_rate (PlaceObj.py:443-454)."""
@@ -251,14 +256,19 @@ class GlobalPlacer:
         overflow_list, divergence_list = [1], []
         last_perturb, min_perturb_interval = -50, 50
         stop = False
+        # move_boundary is idempotent: positions the optimizer left (its last
+        # v_kp1, already constrained) need no second pass
+        constrained = False
         for it in range(iterations):
             if (p["early_stop"] and p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
                     and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
                                          threshold=overflow_list[-1])):
                 x.copy_(self.best_pos)
+                constrained = False
                 stop = True
                 self.rolled_back = True
-            self.constraint(x)
+            if not constrained:
+                self.constraint(x)
             if self.lam == 0.0:
                 self.initialize_density_weight(x)
             cur = self.metrics(x)
@@ -268,6 +278,7 @@ class GlobalPlacer:
                       % (it, cur["hpwl"], cur["overflow"], cur["max_density"], cur["density_weight"],
                          cur["gamma"]), flush=True)
             self.optimizer.step()
+            constrained = True
             if self.best is None or self.best["overflow"] > cur["overflow"]:
                 self.best = cur
                 if self.best_pos is None:
@@ -284,6 +295,7 @@ class GlobalPlacer:
                     if overflow_list[-1] > 0.95:
                         intensity = min(max(40 + (120 - 40) * (overflow_list[-1] - 0.95) * 10, 40), 90)
                         self.entropy_injection(x, 0.996, intensity)
+                        constrained = False
                     last_perturb = it
                     self.perturbations += 1
             if len(hist) > 1:

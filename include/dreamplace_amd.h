@@ -307,9 +307,9 @@ int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* grou
  *                     (slot i of flat_node2pin_map), interleaved;
  *   node_dst[P]       the pin's slot in group order;
  *   slot_node[P]      the physical node owning node-order slot i;
- *   pin_meta[4P]      per group-order slot s: {net | sign bit when pin_mask
+ *   pin_meta[3P]      per group-order slot s: {net | sign bit when pin_mask
  *                     marks the pin (pin_mask nullable), node-order slot,
- *                     rank in the net | degree << 8 (nets of <= 64 pins), 0}.
+ *                     rank in the net | degree << 8 (nets of <= 64 pins)}.
  * flat_netpin and flat_node2pin_map must both be permutations of
  * [0, num_pins) and flat_node2pin_start_map must cover them over the physical
  * nodes, as PlaceDB builds them.  workspace:
@@ -361,6 +361,18 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
                            const int32_t* flat_node2pin_start_map, const void* add,
                            int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                            void* grad_pos_out, void* stream);
+
+/* dp_node_grad_from_pins followed by PreconditionOp.__call__ without fence
+ * regions (PlaceObj.py:69-102) in the same pass: each node's finished
+ * gradient is divided by max(pin_weight_sum[n] + (alpha * *density_weight) *
+ * node_areas[n], 1), as dp_precondition does; density_weight is a device
+ * scalar of `dtype`. */
+int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
+                                const int32_t* flat_node2pin_start_map, const void* add,
+                                int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
+                                const void* pin_weight_sum, const void* node_areas,
+                                const void* density_weight, double alpha, void* grad_pos_out,
+                                void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
  * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64

@@ -104,3 +104,29 @@ def test_entropy_injection_matches_oracle(gpu):
     gp.entropy_injection(x, 0.996, 47.5)
     OP.entropy_injection(gp.d, ref, 0.996, 47.5, OP.injection_noise(d.num_nodes, 0, 0))
     assert np.allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_fused_preconditioner_equals_separate(gpu, dtype):
+    """The preconditioner folded into the node-gradient pass gives the same
+    bits as the separate dp_precondition pass after the evaluation."""
+    from dreamplace_amd import _native as N
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case fused preconditioner %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype)
+    dd = DeviceDesign(d, gpu)
+    ob = PlacementObjective(dd, d.wa_gamma, density_weight=3e-5, precision="fp32", algorithm="tiled",
+                            schedule="pins")
+    n = dd.num_nodes
+    code = N.dtype_code(dd.pos)
+    pws = torch.empty(n, dtype=dd.dtype, device=gpu)
+    N.call("dp_pin_weight_sum", code, N.ptr(dd.net_weights), N.ptr(dd.flat_node2pin_map),
+           N.ptr(dd.flat_node2pin_start_map), N.ptr(dd.pin2net_map), dd.num_physical_nodes, n,
+           N.ptr(pws), N.stream_ptr(gpu))
+    areas = (dd.node_size_x * dd.node_size_y).contiguous()
+    _, g = ob.evaluate(dd.pos)
+    sep = g.clone()
+    N.call("dp_precondition", code, N.ptr(sep), n, N.ptr(pws), N.ptr(areas), N.ptr(ob.lam), 1.0,
+           N.stream_ptr(gpu))
+    _, g2 = ob.evaluate(dd.pos, precondition=(pws, areas, 1.0))
+    assert torch.equal(g2, sep)
