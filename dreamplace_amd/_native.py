@@ -68,9 +68,10 @@ _SIGNATURES = {
                                 _P, _P]),
     "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
     "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
-                                _P, _P, _P, _P, _P]),
+                                _P, _P, _P, _P, _P, _P]),
     "dp_lse_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
-                                 _P, _P, _P, _P, _P]),
+                                 _P, _P, _P, _P, _P, _P]),
+    "dp_wa_pins_slot_weights": (_I, [_I, _L, _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
     "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),

@@ -43,16 +43,15 @@ constexpr int kPackedGrid = 8192;   // cap of workgroups of the packed kernel
 constexpr int kBigGrid = 1024;      // cap of workgroups of the big-net kernel (4 wavefronts each)
 constexpr int kMaskBit = (int)0x80000000u;
 
-// Per-pin metadata of the packed kernel, 12 bytes.
-struct PinMeta {
-  int x, y, z;
-};
+// Per-pin metadata of the packed kernel.
+typedef int4 PinMeta;
 
 // ---- static preparation ----------------------------------------------------
 // group-order slot s of pin r of net e (s = net_slot[e] + r):
-//   meta[s] = {e | (pin masked ? sign bit : 0), node-order slot, r | deg << 8}
-// (12 bytes; deg only for nets of <= 64 pins) and inv[pin] = s.  (.y is
-// filled by k_prep_node.)
+//   meta[s] = {e | (pin masked ? sign bit : 0), node-order slot, r | deg << 8, 0}
+// (deg only for nets of <= 64 pins; 16 bytes: one aligned dwordx4 load per
+// lane, which measured faster here than a 12-byte record) and inv[pin] = s.
+// (.y is filled by k_prep_node.)
 __global__ void k_prep_net(const int* __restrict__ flat_netpin,
                            const int* __restrict__ netpin_start, const int* __restrict__ pin2net,
                            const uint8_t* __restrict__ pin_mask, const int* __restrict__ net_slot,
@@ -68,6 +67,7 @@ __global__ void k_prep_net(const int* __restrict__ flat_netpin,
     m.x = e | (pin_mask && pin_mask[p] ? kMaskBit : 0);
     m.y = 0;
     m.z = deg <= 64 ? (r | (deg << 8)) : 0;
+    m.w = 0;
     meta[s] = m;
     inv[p] = s;
   }
@@ -90,6 +90,20 @@ __global__ void k_prep_node(const int* __restrict__ flat_node2pin, const int* __
     v.x = offx[p];
     v.y = offy[p];
     off[i] = v;
+  }
+}
+
+// Per group-order slot s of net e: meta[s].w = net_mask[e] != 0 and
+// slot_w[s] = w[e] (1 without weights), for callers whose weights and mask
+// stay fixed between calls (dp_wa_pins_slot_weights).
+template <typename T>
+__global__ void k_slot_weights(int64_t num_pins, PinMeta* meta, const T* __restrict__ w,
+                               const uint8_t* __restrict__ net_mask, T* slot_w) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < num_pins;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int e = meta[j].x & ~kMaskBit;
+    meta[j].w = net_mask[e] != 0;
+    slot_w[j] = w ? w[e] : T(1);
   }
 }
 
@@ -144,8 +158,8 @@ template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_pins(
     const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
-    const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial,
-    T* gnode, double* partials) {
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ slot_w,
+    const T* __restrict__ g_p, T* partial, T* gnode, double* partials) {
   typedef typename Vec2<T>::type V;
   // per-lane words read by the net's first lane: one LDS read per pin for
   // (x, y) and one for its four exponentials
@@ -159,6 +173,9 @@ __global__ void __launch_bounds__(256) k_wa_pins(
   T(*s_c)[64] = sh_c[wv];
   const T gam = kLse ? *g_p : T(0);
   const T ig = kLse ? T(1) / gam : *g_p;
+  // weights (and the live flag) per pin slot when the caller prepared them
+  // (contiguous), else gathered per net
+  const bool weighted = slot_w != nullptr || w != nullptr;
   double contrib = 0;
   for (int g = blockIdx.x * kWaves + wv; g < ngroups; g += gridDim.x * kWaves) {
     const int pb = group_start[g], pe = group_start[g + 1];
@@ -166,7 +183,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     const bool on = j < pe;
     int e = 0, head = 0, deg = 0, dst = 0;
     bool live = false, pin_masked = false;
-    T x = 0, y = 0;
+    T x = 0, y = 0, wn = T(1);
     if (on) {
       const PinMeta m = meta[j];
       e = m.x & ~kMaskBit;
@@ -174,7 +191,13 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       dst = m.y;
       head = lane - (m.z & 0xff);
       deg = m.z >> 8;
-      live = net_mask[e] != 0;
+      if (slot_w) {
+        live = m.w != 0;
+        wn = slot_w[j];
+      } else {
+        live = net_mask[e] != 0;
+        if (w) wn = w[e];
+      }
       const typename Vec2<T>::type v = pin_xy[j];
       x = v.x;
       y = v.y;
@@ -228,7 +251,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       }
       const T wl = (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin;
       if (partial) partial[e] = wl;
-      contrib += (double)(w ? wl * w[e] : wl);
+      contrib += (double)(weighted ? wl * wn : wl);
       s_c[0][lane] = (T)(1.0 / es);
       s_c[1][lane] = (T)(1.0 / ens);
       s_c[2][lane] = (T)(1.0 / fs);
@@ -252,7 +275,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       }
       const T wl = xs / es - xns / ens + ys / fs - yns / fns;
       if (partial) partial[e] = wl;
-      contrib += (double)(w ? wl * w[e] : wl);
+      contrib += (double)(weighted ? wl * wn : wl);
       const T bx = ig / es, bnx = -ig / ens, by = ig / fs, bny = -ig / fns;
       s_c[0][lane] = (T)((1.0 - bx * xs) / es);
       s_c[1][lane] = bx;
@@ -274,8 +297,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
           gx = (s_c[0][head] + s_c[1][head] * x) * ex - (s_c[2][head] + s_c[3][head] * x) * enx;
           gy = (s_c[4][head] + s_c[5][head] * y) * ey - (s_c[6][head] + s_c[7][head] * y) * eny;
         }
-        if (w) {
-          const T wn = w[e];
+        if (weighted) {
           gx *= wn;
           gy *= wn;
         }
@@ -549,8 +571,9 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
-                       const uint8_t* net_mask, const void* g_p, void* partial_wl,
-                       void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+                       const uint8_t* net_mask, const void* slot_weights, const void* g_p,
+                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
+                       void* stream) {
   DP_ARG(pos && slot_node && node_offsets && node_dst && pin_meta &&
              (group_start || num_groups == 0) && (big_nets || num_big == 0) && net_mask &&
              g_p && grad_node_order && wl_out && workspace,
@@ -585,8 +608,8 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
               hipLaunchKernelGGL(packed, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
                                  (const PinMeta*)pin_meta, (const int*)group_start, (int)num_groups,
-                                 (const T*)net_weights, net_mask, (const T*)g_p,
-                                 (T*)partial_wl, (T*)grad_node_order, partials));
+                                 (const T*)net_weights, net_mask, (const T*)slot_weights,
+                                 (const T*)g_p, (T*)partial_wl, (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
     hipLaunchKernelGGL(bignet, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
                        (const PinMeta*)pin_meta, (const int4*)big_nets, (int)num_big,
@@ -602,12 +625,13 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
-                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
-                       void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+                       const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
+                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
+                       void* stream) {
   return forward_pins(false, "dp_wa_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
-                      num_nets, net_weights, net_mask, inv_gamma, partial_wl, grad_node_order,
-                      wl_out, workspace, stream);
+                      num_nets, net_weights, net_mask, slot_weights, inv_gamma, partial_wl,
+                      grad_node_order, wl_out, workspace, stream);
 }
 
 int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
@@ -615,12 +639,29 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                         const int32_t* node_dst, const int32_t* pin_meta,
                         const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                         int64_t num_big, int64_t num_nets, const void* net_weights,
-                        const uint8_t* net_mask, const void* gamma, void* partial_wl,
-                        void* grad_node_order, void* wl_out, void* workspace, void* stream) {
+                        const uint8_t* net_mask, const void* slot_weights, const void* gamma,
+                        void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
+                        void* stream) {
   return forward_pins(true, "dp_lse_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
-                      num_nets, net_weights, net_mask, gamma, partial_wl, grad_node_order, wl_out,
-                      workspace, stream);
+                      num_nets, net_weights, net_mask, slot_weights, gamma, partial_wl,
+                      grad_node_order, wl_out, workspace, stream);
+}
+
+int dp_wa_pins_slot_weights(int dtype, int64_t num_pins, int32_t* pin_meta,
+                            const void* net_weights, const uint8_t* net_mask, void* slot_weights,
+                            void* stream) {
+  DP_ARG(pin_meta && net_mask && slot_weights, "dp_wa_pins_slot_weights: null pointer");
+  DP_ARG(num_pins >= 0 && num_pins < ((int64_t)1 << 31), "dp_wa_pins_slot_weights: bad count");
+  if (num_pins == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  return DP_DISPATCH(dtype, "dp_wa_pins_slot_weights", [&] {
+    hipLaunchKernelGGL(k_slot_weights<T>, dim3(grid_for(num_pins, 256)), dim3(256), 0, s,
+                       num_pins, (PinMeta*)pin_meta, (const T*)net_weights, net_mask,
+                       (T*)slot_weights);
+    DP_LAUNCHED("k_slot_weights");
+    return DP_OK;
+  });
 }
 
 static int node_grad(const char* name, int dtype, const void* grad, const void* grad_node_order,

@@ -121,7 +121,7 @@ class PinPlan:
         self.node_offsets = torch.empty(2 * P, dtype=dd.dtype, device=dev)
         self.node_dst = torch.empty(P, dtype=torch.int32, device=dev)
         self.slot_node = torch.empty(P, dtype=torch.int32, device=dev)
-        self.pin_meta = torch.empty(3 * P, dtype=torch.int32, device=dev)
+        self.pin_meta = torch.empty(4 * P, dtype=torch.int32, device=dev)
         prep_ws = torch.empty(N.query("dp_wa_prepare_workspace_bytes", P), dtype=torch.uint8,
                               device=dev)
         pin_mask = dd.pin_mask_ignore_fixed_macros
@@ -201,6 +201,23 @@ class PlacementObjective:
         self.w = dd.net_weights.contiguous()
         self.net_mask = dd.net_mask_ignore_large_degrees
         self.pin_mask = dd.pin_mask_ignore_fixed_macros
+        self.slot_w = None
+        if self.pin_plan is not None:
+            self.slot_w = torch.empty(P, dtype=dt, device=dev)
+            self._slot_weights()
+
+    def _slot_weights(self):
+        """Net weights and mask per group-order pin slot (the pins schedule
+        reads them contiguously); redone whenever either changes."""
+        N.call("dp_wa_pins_slot_weights", N.dtype_code(self.slot_w), self.dd.num_pins,
+               N.ptr(self.pin_plan.pin_meta), N.ptr(self.w), N.ptr(self.net_mask),
+               N.ptr(self.slot_w), N.stream_ptr(self.slot_w.device))
+
+    def set_net_weights(self, w):
+        """New per-net weights (e.g. from timing-driven net weighting)."""
+        self.w = w.to(self.dd.dtype).contiguous()
+        if self.slot_w is not None:
+            self._slot_weights()
 
     @property
     def density_grad(self):
@@ -292,7 +309,7 @@ class PlacementObjective:
             N.call("dp_lse_forward_pins" if self.lse else "dp_wa_forward_pins", code, N.ptr(pos), n,
                    P, N.ptr(pp.slot_node), N.ptr(pp.node_offsets), N.ptr(pp.node_dst),
                    N.ptr(pp.pin_meta), N.ptr(pp.group_start), pp.num_groups, N.ptr(pp.big),
-                   pp.num_big, E, N.ptr(self.w), N.ptr(self.net_mask),
+                   pp.num_big, E, N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.slot_w),
                    N.ptr(self.gamma_t if self.lse else self.inv_gamma),
                    N.ptr(self.partial_wl) if self.keep_partial_wl else None,
                    N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)

@@ -307,9 +307,9 @@ int dp_wa_pack_nets(const int32_t* netpin_start, int64_t num_nets, int32_t* grou
  *                     (slot i of flat_node2pin_map), interleaved;
  *   node_dst[P]       the pin's slot in group order;
  *   slot_node[P]      the physical node owning node-order slot i;
- *   pin_meta[3P]      per group-order slot s: {net | sign bit when pin_mask
+ *   pin_meta[4P]      per group-order slot s: {net | sign bit when pin_mask
  *                     marks the pin (pin_mask nullable), node-order slot,
- *                     rank in the net | degree << 8 (nets of <= 64 pins)}.
+ *                     rank in the net | degree << 8 (nets of <= 64 pins), 0}.
  * flat_netpin and flat_node2pin_map must both be permutations of
  * [0, num_pins) and flat_node2pin_start_map must cover them over the physical
  * nodes, as PlaceDB builds them.  workspace:
@@ -337,8 +337,9 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
-                       const uint8_t* net_mask, const void* inv_gamma, void* partial_wl,
-                       void* grad_node_order, void* wl_out, void* workspace, void* stream);
+                       const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
+                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
+                       void* stream);
 
 /* The log-sum-exp model on the same plan and skeleton: the chain
  * LSE(PinPos(pos)) of logsumexp_wirelength_merged.cpp:14-85 (CUDA twin
@@ -350,8 +351,20 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                         const int32_t* node_dst, const int32_t* pin_meta,
                         const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                         int64_t num_big, int64_t num_nets, const void* net_weights,
-                        const uint8_t* net_mask, const void* gamma, void* partial_wl,
-                        void* grad_node_order, void* wl_out, void* workspace, void* stream);
+                        const uint8_t* net_mask, const void* slot_weights, const void* gamma,
+                        void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
+                        void* stream);
+
+/* Optional companion of the two entry points above, with no counterpart in
+ * the reference: for callers whose net weights and net mask stay fixed
+ * between calls, writes each group-order pin slot's net weight to
+ * slot_weights[num_pins] (`dtype`; 1 where net_weights is null) and its
+ * net's mask into pin_meta.  Passing slot_weights to dp_wa_forward_pins /
+ * dp_lse_forward_pins then replaces the per-net gathers of net_weights and
+ * net_mask by one contiguous read; run it again after either changes. */
+int dp_wa_pins_slot_weights(int dtype, int64_t num_pins, int32_t* pin_meta,
+                            const void* net_weights, const uint8_t* net_mask, void* slot_weights,
+                            void* stream);
 
 /* Backward of that chain plus the gradient combine: grad_pos[n] =
  * (*grad) * (sum of grad_node_order over node n's CSR range) + add[n]

@@ -131,3 +131,23 @@ def test_engine_pins_equals_unfused_fp64(gpu):
     f2, g2 = o2.evaluate(dd2.pos)
     assert rel_l2(g1.cpu().numpy(), g2.cpu().numpy()) <= 1e-14
     assert abs(f1.item() - f2.item()) <= 1e-13 * abs(f2.item())
+
+
+def test_engine_pins_net_weights(gpu):
+    """Non-unit net weights on the pins schedule (read per pin slot) equal the
+    reference-order schedule, also after the weights change."""
+    print("case engine pins net weights", flush=True)
+    d = design(0, big_nets=20)
+    rng = np.random.default_rng(8)
+    d.net_weights = np.exp(rng.uniform(np.log(0.5), np.log(4.0), d.num_nets))
+    dd1, o1 = _objective(d, gpu, 0.3, True, "fp64", algorithm="tiled", schedule="pins")
+    dd2, o2 = _objective(d, gpu, 0.3, False, "fp64")
+    for k in range(2):
+        if k:
+            w2 = torch.from_numpy(np.exp(rng.uniform(-1.0, 1.5, d.num_nets))).to(gpu)
+            o1.set_net_weights(w2)
+            o2.set_net_weights(w2)
+        f1, g1 = o1.evaluate(dd1.pos)
+        f2, g2 = o2.evaluate(dd2.pos)
+        assert rel_l2(g1.cpu().numpy(), g2.cpu().numpy()) <= 1e-14
+        assert abs(f1.item() - f2.item()) <= 1e-13 * abs(f2.item())
