@@ -51,6 +51,9 @@ constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
 constexpr int kFillThreads = 1024;        // threads of a fill workgroup
 constexpr int kChunk = 8192;              // cells per scatter/gather work item
 constexpr int kGatherSplit = 2;           // the gather does each work item in this many parts
+// threads of a scatter/gather workgroup: its (64+2)^2-bin LDS tile (35 KB)
+// allows 4 workgroups per CU, so 8 waves each keep 8 waves per SIMD in flight
+constexpr int kTileThreads = 512;
 
 // Cells per LDS-staged fill sub-batch: 28 (FP32) or 48 (FP64) bytes of LDS
 // per cell, ~115 KB (FP32) / ~100 KB (FP64) in all with the tile arrays.
@@ -441,7 +444,7 @@ __device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& r
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, const int4* work,
+__global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TriArgs<T> a, int nty, const int4* work,
                                                       const int* nwork, Recs<T> rec,
                                                       unsigned long long* map) {
   __shared__ unsigned long long acc[kTileH * kTileH];
@@ -495,7 +498,7 @@ __global__ void __launch_bounds__(256) k_tile_scatter(TriArgs<T> a, int nty, con
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_tile_gather(TriArgs<T> a, int nty, const int4* work,
+__global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int nty, const int4* work,
                                                      const int* nwork, Recs<T> rec,
                                                      const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
@@ -660,7 +663,7 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
                                  (const int*)tile_start, (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
-              hipLaunchKernelGGL(k_tile_scatter<T>, dim3(2048), dim3(256), 0, s, a, L.nty,
+              hipLaunchKernelGGL(k_tile_scatter<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty,
                                  (const int4*)work, (const int*)nwork, rec, map));
     DP_LAUNCHED("k_tile_scatter");
     hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, rec, total,
@@ -698,7 +701,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
   }
   if (total > 0) {
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
-              hipLaunchKernelGGL(k_tile_gather<T>, dim3(2048), dim3(256), 0, s, a, L.nty, work,
+              hipLaunchKernelGGL(k_tile_gather<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty, work,
                                  nwork, rec, fx, fy, grad, num_nodes, interleaved, out));
     DP_LAUNCHED("k_tile_gather");
     hipLaunchKernelGGL(k_rec_gather_large<T>, dim3(256), dim3(256), 0, s, a, rec, total, ntail,
