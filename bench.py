@@ -77,9 +77,11 @@ def op_bytes(d, schedule):
     return ops
 
 
-def poisson_flops(M, N):
-    """Dense-transform flops of one solve: 4 transforms of 2 GEMMs each."""
-    return 4 * 2 * (M * M * N + M * N * N)
+def poisson_flops(M, N, transforms=4):
+    """Dense-transform flops of one solve: `transforms` 2-D transforms of 2
+    GEMMs each (DCT2 + two field transforms, + the potential's IDCT2 when the
+    energy is taken from the potential map)."""
+    return transforms * 2 * (M * M * N + M * N * N)
 
 
 PROBES = {"wa_forward": DP_OP_WA_FORWARD, "density_map": DP_OP_DENSITY_MAP,
@@ -147,14 +149,20 @@ def accuracy(torch, obj32, obj64, dd64):
     at the same positions."""
     ep = obj64.density
     m = obj32.density_map.double()
-    fx64, fy64, phi64, _ = obj64.plan.poisson(m, ep.bin_size_x, ep.bin_size_y)
+    fx64, fy64, phi64, e64 = obj64.plan.poisson(m, ep.bin_size_x, ep.bin_size_y)
+    # the reduced form's potential map (not needed by its step when the energy
+    # is spectral), solved here outside the timed region for the gate
+    _, _, phi32, _ = obj32.plan.poisson(obj32.density_map.to(obj32.plan.dtype), ep.bin_size_x,
+                                        ep.bin_size_y, compute_energy=False)
     one = torch.ones((), dtype=torch.float64, device=m.device)
     pos = dd64.pos
     g64 = ep.electric_force(pos, fx64, fy64, one)
     g32 = ep.electric_force(pos, obj32.field_x.double(), obj32.field_y.double(), one)
     both = lambda a, b: torch.cat([a.reshape(-1), b.reshape(-1)])
     out = {
-        "l2err_solve_potential": rel(torch, obj32.potential, phi64),
+        "l2err_solve_potential": rel(torch, phi32, phi64),
+        "l2err_solve_energy": abs(float(obj32.energy.double().item()) - float(e64.item()))
+                              / abs(float(e64.item())),
         "l2err_solve_field": rel(torch, both(obj32.field_x, obj32.field_y), both(fx64, fy64)),
         "l2err_solve_density_grad": rel(torch, g32, g64),
         "l2err_step_field": rel(torch, both(obj32.field_x, obj32.field_y),
@@ -163,7 +171,7 @@ def accuracy(torch, obj32, obj64, dd64):
         "l2err_step_grad": rel(torch, obj32.grad, obj64.grad),
     }
     out["l2err_gate"] = max(out["l2err_solve_potential"], out["l2err_solve_field"],
-                            out["l2err_solve_density_grad"])
+                            out["l2err_solve_density_grad"], out["l2err_solve_energy"])
     return out
 
 
@@ -205,6 +213,9 @@ def main():
                     help="pins: pin-parallel WA; fused: net-parallel fused; unfused: reference op order")
     ap.add_argument("--algorithm", default="tiled", choices=["tiled", "global"],
                     help="density scatter / field gather: LDS-binned tiles or per-cell global atomics")
+    ap.add_argument("--energy-mode", default="spectral", choices=["spectral", "potential"],
+                    help="energy from the DCT coefficients (Parseval; no potential map) or "
+                         "from the materialised potential map as the reference does")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -241,7 +252,8 @@ def main():
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
-                               algorithm=args.algorithm, schedule=args.schedule)
+                               algorithm=args.algorithm, schedule=args.schedule,
+                               energy_mode=args.energy_mode)
     pos32 = dd32.pos
     ms32, per32 = time_steps(torch, obj32, pos32, args.steps, args.warmup)
     log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32,
@@ -258,7 +270,7 @@ def main():
     ops = {}
     for k in names:
         if k == "poisson":
-            fl = poisson_flops(d.num_bins_x, d.num_bins_y)
+            fl = poisson_flops(d.num_bins_x, d.num_bins_y, 3 if args.energy_mode == "spectral" else 4)
             tf = fl / (op_ms[k] * 1e-3) / 1e12
             ops[k] = {"ms": round(op_ms[k], 4), "bound": "mfma", "flops": fl,
                       "achieved_tflops": round(tf, 1),
@@ -276,7 +288,8 @@ def main():
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
-                                   algorithm=args.algorithm, schedule=args.schedule)
+                                   algorithm=args.algorithm, schedule=args.schedule,
+                                   energy_mode=args.energy_mode)
         ms64, _ = time_steps(torch, obj64, dd64.pos, max(args.steps // 2, 5), 2)
         log("[bench] fp64: %.3f ms/step" % ms64)
         obj64.evaluate(dd64.pos)
@@ -308,8 +321,8 @@ def main():
                         % (args.config, "" if args.scale == 1.0 else " scale %g" % args.scale,
                            d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, d.num_nets,
                            d.num_pins, d.num_bins_x, d.num_bins_y),
-            "form": "fp32 + %s spectral solve, %s schedule, %s density" % (
-                args.precision, args.schedule, args.algorithm),
+            "form": "fp32 + %s spectral solve, %s schedule, %s density, %s energy" % (
+                args.precision, args.schedule, args.algorithm, args.energy_mode),
             "ops": ops,
             **result,
         },

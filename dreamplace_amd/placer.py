@@ -347,7 +347,7 @@ def main(argv=None):
     if a.fixed_iterations:
         prm["early_stop"] = False
     gp = GlobalPlacer(d, dev, params=prm, precision=a.precision, algorithm="tiled", schedule="pins",
-                      seed=a.seed)
+                      energy_mode="spectral", seed=a.seed)
     torch.cuda.synchronize()
     t1 = time.time()
     x, hist = gp.run(a.iterations, log_every=a.log_every)
