@@ -223,7 +223,13 @@ __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
+// SPLIT = 2: a workgroup of 512 threads whose halves take the two halves of
+// K (K a multiple of 2 * BF_BK), each with its own LDS tiles, and add their
+// accumulators in a fixed order at the end.  At 1024^2 one GEMM has only 256
+// tiles, so the serial chain of K steps, not the matrix cores, sets the time;
+// halving it per workgroup and doubling the waves in flight shortens it.
+template <int SPLIT>
+__global__ void __launch_bounds__(256 * SPLIT) k_gemm_bf16x3(GemmBatch bt) {
   const TileIdx ti = map_tile(bt);
   if (!ti.ok) return;  // whole workgroup: before any barrier
   const GemmDesc& gd = desc_of(bt, ti.g);
@@ -232,13 +238,19 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
   float* C = (float*)gd.C;
   const int N = bt.N, K = bt.K;
   const int m0 = ti.ty * BT, n0 = ti.tx * BT;
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[BT * BF_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Al[BT * BF_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[BT * BF_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bl[BT * BF_LD];
-  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  __shared__ __attribute__((aligned(16))) __bf16 Ah_[SPLIT][BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Al_[SPLIT][BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh_[SPLIT][BT * BF_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl_[SPLIT][BT * BF_LD];
+  const int half = threadIdx.x >> 8;  // 0 when SPLIT == 1
+  const int t = threadIdx.x & 255, lane = t & 63, wid = t >> 6;
   const int wm = wid >> 1, wn = wid & 1;
   const int h = lane >> 5, l32 = lane & 31;
+  __bf16* Ah = Ah_[half];
+  __bf16* Al = Al_[half];
+  __bf16* Bh = Bh_[half];
+  __bf16* Bl = Bl_[half];
+  const int kspan = K / SPLIT, kbase = half * kspan;
   f32x16 acc = {};
   // A: 16 consecutive k of one row; B: 16 consecutive k of one column.  The
   // next K tile is loaded into registers while the current one is computed.
@@ -258,8 +270,9 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
   };
-  load(0);
-  for (int k0 = 0; k0 < K; k0 += BF_BK) {
+  load(kbase);
+  // the same trip count in both halves: every thread meets every barrier
+  for (int k0 = kbase; k0 < kbase + kspan; k0 += BF_BK) {
     bf16x8 ahi0, alo0, ahi1, alo1, bhi0, blo0, bhi1, blo1;
     split8(av, ahi0, alo0);
     split8(av + 8, ahi1, alo1);
@@ -275,7 +288,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
     *(bf16x8*)(Bl + b_col * BF_LD + b_kc) = blo0;
     *(bf16x8*)(Bl + b_col * BF_LD + b_kc + 8) = blo1;
     __syncthreads();
-    if (k0 + BF_BK < K) load(k0 + BF_BK);
+    if (k0 + BF_BK < kbase + kspan) load(k0 + BF_BK);
 #pragma unroll
     for (int s = 0; s < BF_BK / 16; ++s) {
       const int ko = 16 * s + 8 * h;
@@ -290,21 +303,40 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch bt) {
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc, 0, 0, 0);
     }
   }
+  if constexpr (SPLIT == 2) {
+    // the upper half's accumulators (4 waves x 16 x 64 floats = 16 KB)
+    // through the A-tile array of both halves (18 KB, no longer read), added
+    // to the lower half's: (first K half) + (second K half)
+    static_assert(sizeof(Ah_) >= 4 * 16 * 64 * sizeof(float), "reduction buffer");
+    float* red = (float*)&Ah_[0][0];
+    __syncthreads();
+    if (half == 1) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[(r * 4 + wid) * 64 + lane] = acc[r];
+    }
+    __syncthreads();
+    if (half == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = acc[r] + red[(r * 4 + wid) * 64 + lane];
+    }
+  }
   const float alpha = (float)bt.alpha;
   const float* R = (const float*)gd.R;
   double e = 0;
-  const int col = n0 + wn * 32 + l32;
+  if (half == 0) {
+    const int col = n0 + wn * 32 + l32;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-    float v = alpha * acc[r];
-    C[(size_t)row * N + col] = v;
-    if (R) e += (double)v * (double)R[(size_t)row * N + col];
+    for (int r = 0; r < 16; ++r) {
+      const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+      float v = alpha * acc[r];
+      C[(size_t)row * N + col] = v;
+      if (R) e += (double)v * (double)R[(size_t)row * N + col];
+    }
   }
   if (R) {
-    __shared__ double smem[4];
-    double bs = block_sum<256>(e, smem);
-    if (t == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
+    __shared__ double smem[4 * SPLIT];
+    double bs = block_sum<256 * SPLIT>(e, smem);
+    if (threadIdx.x == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
   }
 }
 
@@ -444,8 +476,10 @@ static int launch_gemm(int precision, GemmBatch& bt, int count, hipStream_t s, i
       hipLaunchKernelGGL(k_gemm_f64, grid, block, 0, s, bt);
     else if (precision == DP_SPECTRAL_FP32)
       hipLaunchKernelGGL(k_gemm_f32, grid, block, 0, s, bt);
+    else if (bt.K % (2 * BF_BK) == 0)
+      hipLaunchKernelGGL(k_gemm_bf16x3<2>, grid, dim3(512), 0, s, bt);
     else
-      hipLaunchKernelGGL(k_gemm_bf16x3, grid, block, 0, s, bt);
+      hipLaunchKernelGGL(k_gemm_bf16x3<1>, grid, block, 0, s, bt);
   } else {
     dim3 grid((unsigned)ceil_div(bt.N, 16), (unsigned)ceil_div(bt.M, 16), count);
     bool energy = false;

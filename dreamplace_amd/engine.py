@@ -141,12 +141,14 @@ class PinPlan:
 class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
                  energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
-                 keep_partial_wl=False, wirelength="weighted_average"):
+                 keep_partial_wl=False, wirelength="weighted_average", overlap=True):
         """keep_partial_wl: also store the per-net (pre-weight) wirelength in
         `partial_wl` (what the net-weight adjoint needs); the pins schedule
         skips that store otherwise.  wirelength: "weighted_average" or
         "logsumexp" (PlaceObj.py:467-503); the log-sum-exp model runs on the
-        "pins" and "unfused" schedules."""
+        "pins" and "unfused" schedules.  overlap (pins schedule): run the
+        wirelength forward on a second stream, concurrently with the density
+        chain it does not depend on; the node-gradient pass joins them."""
         if wirelength not in ("weighted_average", "logsumexp"):
             raise ValueError("wirelength must be weighted_average or logsumexp, got %r" % (wirelength,))
         self.lse = wirelength == "logsumexp"
@@ -205,6 +207,12 @@ class PlacementObjective:
         if self.pin_plan is not None:
             self.slot_w = torch.empty(P, dtype=dt, device=dev)
             self._slot_weights()
+        # a second stream (default priority) for the wirelength forward
+        self.side = None
+        if self.schedule == "pins" and overlap:
+            self.side = torch.cuda.Stream(device=dev)
+            self._ev_fork = torch.cuda.Event()
+            self._ev_join = torch.cuda.Event()
 
     def _slot_weights(self):
         """Net weights and mask per group-order pin slot (the pins schedule
@@ -306,15 +314,27 @@ class PlacementObjective:
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
         if self.schedule == "pins":
             pp = self.pin_plan
+            s_wl = s
+            if self.side is not None:
+                # fork: the side stream starts after everything queued so far
+                main = torch.cuda.current_stream(pos.device)
+                self._ev_fork.record(main)
+                self.side.wait_event(self._ev_fork)
+                s_wl = N.ctypes.c_void_p(self.side.cuda_stream)
             N.call("dp_lse_forward_pins" if self.lse else "dp_wa_forward_pins", code, N.ptr(pos), n,
                    P, N.ptr(pp.slot_node), N.ptr(pp.node_offsets), N.ptr(pp.node_dst),
                    N.ptr(pp.pin_meta), N.ptr(pp.group_start), pp.num_groups, N.ptr(pp.big),
                    pp.num_big, E, N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.slot_w),
                    N.ptr(self.gamma_t if self.lse else self.inv_gamma),
                    N.ptr(self.partial_wl) if self.keep_partial_wl else None,
-                   N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s)
+                   N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s_wl)
+            if self.side is not None:
+                self._ev_join.record(self.side)
             self._density_forward(pos)
             self._density_backward(pos)
+            if self.side is not None:
+                # join: the node gradient and the objective read the wirelength outputs
+                torch.cuda.current_stream(pos.device).wait_event(self._ev_join)
             if precondition is not None:
                 pws, areas, alpha = precondition
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),

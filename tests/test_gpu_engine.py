@@ -151,3 +151,23 @@ def test_engine_pins_net_weights(gpu):
         f2, g2 = o2.evaluate(dd2.pos)
         assert rel_l2(g1.cpu().numpy(), g2.cpu().numpy()) <= 1e-14
         assert abs(f1.item() - f2.item()) <= 1e-13 * abs(f2.item())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_engine_overlap_is_deterministic(gpu, dtype):
+    """The wirelength forward on its own stream gives the same bits as the
+    single-stream schedule, and repeated evaluations give the same bits."""
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case engine overlap determinism %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype, big_nets=10)
+    out = []
+    for overlap in (True, False):
+        dd = DeviceDesign(d, gpu)
+        ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
+                                else "fp64", algorithm="tiled", schedule="pins", overlap=overlap)
+        for _ in range(3):
+            f, g = ob.evaluate(dd.pos)
+            out.append((f.item(), g.clone()))
+    for f, g in out[1:]:
+        assert f == out[0][0]
+        assert torch.equal(g, out[0][1])
