@@ -23,6 +23,8 @@ its stream; achieved GB/s = the op's algorithmic bytes / its duration.
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import statistics
@@ -175,7 +177,29 @@ def accuracy(torch, obj32, obj64, dd64):
     return out
 
 
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send file descriptor 1 to stderr, so that native code printing to
+    stdout (the reference's CPU ops log through printf) cannot add a line
+    to the JSON output; C stdio is flushed before stdout comes back."""
+    sys.stdout.flush()
+    libc = ctypes.CDLL(None)
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def cpu_baseline(d, budget_s):
+    with stdout_to_stderr():
+        return _cpu_baseline(d, budget_s)
+
+
+def _cpu_baseline(d, budget_s):
     """The reference's own CPU ops (oracle/_ref) on the same design, one
     warm-up then timed iterations until the budget is used (at least one)."""
     import torch
@@ -201,6 +225,219 @@ def cpu_baseline(d, budget_s):
                        "%d threads): 1 warm-up + %d timed, median" % (threads, len(times)))
 
 
+def summarize_trace(trace_dir, warmup, steps):
+    """Text summary of `rocprofv3 --kernel-trace --stats --output-format csv`
+    over `bench.py --steps K --warmup W --skip-cpu`: per library kernel, the
+    calls and average over the whole run (the --stats table), and its
+    average in three phases of the FP32 run: the timed steps (wirelength
+    forward on its own stream), the timed one-stream steps, and the per-op
+    probe evaluations whose HIP-event durations make `config.ops` and the
+    roofline.  A dispatch belongs to evaluation i when i k_node_grad<float>
+    dispatches precede it (one per evaluation, its last kernel)."""
+    import csv
+    import glob
+    import re
+
+    def short(name):
+        m = re.match(r"(?:void )?([\w:]+(?:<[^>(]*>)?)", name)
+        return m.group(1) if m else name[:60]
+
+    stats = list(csv.DictReader(open(glob.glob(os.path.join(trace_dir, "*kernel_stats.csv"))[0])))
+    trace = sorted(csv.DictReader(open(glob.glob(os.path.join(trace_dir, "*kernel_trace.csv"))[0])),
+                   key=lambda r: int(r["Dispatch_Id"]))
+    K, W = steps, warmup
+    phases = {"overlap": (W, W + K), "one_stream": (W + K + 2, W + 2 * K + 2),
+              "probes": (W + 2 * K + 2, W + 2 * K + 2 + 5 * 3 + max(K, 10))}
+    acc = {}
+    ev = 0
+    for r in trace:
+        name = short(r["Kernel_Name"])
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        for ph, (a, b) in phases.items():
+            if a <= ev < b and name.startswith("dp::"):
+                acc.setdefault(name, {}).setdefault(ph, []).append(dur)
+        if name.startswith("dp::k_node_grad<float"):
+            ev += 1
+    lines = ["# %-44s %6s %10s %10s | %10s %10s %10s" % ("kernel", "calls", "total", "average",
+                                                         "overlap", "one-stream", "probes")]
+    for r in stats:
+        name = short(r["Name"])
+        if not name.startswith("dp::"):
+            continue
+        cols = []
+        for ph in ("overlap", "one_stream", "probes"):
+            v = acc.get(name, {}).get(ph)
+            cols.append("%10.1f" % statistics.mean(v) if v else "%10s" % "-")
+        lines.append("%-46s %6d %10.1f %10.1f | %s" % (name[:46], int(r["Calls"]),
+                                                       float(r["TotalDurationNs"]) / 1e3,
+                                                       float(r["AverageNs"]) / 1e3, " ".join(cols)))
+    lines.append("# evaluations counted: %d; phases (evaluation index ranges): %s" % (ev, phases))
+    return "\n".join(lines)
+
+
+# kernels whose HBM bytes make each op's measured traffic (per-dispatch
+# medians; the int64 map fill belongs to the scatter, the float fill to the
+# gather, one k_finish to the wirelength reduction)
+TRAFFIC_KERNELS = {
+    "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<", "dp::k_finish"],
+    "density_map": ["dp::k_fill<unsigned long long>", "dp::k_bin_count<", "dp::k_bin_scan",
+                    "dp::k_bin_fill<", "dp::k_tile_scatter<", "dp::k_rec_scatter_large<",
+                    "dp::k_convert_cpu<"],
+    "electric_force": ["dp::k_fill<float>", "dp::k_tile_gather<", "dp::k_rec_gather_large<"],
+    "node_grad": ["dp::k_node_grad<"],
+}
+
+
+def summarize_traffic(cal_fetch, cal_write, pmc_fetch, pmc_write, workload):
+    """HBM bytes per dispatch of each library kernel, and per op, from four
+    `rocprofv3 --pmc ... --output-format csv` runs: FETCH_SIZE and WRITE_SIZE
+    of `bench.py --calibrate-copy` (1 GiB read and written by one elementwise
+    kernel) and of the step.  bytes = median counter (KiB) x 1024 x scale,
+    scale = 1 GiB / what the counter reads for the copy."""
+    import collections
+    import csv
+    import glob
+    import re
+
+    def short(name):
+        m = re.match(r"(?:void )?([\w:]+(?:<[^>(]*>)?)", name)
+        return m.group(1) if m else name[:60]
+
+    def counters(d):
+        out = collections.defaultdict(list)
+        for r in csv.DictReader(open(glob.glob(os.path.join(d, "*counter_collection.csv"))[0])):
+            out[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return out
+
+    def copy_kib(d):
+        for k, v in counters(d).items():
+            if "elementwise" in k:
+                return statistics.median(v)
+        raise SystemExit("no elementwise copy kernel in " + d)
+
+    gib_kib = (1 << 30) / 1024.0
+    fscale, wscale = gib_kib / copy_kib(cal_fetch), gib_kib / copy_kib(cal_write)
+    f, w = counters(pmc_fetch), counters(pmc_write)
+    kernels = {}
+    for k in f:
+        if k.startswith("dp::"):
+            kernels[k] = {"fetch_bytes": round(statistics.median(f[k]) * 1024 * fscale),
+                          "write_bytes": round(statistics.median(w.get(k, [0])) * 1024 * wscale),
+                          "dispatches": len(f[k])}
+    ops = {}
+    for op, prefixes in TRAFFIC_KERNELS.items():
+        ops[op] = sum(v["fetch_bytes"] + v["write_bytes"] for k, v in kernels.items()
+                      if any(k.startswith(p) for p in prefixes))
+    return {"workload": workload,
+            "scaling": "bytes = median per-dispatch counter (KiB) x 1024 x scale; scale = 1 GiB / "
+                       "what the counter reads for the 1 GiB elementwise copy of --calibrate-copy",
+            "fetch_scale": fscale, "write_scale": wscale, "ops": ops, "kernels": kernels}
+
+
+def wa_terms_study(torch, args):
+    """Differentiation study on the config-3 netlist (10M pins, 2.5M nets,
+    net weights log-uniform in [0.5, 4]), in FP32 and FP64, gamma in
+    {1, 4, 16} x bin width:
+    * the WA module's two ways of keeping the exp/max terms for the backward
+      (the stored per-pin derivative, or recomputing it from the positions):
+      forward and backward ms (events, median), the HBM the forward leaves
+      held for the backward and the peak over forward + backward (torch
+      allocator statistics above what was allocated before the forward);
+    * the gradients against FP64 central differences (h = 1e-4 x bin width)
+      at 1000 random pin coordinates and 1000 random net weights; FP32
+      gradients are compared at the FP32-rounded point.
+    Prints one JSON line per (dtype, gamma, terms)."""
+    from dreamplace_amd import _native as N
+    from dreamplace_amd import gradcheck, synth
+    from dreamplace_amd.pin_pos import PinPos
+    from dreamplace_amd.weighted_average_wirelength import WeightedAverageWirelength
+    dev = torch.device("cuda:0")
+    N.lib()
+    t0 = time.perf_counter()
+    d = synth.make_design(3, seed=0, dtype=np.float64, scale=args.scale)
+    log("[study] design built in %.1fs: %s" % (time.perf_counter() - t0, synth.summary(d)))
+
+    def T(a, dtype=None):
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        return t if dtype is None else t.to(dtype)
+
+    fnp, nps = T(d.flat_net2pin_map), T(d.flat_net2pin_start_map)
+    p2n, net_mask = T(d.pin2net_map), T(d.net_mask_ignore_large_degrees)
+    pin_mask = T(d.pin_mask_ignore_fixed_macros)
+    pp = PinPos(T(d.pin_offset_x), T(d.pin_offset_y), T(d.pin2node_map), T(d.flat_node2pin_map),
+                T(d.flat_node2pin_start_map), d.num_physical_nodes)
+    pin32 = pp(T(d.pos)).detach().float()
+    w32 = T(d.net_weights, torch.float32)
+    x64, w64 = pin32.double(), w32.double()
+    P, b = d.num_pins, d.bin_size_x
+    pmask = d.pin_mask_ignore_fixed_macros
+    rng = np.random.default_rng(0)
+    reps = 10
+    for gamma_bins in (1, 4, 16):
+        gamma = gamma_bins * b
+        h = 1e-4 * b
+        fwd = gradcheck.WaForward64(fnp, nps, net_mask, gamma)
+        nets, slots, axes = gradcheck.pick_pin_coordinates(rng, d.flat_net2pin_start_map, 1000)
+        pins = d.flat_net2pin_map[slots]
+        t1 = time.perf_counter()
+        fd = gradcheck.pin_fd(fwd, x64, w64, nets, pins, axes, h)
+        fd = np.where(pmask[pins], 0.0, fd)
+        wnets = rng.choice(d.num_nets, size=1000, replace=False)
+        fdw = gradcheck.weight_fd(fwd, x64, w64, wnets, h)
+        log("[study] gamma %dxbin: FD in %.1fs" % (gamma_bins, time.perf_counter() - t1))
+        coord = axes * P + pins
+        for dt, x0, w0 in ((torch.float32, pin32, w32), (torch.float64, x64, w64)):
+            auto = WeightedAverageWirelength(fnp, nps, p2n, w0, net_mask, pin_mask,
+                                             torch.tensor(gamma, dtype=dt, device=dev),
+                                             algorithm="merged").recomputes(x0)
+            for terms in ("store", "recompute"):
+                w = w0.clone().requires_grad_(True)
+                op = WeightedAverageWirelength(fnp, nps, p2n, w, net_mask, pin_mask,
+                                               torch.tensor(gamma, dtype=dt, device=dev),
+                                               algorithm="merged", terms=terms)
+                x = x0.detach().clone().requires_grad_(True)
+                for _ in range(2):
+                    x.grad, w.grad = None, None
+                    op(x).backward()
+                fms, bms = [], []
+                for _ in range(reps):
+                    x.grad, w.grad = None, None
+                    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+                    e0.record()
+                    wl = op(x)
+                    e1.record()
+                    wl.backward()
+                    e2.record()
+                    torch.cuda.synchronize()
+                    fms.append(e0.elapsed_time(e1))
+                    bms.append(e1.elapsed_time(e2))
+                x.grad, w.grad = None, None
+                del wl
+                torch.cuda.synchronize()
+                torch.cuda.reset_peak_memory_stats(dev)
+                base = torch.cuda.memory_allocated(dev)
+                wl = op(x)
+                torch.cuda.synchronize()
+                held = torch.cuda.memory_allocated(dev) - base
+                wl.backward()
+                torch.cuda.synchronize()
+                peak = torch.cuda.max_memory_allocated(dev) - base
+                gx = x.grad.double().cpu().numpy()[coord]
+                gw = w.grad.double().cpu().numpy()[wnets]
+                rec = dict(study="wa_terms", config=3, dtype=str(dt).replace("torch.", ""),
+                           gamma_bins=gamma_bins, terms=terms, num_pins=P, num_nets=d.num_nets,
+                           ms_forward=round(statistics.median(fms), 4),
+                           ms_backward=round(statistics.median(bms), 4),
+                           mb_held_forward_to_backward=round(held / 1e6, 2),
+                           mb_peak_forward_backward=round(peak / 1e6, 2),
+                           auto_choice="recompute" if auto else "store",
+                           fd_max_err_coordinates=float(np.abs(gx - fd).max() / max(1.0, np.abs(fd).max())),
+                           fd_max_err_weights=float(np.abs(gw - fdw).max() / np.abs(fdw).max()),
+                           fd_points=[len(fd), len(fdw)], fd_h=h)
+                print(json.dumps(rec), flush=True)
+                del wl, op, x, w
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +456,16 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--study", choices=["wa-terms"],
+                    help="wa-terms: the config-3 differentiation study (store vs recompute of the WA "
+                         "exp/max terms, FD checks) instead of the step benchmark")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the per-phase kernel summary of a rocprofv3 csv kernel trace of "
+                         "`bench.py --steps K --warmup W --skip-cpu` (pass the same K and W) and exit")
+    ap.add_argument("--summarize-traffic", nargs=4,
+                    metavar=("CAL_FETCH", "CAL_WRITE", "PMC_FETCH", "PMC_WRITE"),
+                    help="print the HBM-traffic JSON from four rocprofv3 --pmc csv output folders "
+                         "(FETCH_SIZE and WRITE_SIZE of --calibrate-copy, then of the step) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB -> 1 GiB device copy kernel 5 times (the known-size "
                          "reference for FETCH_SIZE / WRITE_SIZE counter scaling) and exit")
@@ -228,7 +475,18 @@ def main():
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
+    if args.summarize_traffic:
+        wl = ("config1: 2200000 movable cells, 25000 fixed, 10924750 fillers, 2200000 nets, "
+              "9623921 pins, 1024x1024 bins; FP32 + bf16x3 solve, pins schedule, tiled density")
+        print(json.dumps(summarize_traffic(*args.summarize_traffic, workload=wl), indent=1))
+        return
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.warmup, args.steps))
+        return
     import torch
+    if args.study == "wa-terms":
+        wa_terms_study(torch, args)
+        return
     if args.calibrate_copy:
         n = (1 << 30) // 4
         a = torch.rand(n, device="cuda:0")
@@ -259,6 +517,16 @@ def main():
     log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32,
                                                                 statistics.median(per32)))
 
+    # The step above runs the wirelength forward on a second stream beside the
+    # density chain.  The ops are timed each alone (all on one stream), so each
+    # op's duration, and its roofline, is its own and not shared with a
+    # concurrent op; the one-stream step time is reported beside the value.
+    ms_serial = None
+    if obj32.overlap:
+        obj32.set_overlap(False)
+        ms_serial, _ = time_steps(torch, obj32, pos32, args.steps, 2)
+        log("[bench] fp32/%s one stream: %.3f ms/step" % (args.precision, ms_serial))
+
     # per-op live timing
     hbm_bytes = op_bytes(d, args.schedule)
     names = [k for k in PROBES if k in hbm_bytes or k == "poisson"]
@@ -285,6 +553,9 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
     result = {}
+    if ms_serial is not None:
+        result.update(ms_per_step_one_stream=round(ms_serial, 4),
+                      ops_timing="each op alone, every kernel on one stream")
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",

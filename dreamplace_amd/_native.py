@@ -57,6 +57,8 @@ _SIGNATURES = {
     "dp_wa_merged_forward": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dp_lse_merged_forward": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_merged_backward": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
+    "dp_wa_merged_backward_recompute": (_I, [_I, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P,
+                                             _P]),
     "dp_wa_net_weight_grad": (_I, [_I, _P, _P, _P, _L, _P, _P]),
     "dp_wa_forward_from_nodes": (_I, [_I, _P, _L, _P, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P,
                                       _P, _P, _P, _P]),

@@ -43,9 +43,75 @@ __global__ void k_pin_pos_bwd(const T* __restrict__ gpx, const T* __restrict__ g
   gy[n] = sy;
 }
 
-// Merged WA, one thread per net (both axes): pass 1 max/min, pass 2 the
-// max-shifted exponential sums, pass 3 the per-pin derivative
-// (weighted_average_wirelength_merged.cpp:31-116).  Block partials of
+// One net of the merged WA model, both axes: the extreme pin coordinates
+// (pass 1) and the max-shifted exponential sums (pass 2),
+// weighted_average_wirelength_merged.cpp:49-92.
+template <typename T>
+struct WaNet {
+  T xmax, xmin, ymax, ymin;
+  T xs, xns, es, ens, ys, yns, fs, fns;
+  __device__ __forceinline__ T wl() const { return xs / es - xns / ens + ys / fs - yns / fns; }
+};
+
+template <typename T>
+__device__ __forceinline__ WaNet<T> wa_net(const T* __restrict__ px, const T* __restrict__ py,
+                                           const int* __restrict__ flat_netpin, int b, int en,
+                                           T ig) {
+  WaNet<T> n;
+  n.xmax = n.ymax = -tmax_value<T>();
+  n.xmin = n.ymin = tmax_value<T>();
+  for (int j = b; j < en; ++j) {
+    int p = flat_netpin[j];
+    T xx = px[p], yy = py[p];
+    n.xmax = xx > n.xmax ? xx : n.xmax;
+    n.xmin = xx < n.xmin ? xx : n.xmin;
+    n.ymax = yy > n.ymax ? yy : n.ymax;
+    n.ymin = yy < n.ymin ? yy : n.ymin;
+  }
+  n.xs = n.xns = n.es = n.ens = n.ys = n.yns = n.fs = n.fns = 0;
+  for (int j = b; j < en; ++j) {
+    int p = flat_netpin[j];
+    T xx = px[p], yy = py[p];
+    T ex = exp((xx - n.xmax) * ig), enx = exp((n.xmin - xx) * ig);
+    n.xs += xx * ex;
+    n.xns += xx * enx;
+    n.es += ex;
+    n.ens += enx;
+    T ey = exp((yy - n.ymax) * ig), eny = exp((n.ymin - yy) * ig);
+    n.ys += yy * ey;
+    n.yns += yy * eny;
+    n.fs += ey;
+    n.fns += eny;
+  }
+  return n;
+}
+
+// Pass 3: d(WL_n)/dx_p = (a+ + b+ x) e+ - (a- + b- x) e- per axis, with
+// b+ = (1/gamma)/S+, a+ = (1 - b+ XS+)/S+, b- = -(1/gamma)/S-,
+// a- = (1 - b- XS-)/S- (weighted_average_wirelength_merged.cpp:94-116).
+template <typename T>
+struct WaDeriv {
+  T xmax, xmin, ymax, ymin, ig;
+  T bx, ax, bnx, anx, by, ay, bny, any;
+  __device__ __forceinline__ WaDeriv(const WaNet<T>& n, T ig_) {
+    xmax = n.xmax; xmin = n.xmin; ymax = n.ymax; ymin = n.ymin; ig = ig_;
+    bx = ig / n.es; ax = (T)((1.0 - bx * n.xs) / n.es);
+    bnx = -ig / n.ens; anx = (T)((1.0 - bnx * n.xns) / n.ens);
+    by = ig / n.fs; ay = (T)((1.0 - by * n.ys) / n.fs);
+    bny = -ig / n.fns; any = (T)((1.0 - bny * n.yns) / n.fns);
+  }
+  __device__ __forceinline__ void at(T xx, T yy, T& gx, T& gy) const {
+    T ex = exp((xx - xmax) * ig), enx = exp((xmin - xx) * ig);
+    gx = (ax + bx * xx) * ex - (anx + bnx * xx) * enx;
+    T ey = exp((yy - ymax) * ig), eny = exp((ymin - yy) * ig);
+    gy = (ay + by * yy) * ey - (any + bny * yy) * eny;
+  }
+};
+
+// Merged WA, one thread per net (both axes): passes 1-2, then (store form)
+// pass 3 writes the per-pin derivative into gix/giy
+// (weighted_average_wirelength_merged.cpp:31-116).  With gix == nullptr (the
+// recompute form) nothing per pin is written.  Block partials of
 // sum_n w_n*WL_n (FP64) feed the deterministic final reduction.
 template <typename T>
 __global__ void __launch_bounds__(256) k_wa_fwd(const T* __restrict__ px, const T* __restrict__ py,
@@ -65,49 +131,59 @@ __global__ void __launch_bounds__(256) k_wa_fwd(const T* __restrict__ px, const 
       continue;
     }
     const int b = netpin_start[e], en = netpin_start[e + 1];
-    T xmax = -tmax_value<T>(), xmin = tmax_value<T>();
-    T ymax = -tmax_value<T>(), ymin = tmax_value<T>();
-    for (int j = b; j < en; ++j) {
-      int p = flat_netpin[j];
-      T xx = px[p], yy = py[p];
-      xmax = xx > xmax ? xx : xmax;
-      xmin = xx < xmin ? xx : xmin;
-      ymax = yy > ymax ? yy : ymax;
-      ymin = yy < ymin ? yy : ymin;
-    }
-    T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
-    for (int j = b; j < en; ++j) {
-      int p = flat_netpin[j];
-      T xx = px[p], yy = py[p];
-      T ex = exp((xx - xmax) * ig), enx = exp((xmin - xx) * ig);
-      xs += xx * ex;
-      xns += xx * enx;
-      es += ex;
-      ens += enx;
-      T ey = exp((yy - ymax) * ig), eny = exp((ymin - yy) * ig);
-      ys += yy * ey;
-      yns += yy * eny;
-      fs += ey;
-      fns += eny;
-    }
-    T wl = xs / es - xns / ens + ys / fs - yns / fns;
+    const WaNet<T> n = wa_net(px, py, flat_netpin, b, en, ig);
+    T wl = n.wl();
     if (partial) partial[e] = wl;
     contrib += (double)(w ? wl * w[e] : wl);
-    T bx = ig / es, ax = (T)((1.0 - bx * xs) / es);
-    T bnx = -ig / ens, anx = (T)((1.0 - bnx * xns) / ens);
-    T by = ig / fs, ay = (T)((1.0 - by * ys) / fs);
-    T bny = -ig / fns, any = (T)((1.0 - bny * yns) / fns);
-    for (int j = b; j < en; ++j) {
-      int p = flat_netpin[j];
-      T xx = px[p], yy = py[p];
-      T ex = exp((xx - xmax) * ig), enx = exp((xmin - xx) * ig);
-      gix[p] = (ax + bx * xx) * ex - (anx + bnx * xx) * enx;
-      T ey = exp((yy - ymax) * ig), eny = exp((ymin - yy) * ig);
-      giy[p] = (ay + by * yy) * ey - (any + bny * yy) * eny;
+    if (gix) {
+      const WaDeriv<T> d(n, ig);
+      for (int j = b; j < en; ++j) {
+        int p = flat_netpin[j];
+        d.at(px[p], py[p], gix[p], giy[p]);
+      }
     }
   }
   double bs = block_sum<256>(contrib, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// Backward of the recompute form: passes 1-3 again from the pin positions,
+// and the pin gradient written directly as ((dWL_n/dx_p) * grad) * w_n, the
+// store form's k_wa_bwd arithmetic on the same derivative; pins under
+// pin_mask get 0.  Pins of masked nets are not written (the caller's output
+// holds zeros there, as the store form's derivative buffer does).
+template <typename T>
+__global__ void __launch_bounds__(256) k_wa_bwd_recompute(
+    const T* __restrict__ px, const T* __restrict__ py, const int* __restrict__ flat_netpin,
+    const int* __restrict__ netpin_start, int64_t E, const T* __restrict__ w,
+    const uint8_t* __restrict__ mask, const uint8_t* __restrict__ pin_mask,
+    const T* __restrict__ inv_gamma_p, const T* __restrict__ grad, T* gx_out, T* gy_out) {
+  const T ig = *inv_gamma_p;
+  const T g = *grad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (!mask[e]) continue;
+    const int b = netpin_start[e], en = netpin_start[e + 1];
+    const WaDeriv<T> d(wa_net(px, py, flat_netpin, b, en, ig), ig);
+    for (int j = b; j < en; ++j) {
+      int p = flat_netpin[j];
+      T gx, gy;
+      d.at(px[p], py[p], gx, gy);
+      gx = gx * g;
+      gy = gy * g;
+      if (w) {
+        T wn = w[e];
+        gx *= wn;
+        gy *= wn;
+      }
+      if (pin_mask && pin_mask[p]) {
+        gx = 0;
+        gy = 0;
+      }
+      gx_out[p] = gx;
+      gy_out[p] = gy;
+    }
+  }
 }
 
 // Merged log-sum-exp wirelength, one thread per net (both axes)
@@ -419,8 +495,7 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
                          int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
                          const void* inv_gamma, void* partial_wl, void* grad_intermediate,
                          void* wl_out, void* workspace, void* stream) {
-  DP_ARG(pin_pos && flat_netpin && netpin_start && net_mask && inv_gamma && grad_intermediate &&
-             wl_out && workspace,
+  DP_ARG(pin_pos && flat_netpin && netpin_start && net_mask && inv_gamma && wl_out && workspace,
          "dp_wa_merged_forward: null pointer");
   hipStream_t s = S(stream);
   DP_PROBE_OP(DP_OP_WA_FORWARD, s);
@@ -433,7 +508,7 @@ int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
               hipLaunchKernelGGL(k_wa_fwd<T>, dim3(grid), dim3(256), 0, s, pp, pp + num_pins,
                                  (const int*)flat_netpin, (const int*)netpin_start, num_nets,
                                  (const T*)net_weights, net_mask, (const T*)inv_gamma,
-                                 (T*)partial_wl, gi, gi + num_pins, partials));
+                                 (T*)partial_wl, gi, gi ? gi + num_pins : nullptr, partials));
     DP_LAUNCHED("k_wa_fwd");
     return finish_reduction(partials, grid, 1, nullptr, wl_out, dtype, s);
   });
@@ -479,6 +554,31 @@ int dp_wa_merged_backward(int dtype, const void* grad, void* grad_intermediate, 
                                  (const int*)pin2net_map, (const T*)net_weights, net_mask,
                                  pin_mask));
     DP_LAUNCHED("k_wa_bwd");
+    return DP_OK;
+  });
+}
+
+int dp_wa_merged_backward_recompute(int dtype, const void* grad, const void* pin_pos,
+                                    int64_t num_pins, const int32_t* flat_netpin,
+                                    const int32_t* netpin_start, int64_t num_nets,
+                                    const void* net_weights, const uint8_t* net_mask,
+                                    const uint8_t* pin_mask, const void* inv_gamma,
+                                    void* grad_pin_pos, void* stream) {
+  DP_ARG(grad && pin_pos && flat_netpin && netpin_start && net_mask && inv_gamma && grad_pin_pos,
+         "dp_wa_merged_backward_recompute: null pointer");
+  if (num_nets == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  int grid = reduce_grid(num_nets);
+  return DP_DISPATCH(dtype, "dp_wa_merged_backward_recompute", [&] {
+    const T* pp = (const T*)pin_pos;
+    T* g = (T*)grad_pin_pos;
+    DP_PROBED(DP_KERNEL_WA_BACKWARD, s,
+              hipLaunchKernelGGL(k_wa_bwd_recompute<T>, dim3(grid), dim3(256), 0, s, pp,
+                                 pp + num_pins, (const int*)flat_netpin,
+                                 (const int*)netpin_start, num_nets, (const T*)net_weights,
+                                 net_mask, pin_mask, (const T*)inv_gamma, (const T*)grad, g,
+                                 g + num_pins));
+    DP_LAUNCHED("k_wa_bwd_recompute");
     return DP_OK;
   });
 }

@@ -209,10 +209,24 @@ class PlacementObjective:
             self._slot_weights()
         # a second stream (default priority) for the wirelength forward
         self.side = None
-        if self.schedule == "pins" and overlap:
-            self.side = torch.cuda.Stream(device=dev)
-            self._ev_fork = torch.cuda.Event()
-            self._ev_join = torch.cuda.Event()
+        self._side_stream = None
+        self.set_overlap(overlap)
+
+    def set_overlap(self, on):
+        """Run the pins schedule's wirelength forward on a second stream
+        (on=True) or on the caller's stream.  Either gives the same bits."""
+        if on and self.schedule == "pins":
+            if self._side_stream is None:
+                self._side_stream = torch.cuda.Stream(device=self.dd.device)
+                self._ev_fork = torch.cuda.Event()
+                self._ev_join = torch.cuda.Event()
+            self.side = self._side_stream
+        else:
+            self.side = None
+
+    @property
+    def overlap(self):
+        return self.side is not None
 
     def _slot_weights(self):
         """Net weights and mask per group-order pin slot (the pins schedule

@@ -11,6 +11,15 @@ pin positions.  Every algorithm name runs the merged net-parallel kernel
 New: when ``net_weights.requires_grad`` the backward also returns
 d(wl)/d(net_weights) = grad * WL_n for unmasked nets (the reference returns
 None there, weighted_average_wirelength.py:213).
+
+New: ``terms`` chooses what the forward keeps for the backward.  "store"
+keeps the per-pin derivative [2P], as the reference's merged function does
+(weighted_average_wirelength.py:189).  "recompute" keeps nothing per pin and
+recomputes the exp/max terms in the backward (one more pass over the pins),
+writing the pin gradient directly; the result is the same.  "auto" (the
+default) stores when the derivative's 2*P*itemsize bytes fit in
+``store_limit_bytes`` (default: 1/32 of the device's memory) and recomputes
+otherwise.
 """
 import torch
 from torch import nn
@@ -22,11 +31,12 @@ from . import _native as N
 class WeightedAverageWirelengthMergedFunction(Function):
     @staticmethod
     def forward(ctx, pos, flat_netpin, netpin_start, pin2net_map, net_weights, net_mask, pin_mask,
-                inv_gamma):
-        """Same inputs as weighted_average_wirelength.py:165-195; returns the
-        0-dim weighted wirelength."""
+                inv_gamma, recompute=False):
+        """Same inputs as weighted_average_wirelength.py:165-195 (plus the
+        store/recompute choice); returns the 0-dim weighted wirelength."""
         name = "WeightedAverageWirelength"
         N.require_gpu(pos, flat_netpin, netpin_start, net_weights, net_mask, name=name)
+        pos_in = pos
         pos = pos.view(pos.numel())
         N.require_contiguous(pos, flat_netpin, netpin_start, name=name)
         if pos.numel() % 2:
@@ -38,7 +48,7 @@ class WeightedAverageWirelengthMergedFunction(Function):
         w = net_weights.detach().to(dt).contiguous() if net_weights is not None and net_weights.numel() else None
         mask = N.as_u8(net_mask).contiguous()
         partial = torch.empty(num_nets, dtype=dt, device=pos.device)
-        grad_intermediate = torch.zeros_like(pos)
+        grad_intermediate = None if recompute else torch.zeros_like(pos)
         wl = torch.empty((), dtype=dt, device=pos.device)
         ws = N.workspace.get("wa", N.query("dp_wa_workspace_bytes", num_nets), pos.device)
         fnp, nps = N.as_i32(flat_netpin), N.as_i32(netpin_start)
@@ -50,6 +60,13 @@ class WeightedAverageWirelengthMergedFunction(Function):
         ctx.net_mask = mask
         ctx.pin_mask = N.as_u8(pin_mask).contiguous() if pin_mask is not None else None
         ctx.grad_intermediate = grad_intermediate
+        if recompute:
+            # what the backward recomputes from: the pin positions (saved
+            # without a copy; autograd rejects a backward after they were
+            # changed in place), the net CSR and 1/gamma
+            ctx.save_for_backward(pos_in)
+            ctx.flat_netpin, ctx.netpin_start = fnp, nps
+            ctx.inv_gamma, ctx.num_nets = inv_gamma, num_nets
         ctx.partial_wl = partial
         ctx.num_pins = num_pins
         ctx.weights_need_grad = net_weights is not None and net_weights.requires_grad
@@ -59,11 +76,23 @@ class WeightedAverageWirelengthMergedFunction(Function):
     @staticmethod
     def backward(ctx, grad_pos):
         gi = ctx.grad_intermediate
-        grad = grad_pos.reshape(()).to(gi.dtype).contiguous()
-        # in place on grad_intermediate, as the reference does
-        N.call("dp_wa_merged_backward", N.dtype_code(gi), N.ptr(grad), N.ptr(gi), ctx.num_pins,
-               N.ptr(ctx.pin2net_map), N.ptr(ctx.net_weights), N.ptr(ctx.net_mask),
-               N.ptr(ctx.pin_mask), N.stream_ptr(gi.device))
+        # drop the context's reference, so autograd can take the buffer as
+        # the gradient instead of copying it
+        ctx.grad_intermediate = None
+        if gi is None:
+            pos = ctx.saved_tensors[0].reshape(-1)
+            grad = grad_pos.reshape(()).to(pos.dtype).contiguous()
+            gi = torch.zeros_like(pos)
+            N.call("dp_wa_merged_backward_recompute", N.dtype_code(pos), N.ptr(grad), N.ptr(pos),
+                   ctx.num_pins, N.ptr(ctx.flat_netpin), N.ptr(ctx.netpin_start), ctx.num_nets,
+                   N.ptr(ctx.net_weights), N.ptr(ctx.net_mask), N.ptr(ctx.pin_mask),
+                   N.ptr(ctx.inv_gamma), N.ptr(gi), N.stream_ptr(pos.device))
+        else:
+            grad = grad_pos.reshape(()).to(gi.dtype).contiguous()
+            # in place on grad_intermediate, as the reference does
+            N.call("dp_wa_merged_backward", N.dtype_code(gi), N.ptr(grad), N.ptr(gi),
+                   ctx.num_pins, N.ptr(ctx.pin2net_map), N.ptr(ctx.net_weights),
+                   N.ptr(ctx.net_mask), N.ptr(ctx.pin_mask), N.stream_ptr(gi.device))
         grad_w = None
         if ctx.weights_need_grad:
             grad_w = torch.empty_like(ctx.partial_wl)
@@ -71,12 +100,13 @@ class WeightedAverageWirelengthMergedFunction(Function):
                    N.ptr(ctx.net_mask), ctx.partial_wl.numel(), N.ptr(grad_w),
                    N.stream_ptr(gi.device))
             grad_w = grad_w.to(ctx.weights_dtype)
-        return gi, None, None, None, grad_w, None, None, None
+        return gi, None, None, None, grad_w, None, None, None, None
 
 
 class WeightedAverageWirelength(nn.Module):
     def __init__(self, flat_netpin=None, netpin_start=None, pin2net_map=None, net_weights=None,
-                 net_mask=None, pin_mask=None, gamma=None, algorithm='atomic'):
+                 net_mask=None, pin_mask=None, gamma=None, algorithm='atomic', terms="auto",
+                 store_limit_bytes=None):
         super(WeightedAverageWirelength, self).__init__()
         assert net_weights is not None and net_mask is not None and pin_mask is not None \
             and gamma is not None, "net_weights, net_mask, pin_mask, gamma are requried parameters"
@@ -92,8 +122,21 @@ class WeightedAverageWirelength(nn.Module):
         self.pin_mask = pin_mask
         self.gamma = gamma
         self.algorithm = algorithm
+        if terms not in ("auto", "store", "recompute"):
+            raise ValueError("terms must be auto, store or recompute, got %r" % (terms,))
+        self.terms = terms
+        self.store_limit_bytes = store_limit_bytes
+
+    def recomputes(self, pos):
+        """Whether a forward on `pos` keeps nothing per pin for the backward."""
+        if self.terms != "auto":
+            return self.terms == "recompute"
+        limit = self.store_limit_bytes
+        if limit is None:
+            limit = torch.cuda.get_device_properties(pos.device).total_memory // 32
+        return pos.numel() * pos.element_size() > limit
 
     def forward(self, pos):
         return WeightedAverageWirelengthMergedFunction.apply(
             pos, self.flat_netpin, self.netpin_start, self.pin2net_map, self.net_weights,
-            self.net_mask, self.pin_mask, 1.0 / self.gamma)
+            self.net_mask, self.pin_mask, 1.0 / self.gamma, self.recomputes(pos))

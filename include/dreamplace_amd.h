@@ -224,7 +224,9 @@ size_t dp_wa_workspace_bytes(int64_t num_nets);
  * pre-weight per-net WL (partial_wl, nullable), the per-pin derivative of the
  * net WL (grad_intermediate[2P], entries of masked nets untouched) and
  * *wl_out = sum_n w_n*WL_n (device scalar of dtype, FP64 accumulation,
- * deterministic).  net_weights nullable (all 1).  inv_gamma is a device scalar. */
+ * deterministic).  net_weights nullable (all 1).  inv_gamma is a device scalar.
+ * grad_intermediate may be null: then nothing per pin is stored, the forward
+ * of the recompute form (dp_wa_merged_backward_recompute). */
 int dp_wa_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
                          const int32_t* flat_netpin, const int32_t* netpin_start,
                          int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
@@ -255,6 +257,23 @@ int dp_lse_merged_forward(int dtype, const void* pin_pos, int64_t num_pins,
 int dp_wa_merged_backward(int dtype, const void* grad, void* grad_intermediate, int64_t num_pins,
                           const int32_t* pin2net_map, const void* net_weights,
                           const uint8_t* net_mask, const uint8_t* pin_mask, void* stream);
+
+/* The backward of the recompute form: the reference keeps the per-pin
+ * derivative from the forward (weighted_average_wirelength.py:189, the store
+ * form above; its net-by-net function keeps the exponential terms themselves,
+ * :59-64); this entry point recomputes it from pin_pos instead,
+ * one thread per net (weighted_average_wirelength_merged.cpp:49-116), and
+ * writes grad_pin_pos[2P] = (dWL_n/dx_p * (*grad)) * w_n, 0 where
+ * pin_mask[p] != 0: the values dp_wa_merged_backward leaves in
+ * grad_intermediate.  Pins of masked nets are not written (the caller zeroes
+ * grad_pin_pos).  Saves the [2P] derivative held between forward and
+ * backward, for one more pass over the pins. */
+int dp_wa_merged_backward_recompute(int dtype, const void* grad, const void* pin_pos,
+                                    int64_t num_pins, const int32_t* flat_netpin,
+                                    const int32_t* netpin_start, int64_t num_nets,
+                                    const void* net_weights, const uint8_t* net_mask,
+                                    const uint8_t* pin_mask, const void* inv_gamma,
+                                    void* grad_pin_pos, void* stream);
 
 /* New (no counterpart in the reference, which returns None for the weights,
  * weighted_average_wirelength.py:213): d(sum_n w_n WL_n)/d(w_n) times *grad,
