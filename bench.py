@@ -453,6 +453,9 @@ def main():
     ap.add_argument("--energy-mode", default="spectral", choices=["spectral", "potential"],
                     help="energy from the DCT coefficients (Parseval; no potential map) or "
                          "from the materialised potential map as the reference does")
+    ap.add_argument("--filler-order", default="tile", choices=["tile", "generation"],
+                    help="tile: fillers relabelled into the density kernels' tile order before the "
+                         "timed steps (as the placer keeps them); generation: as generated")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -509,6 +512,14 @@ def main():
     gamma = d.wa_gamma
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
+    perm = None
+    if args.filler_order == "tile":
+        # fillers are interchangeable (one size, no pins): the engine keeps
+        # them in the density kernels' tile order; the FP64 form and the CPU
+        # baseline get the same order
+        perm = dd32.reorder_fillers()
+        d = synth.Design(d)
+        d.pos = dd32.pos.cpu().numpy()
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
                                algorithm=args.algorithm, schedule=args.schedule,
                                energy_mode=args.energy_mode)
@@ -594,6 +605,7 @@ def main():
                            d.num_pins, d.num_bins_x, d.num_bins_y),
             "form": "fp32 + %s spectral solve, %s schedule, %s density, %s energy" % (
                 args.precision, args.schedule, args.algorithm, args.energy_mode),
+            "filler_order": args.filler_order,
             "ops": ops,
             **result,
         },

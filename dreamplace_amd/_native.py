@@ -81,6 +81,8 @@ _SIGNATURES = {
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_optimizer_workspace_bytes": (_S, []),
     "dp_nesterov_extrapolate": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dp_nesterov_extrapolate_bounded": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D,
+                                             _D, _D, _D, _L, _L, _P, _P]),
     "dp_step_norms": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P]),
     "dp_l1_norm": (_I, [_I, _L, _P, _P, _P, _P]),
     "dp_precondition": (_I, [_I, _P, _L, _P, _P, _P, _D, _P]),

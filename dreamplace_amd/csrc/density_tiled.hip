@@ -180,6 +180,27 @@ __device__ __forceinline__ int64_t node_of(int64_t t, int64_t n_mov, int64_t fil
   return t < n_mov ? t : fill_begin + (t - n_mov);
 }
 
+// counters[key] += 1 for every lane with key >= 0; returns the lane's rank
+// (the counter before its own increment; -1 for key < 0).  The lanes whose
+// key equals the first active lane's add with one atomic from their lowest
+// lane, and rank by their position among those lanes: cells in tile order
+// put most of a wavefront in one tile, where per-lane atomics on one LDS
+// word would serialise.  Lanes that are not active take no part; every
+// active lane counts once per call.
+__device__ __forceinline__ int wave_counter_inc(int* counters, int key) {
+  const int k0 = __builtin_amdgcn_readfirstlane(key);
+  const unsigned long long same = __ballot(key >= 0 && key == k0);
+  if (k0 >= 0 && key == k0) {
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)same) - 1;  // uniform: from the ballot
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counters[k0], (int)__popcll(same));
+    base = __builtin_amdgcn_readlane(base, leader);
+    return base + (int)__popcll(same & ((1ull << lane) - 1ull));
+  }
+  return key >= 0 ? atomicAdd(&counters[key], 1) : -1;
+}
+
 // Tile of a cell whose span is 1..3 bins on both axes; -1 for a tail cell.
 template <typename T>
 __device__ __forceinline__ int tile_key(const CellBins<T>& c, int nty) {
@@ -218,6 +239,8 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
         const int q = atomicAdd(ntail, 1);
         rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i);
       } else {
+        // per-lane atomics: this pass is bound by its loads, and wave
+        // aggregation (as in k_bin_fill) measured slower here
         atomicAdd(&hist[k], 1);
       }
     }
@@ -360,8 +383,8 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
         c[m] = bins_of(a, cur[m]);
         node[m] = (int)node_of(t, n_mov, fill_begin);
         key[m] = tile_key(c[m], nty);
-        if (key[m] >= 0) rank[m] = atomicAdd(&s_hist[key[m]], 1);
       }
+      rank[m] = wave_counter_inc(s_hist, key[m]);
     }
     __syncthreads();
     block_exclusive_scan(s_hist, s_off, ntiles, s_wsum);

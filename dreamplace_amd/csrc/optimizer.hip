@@ -38,6 +38,57 @@ __global__ void k_nesterov_extrapolate(int64_t n, const T* __restrict__ v_k,
   }
 }
 
+// The die box move_boundary applies to movable and filler nodes
+// (move_boundary_cuda_kernel.cu:10-40): x in [xl, xh - size_x],
+// y in [yl, yh - size_y].  Uniform fillers take their size from
+// filler_sx/sy instead of the per-node arrays.
+template <typename T>
+struct Box {
+  const T* sx;
+  const T* sy;
+  T xl, yl, xh, yh, filler_sx, filler_sy;
+  int64_t num_nodes, num_movable, num_filler;
+  bool uniform_fillers;
+};
+
+// k_nesterov_extrapolate followed by move_boundary on v_kp1, in one pass:
+// v_kp1 is formed as above, then clamped to the box of its node.  Entries
+// of fixed nodes are left as formed, as move_boundary leaves them.
+template <typename T>
+__global__ void k_nesterov_extrapolate_box(int64_t n, const T* __restrict__ v_k,
+                                           const T* __restrict__ g_k, const T* __restrict__ u_k,
+                                           const T* __restrict__ alpha_p,
+                                           const T* __restrict__ a_p, T* u_kp1, T* v_kp1,
+                                           T* a_kp1_out, Box<T> box) {
+  const T a = *a_p;
+  const T a_kp1 = (T(1) + sqrt(T(4) * (a * a) + T(1))) / T(2);
+  const T coef = (a - T(1)) / a_kp1;
+  const T alpha = *alpha_p;
+  const int64_t N = box.num_nodes, fill_begin = N - box.num_filler;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 == 0 && a_kp1_out) *a_kp1_out = a_kp1;
+  for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T u = v_k[i] - alpha * g_k[i];
+    const T uk = u_k[i];
+    u_kp1[i] = u;
+    T v = u + coef * (u - uk);
+    const bool is_y = i >= N;
+    const int64_t j = is_y ? i - N : i;
+    if (j < box.num_movable || j >= fill_begin) {
+      const T lo = is_y ? box.yl : box.xl;
+      T size;
+      if (box.uniform_fillers && j >= fill_begin)
+        size = is_y ? box.filler_sy : box.filler_sx;
+      else
+        size = is_y ? box.sy[j] : box.sx[j];
+      const T hi = (is_y ? box.yh : box.xh) - size;
+      v = v < lo ? lo : v;
+      v = hi < v ? hi : v;
+    }
+    v_kp1[i] = v;
+  }
+}
+
 // partials of sum(da^2), sum(db^2), sum(da*db), da = a1 - a0, db = b1 - b0
 template <typename T>
 __global__ void __launch_bounds__(kReduceBlock) k_step_norms(int64_t n, const T* __restrict__ a1,
@@ -119,6 +170,43 @@ int dp_nesterov_extrapolate(int dtype, int64_t n, const void* v_k, const void* g
                        0, s, n, (const T*)v_k, (const T*)g_k, (const T*)u_k, (const T*)alpha_k,
                        (const T*)a_k, (T*)u_kp1, (T*)v_kp1, (T*)a_kp1);
     DP_LAUNCHED("k_nesterov_extrapolate");
+    return DP_OK;
+  });
+}
+
+int dp_nesterov_extrapolate_bounded(int dtype, int64_t num_nodes, const void* v_k, const void* g_k,
+                                    const void* u_k, const void* alpha_k, const void* a_k,
+                                    void* u_kp1, void* v_kp1, void* a_kp1,
+                                    const void* node_size_x, const void* node_size_y, double xl,
+                                    double yl, double xh, double yh, int64_t num_movable_nodes,
+                                    int64_t num_filler_nodes, const double* filler_size,
+                                    void* stream) {
+  DP_ARG(v_k && g_k && u_k && alpha_k && a_k && u_kp1 && v_kp1 && node_size_x && node_size_y,
+         "dp_nesterov_extrapolate_bounded: null pointer");
+  DP_ARG(num_nodes >= 0 && num_movable_nodes >= 0 && num_filler_nodes >= 0 &&
+             num_movable_nodes + num_filler_nodes <= num_nodes,
+         "dp_nesterov_extrapolate_bounded: bad node counts");
+  DP_ARG(a_kp1 != a_k, "dp_nesterov_extrapolate_bounded: a_kp1 must not alias a_k");
+  const int64_t n = 2 * num_nodes;
+  hipStream_t s = S(stream);
+  return DP_DISPATCH(dtype, "dp_nesterov_extrapolate_bounded", [&] {
+    Box<T> box;
+    box.sx = (const T*)node_size_x;
+    box.sy = (const T*)node_size_y;
+    box.xl = (T)xl;
+    box.yl = (T)yl;
+    box.xh = (T)xh;
+    box.yh = (T)yh;
+    box.uniform_fillers = filler_size != nullptr;
+    box.filler_sx = filler_size ? (T)filler_size[0] : T(0);
+    box.filler_sy = filler_size ? (T)filler_size[1] : T(0);
+    box.num_nodes = num_nodes;
+    box.num_movable = num_movable_nodes;
+    box.num_filler = num_filler_nodes;
+    hipLaunchKernelGGL(k_nesterov_extrapolate_box<T>, dim3(grid_for(n > 0 ? n : 1, 256)),
+                       dim3(256), 0, s, n, (const T*)v_k, (const T*)g_k, (const T*)u_k,
+                       (const T*)alpha_k, (const T*)a_k, (T*)u_kp1, (T*)v_kp1, (T*)a_kp1, box);
+    DP_LAUNCHED("k_nesterov_extrapolate_box");
     return DP_OK;
   });
 }

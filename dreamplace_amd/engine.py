@@ -58,6 +58,60 @@ class DeviceDesign:
                 x = x.to(dt)
             self.t[k] = x.to(self.device)
 
+    def uniform_fillers(self):
+        """Whether every filler has the same size (PlaceDB.py:917-936 makes
+        them so); only then may their order change."""
+        n, nf = self.num_nodes, self.num_filler_nodes
+        if nf == 0:
+            return True
+        sx, sy = self.node_size_x[n - nf:], self.node_size_y[n - nf:]
+        return bool((sx == sx[0]).all().item() and (sy == sy[0]).all().item())
+
+    def reorder_fillers(self, perm=None):
+        """Relabel the fillers so that those in one 64x64-bin tile of the
+        density kernels are consecutive: the tiled scatter's records and the
+        field gather's gradient stores then run in node order instead of
+        scattering.  Fillers are interchangeable (one size, no pins), so the
+        objective is unchanged and each filler's gradient moves with it.
+        `perm` (filler-relative, new slot -> old slot) defaults to the tile
+        order of the current positions.  Applies it to `pos`, keeps the
+        accumulated relabelling in `filler_perm` (slot -> generated filler)
+        and returns `perm` for the caller's other per-node vectors
+        (permute_fillers_)."""
+        n, nf = self.num_nodes, self.num_filler_nodes
+        if nf == 0:
+            return None
+        if not self.uniform_fillers():
+            raise ValueError("reorder_fillers: fillers of different sizes are not interchangeable")
+        if perm is None:
+            perm = filler_tile_order(self.pos, n, nf, self.xl, self.yl, self.bin_size_x,
+                                     self.bin_size_y, self.num_bins_x, self.num_bins_y)
+        perm = perm.to(device=self.device, dtype=torch.int64)
+        permute_fillers_(self.t["pos"], n, nf, perm)
+        fp = self.__dict__.get("filler_perm")
+        self.filler_perm = perm.clone() if fp is None else fp[perm]
+        return perm
+
+    def current_order(self, v):
+        """A copy of a [2N] per-node vector given in the generation order,
+        with the fillers in the current order."""
+        out = v.clone()
+        fp = self.__dict__.get("filler_perm")
+        if fp is not None:
+            permute_fillers_(out, self.num_nodes, self.num_filler_nodes, fp)
+        return out
+
+    def generation_order(self, v):
+        """A copy of a [2N] per-node vector with the fillers back in the
+        order the design was generated in."""
+        fp = self.__dict__.get("filler_perm")
+        if fp is None:
+            return v.clone()
+        n, nf = self.num_nodes, self.num_filler_nodes
+        out = v.clone()
+        out.view(2, n)[:, n - nf:][:, fp] = v.view(2, n)[:, n - nf:]
+        return out
+
     def __getattr__(self, k):
         t = self.__dict__.get("t", {})
         if k in t:
@@ -66,6 +120,28 @@ class DeviceDesign:
         if k in s:
             return s[k]
         raise AttributeError(k)
+
+
+def filler_tile_order(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_y, num_bins_x,
+                      num_bins_y, tile=64):
+    """Filler-relative permutation (new slot -> old slot) that sorts the
+    fillers of a [2N] position vector by the tile of their lower-left bin,
+    tile-major as the tiled density kernels number tiles; stable, so fillers
+    within a tile keep their relative order."""
+    n, f0 = int(num_nodes), int(num_nodes) - int(num_fillers)
+    x, y = pos[f0:n], pos[n + f0:2 * n]
+    bx = torch.floor((x - xl) / bin_size_x).clamp_(0, num_bins_x - 1).to(torch.int64) // tile
+    by = torch.floor((y - yl) / bin_size_y).clamp_(0, num_bins_y - 1).to(torch.int64) // tile
+    nty = (int(num_bins_y) + tile - 1) // tile
+    return torch.argsort(bx * nty + by, stable=True)
+
+
+def permute_fillers_(v, num_nodes, num_fillers, perm):
+    """In place: v's filler entries (both halves of a [2N] vector) take the
+    order perm (filler-relative, new slot -> old slot)."""
+    n, f0 = int(num_nodes), int(num_nodes) - int(num_fillers)
+    v2 = v.view(2, n)
+    v2[:, f0:] = v2[:, f0:][:, perm]
 
 
 def check_netlist(flat_netpin, netpin_start, pin2net, flat_node2pin, node2pin_start, num_pins,

@@ -31,7 +31,13 @@ class _Norms:
 
 class NesterovAcceleratedGradientOptimizer(Optimizer):
     def __init__(self, params, lr=required, obj_and_grad_fn=required, constraint_fn=None,
-                 use_bb=True):
+                 use_bb=True, bounds=None):
+        """bounds (new, optional): the move_boundary box as a dict with
+        node_size_x, node_size_y (device), xl, yl, xh, yh, num_movable_nodes,
+        num_filler_nodes and filler_size ((w, h) shared by all fillers, or
+        None).  The extrapolation then clamps v_kp1 in the same pass
+        (dp_nesterov_extrapolate_bounded) and constraint_fn, which must be
+        that same clamp, is not called after it."""
         if lr is not required and lr < 0.0:
             raise ValueError("Invalid learning rate: {}".format(lr))
         defaults = dict(lr=lr, u_k=[], v_k=[], g_k=[], obj_k=[], a_k=[], alpha_k=[], v_k_1=[],
@@ -40,6 +46,7 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         self.obj_and_grad_fn = obj_and_grad_fn
         self.constraint_fn = constraint_fn
         self.use_bb = use_bb
+        self.bounds = bounds
         if len(self.param_groups) != 1 or len(self.param_groups[0]["params"]) != 1:
             raise ValueError("Only parameters with single tensor is supported")
         self._norms = None
@@ -58,13 +65,23 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
             self._a_kp1 = torch.empty(1, dtype=p.dtype, device=p.device)
 
     def _constrain(self, x):
-        if self.constraint_fn is not None:
+        if self.constraint_fn is not None and self.bounds is None:
             self.constraint_fn(x)
 
     def _extrapolate(self, v_k, g_k, u_k, alpha, a_k, v_kp1):
-        N.call("dp_nesterov_extrapolate", N.dtype_code(v_k), v_k.numel(), N.ptr(v_k), N.ptr(g_k),
-               N.ptr(u_k), N.ptr(alpha), N.ptr(a_k), N.ptr(self._u_kp1), N.ptr(v_kp1),
-               N.ptr(self._a_kp1), N.stream_ptr(v_k.device))
+        b = self.bounds
+        if b is None:
+            N.call("dp_nesterov_extrapolate", N.dtype_code(v_k), v_k.numel(), N.ptr(v_k),
+                   N.ptr(g_k), N.ptr(u_k), N.ptr(alpha), N.ptr(a_k), N.ptr(self._u_kp1),
+                   N.ptr(v_kp1), N.ptr(self._a_kp1), N.stream_ptr(v_k.device))
+            return
+        fs = b.get("filler_size")
+        N.call("dp_nesterov_extrapolate_bounded", N.dtype_code(v_k), v_k.numel() // 2, N.ptr(v_k),
+               N.ptr(g_k), N.ptr(u_k), N.ptr(alpha), N.ptr(a_k), N.ptr(self._u_kp1),
+               N.ptr(v_kp1), N.ptr(self._a_kp1), N.ptr(b["node_size_x"]), N.ptr(b["node_size_y"]),
+               float(b["xl"]), float(b["yl"]), float(b["xh"]), float(b["yh"]),
+               int(b["num_movable_nodes"]), int(b["num_filler_nodes"]),
+               N.host_doubles(fs) if fs is not None else None, N.stream_ptr(v_k.device))
 
     def step_nobb(self, closure=None):
         """Lines 65-166: Lipschitz step size with backtracking (at most 10
@@ -122,6 +139,19 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         g_k_1.copy_(g_kp1)
         group["u_k"][0], self._u_kp1 = self._u_kp1, u_k
         return loss
+
+    def permute_state_(self, fn, include_param=True):
+        """Apply an in-place relabelling fn(v) of the nodes to the parameter
+        (unless the caller has) and to every per-node vector the state
+        carries from one step to the next (u_k, g_k, v_k_1, g_k_1).  The
+        extrapolation's outputs (v_kp1 and u_kp1) are written whole before
+        they are read, so they need none."""
+        group = self.param_groups[0]
+        if include_param:
+            fn(group["params"][0].data)
+        for key in ("u_k", "g_k", "v_k_1", "g_k_1"):
+            if group[key]:
+                fn(group[key][0])
 
     @staticmethod
     def _rotate_positions(group, v_k, v_kp1):

@@ -24,13 +24,14 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .engine import DeviceDesign, PlacementObjective
+from .engine import DeviceDesign, PlacementObjective, permute_fillers_
 from .nesterov import NesterovAcceleratedGradientOptimizer
 
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
-                density_factor=1.0, early_stop=True)
+                density_factor=1.0, early_stop=True, filler_sort_interval=50,
+                fuse_move_boundary=True)
 
 
 def noise(d, ratio, seed):
@@ -96,7 +97,15 @@ class GlobalPlacer:
     (precision, algorithm, schedule) go to PlacementObjective.  `params`
     override DEFAULTS; early_stop=False skips the stop tests and the
     divergence roll-back, so a run takes exactly the iterations asked for
-    (to compare runs step for step)."""
+    (to compare runs step for step).
+
+    filler_sort_interval > 0: the (interchangeable) fillers are kept in the
+    tile order of the density kernels, sorted at set-up and by `run` every
+    that many iterations (DeviceDesign.reorder_fillers).  Positions
+    are then in that order; `positions()` gives them in the design's.
+    fuse_move_boundary: the optimizer clamps its extrapolated positions to
+    the die in the extrapolation pass instead of a move_boundary pass after
+    it (the same bits)."""
 
     def __init__(self, d, device, params=None, dtype=None, seed=0, **objective_kw):
         p = dict(DEFAULTS)
@@ -141,6 +150,42 @@ class GlobalPlacer:
         self.best_pos = None
         self.rolled_back = False
         self.perturbations = 0
+        self.optimizer = None
+        self.filler_sorts = 0
+        interval = int(p["filler_sort_interval"])
+        self.sorting = interval > 0 and dd.num_filler_nodes > 0 and dd.uniform_fillers()
+        if self.sorting:
+            self.sort_fillers()  # at set-up; `run` sorts again every `interval` iterations
+
+    def positions(self):
+        """The current positions with the fillers in the design's order."""
+        return self.dd.generation_order(self.pos.data)
+
+    def sort_fillers(self):
+        """Relabel the fillers into the density kernels' tile order: the
+        positions (the optimizer's parameter), the optimizer's per-node state
+        and the best positions take the same permutation."""
+        dd = self.dd
+        perm = dd.reorder_fillers()  # relabels dd.pos, which is self.pos
+        n, nf = dd.num_nodes, dd.num_filler_nodes
+        fn = lambda v: permute_fillers_(v, n, nf, perm)
+        if self.optimizer is not None:
+            self.optimizer.permute_state_(fn, include_param=False)
+        if self.best_pos is not None:
+            fn(self.best_pos)
+        self.filler_sorts += 1
+
+    def boundary_box(self):
+        """The move_boundary box of `constraint` for the optimizer's fused
+        extrapolation; fillers of one size pass that size instead of their
+        per-node entries."""
+        dd = self.dd
+        fs = None
+        if dd.num_filler_nodes > 0 and dd.uniform_fillers():
+            fs = (float(dd.node_size_x[-1].item()), float(dd.node_size_y[-1].item()))
+        return dict(node_size_x=dd.node_size_x, node_size_y=dd.node_size_y, xl=dd.xl, yl=dd.yl,
+                    xh=dd.xh, yh=dd.yh, num_movable_nodes=dd.num_movable_nodes,
+                    num_filler_nodes=dd.num_filler_nodes, filler_size=fs)
 
     # -- pieces of the reference loop ----------------------------------------
     def constraint(self, x):
@@ -238,7 +283,7 @@ class GlobalPlacer:
             x[n:] = (x[n:] - yc) * shrink_factor + yc
         if noise_intensity > 0.01:
             z = torch.from_numpy(injection_noise(n, self.seed, self.perturbations)).to(x.device, x.dtype)
-            x.add_(noise_intensity * z)
+            x.add_(noise_intensity * dd.current_order(z))  # each filler keeps its draw
         x[m0:m1] = fx
         x[n + m0:n + m1] = fy
 
@@ -247,10 +292,11 @@ class GlobalPlacer:
         """At most `iterations` steps; returns (positions, metrics before each
         step).  `log_every` > 0 prints every that many iterations' metrics."""
         p, x = self.p, self.pos
+        interval, sorting = int(p["filler_sort_interval"]), self.sorting
         lr = self.estimate_learning_rate(x)
         self.optimizer = NesterovAcceleratedGradientOptimizer(
             [x], lr=lr, obj_and_grad_fn=self.obj_and_grad, constraint_fn=self.constraint,
-            use_bb=p["use_bb"])
+            use_bb=p["use_bb"], bounds=self.boundary_box() if p["fuse_move_boundary"] else None)
         hist = self.history
         T = np.float32 if self.dd.dtype == torch.float32 else np.float64
         overflow_list, divergence_list = [1], []
@@ -260,6 +306,8 @@ class GlobalPlacer:
         # v_kp1, already constrained) need no second pass
         constrained = False
         for it in range(iterations):
+            if sorting and it > 0 and it % interval == 0:
+                self.sort_fillers()
             if (p["early_stop"] and p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
                     and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
                                          threshold=overflow_list[-1])):

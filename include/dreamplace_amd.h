@@ -443,6 +443,22 @@ int dp_nesterov_extrapolate(int dtype, int64_t n, const void* v_k, const void* g
                             const void* u_k, const void* alpha_k, const void* a_k, void* u_kp1,
                             void* v_kp1, void* a_kp1, void* stream);
 
+/* dp_nesterov_extrapolate over [2*num_nodes] positions followed by the
+ * optimizer's constraint, move_boundary on v_kp1
+ * (NesterovAcceleratedGradientOptimizer.py:123,212 with PlaceObj's
+ * constraint_fn; move_boundary_cuda_kernel.cu:10-40), in one pass: movable
+ * and filler nodes are clamped to x in [xl, xh - node_size_x],
+ * y in [yl, yh - node_size_y].  The same bits as the two calls.
+ * filler_size (host double[2], nullable): the one size all fillers share,
+ * read instead of their node_size entries. */
+int dp_nesterov_extrapolate_bounded(int dtype, int64_t num_nodes, const void* v_k, const void* g_k,
+                                    const void* u_k, const void* alpha_k, const void* a_k,
+                                    void* u_kp1, void* v_kp1, void* a_kp1,
+                                    const void* node_size_x, const void* node_size_y, double xl,
+                                    double yl, double xh, double yh, int64_t num_movable_nodes,
+                                    int64_t num_filler_nodes, const double* filler_size,
+                                    void* stream);
+
 /* out3 (device double[3]) = {sum((a1-a0)^2), sum((b1-b0)^2), sum((a1-a0)(b1-b0))}:
  * differences in `dtype`, squares and sums in FP64, deterministic.  With
  * a = positions and b = gradients these give the Lipschitz step

@@ -171,3 +171,24 @@ def test_engine_overlap_is_deterministic(gpu, dtype):
     for f, g in out[1:]:
         assert f == out[0][0]
         assert torch.equal(g, out[0][1])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_engine_filler_tile_order(gpu, dtype):
+    """Fillers relabelled into tile order: the same objective bits, and the
+    same gradient bits once put back in the generation order."""
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case engine filler tile order %s" % np.dtype(dtype).name, flush=True)
+    d = design(1, dtype=dtype, scale=0.01)  # 102 x 102 bins: 2 x 2 tiles
+    out = []
+    for sort in (False, True):
+        dd = DeviceDesign(d, gpu)
+        if sort:
+            perm = dd.reorder_fillers()
+            assert not torch.equal(perm.cpu(), torch.arange(d.num_filler_nodes))
+        ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
+                                else "fp64", algorithm="tiled", schedule="pins")
+        f, g = ob.evaluate(dd.pos)
+        out.append((f.item(), dd.generation_order(g)))
+    assert out[0][0] == out[1][0]
+    assert torch.equal(out[0][1], out[1][1])

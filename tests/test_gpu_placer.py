@@ -48,12 +48,13 @@ def test_placer_matches_oracle(gpu):
     gp = GlobalPlacer(d, gpu, dtype=torch.float64, precision="fp64", algorithm="tiled",
                       schedule="pins")
     x, hist = gp.run(8)
+    assert gp.filler_sorts == 1  # at set-up: the loop keeps the fillers in tile order
     xr, hr, _ = OP.place(d, 8)
     assert len(hist) == len(hr)
     for h, r in zip(hist, hr):
         for k in ("hpwl", "overflow", "max_density", "density_weight", "gamma"):
             assert h[k] == pytest.approx(r[k], rel=1e-9), k
-    assert rel_l2(x.cpu().numpy(), xr) <= 1e-9
+    assert rel_l2(gp.positions().cpu().numpy(), xr) <= 1e-9
 
 
 def test_placer_fp32_tracks_fp64(gpu):
@@ -86,7 +87,7 @@ def test_placer_quadratic_penalty_matches_oracle(gpu):
     for h, r in zip(hist, hr):
         for k in ("hpwl", "overflow", "density_weight"):
             assert h[k] == pytest.approx(r[k], rel=1e-9), k
-    assert rel_l2(x.cpu().numpy(), xr) <= 1e-9
+    assert rel_l2(gp.positions().cpu().numpy(), xr) <= 1e-9
     assert gp.best["overflow"] == pytest.approx(info["best"]["overflow"], rel=1e-9)
 
 
@@ -99,11 +100,12 @@ def test_entropy_injection_matches_oracle(gpu):
     d = design(0)
     gp = GlobalPlacer(d, gpu, dtype=torch.float64, precision="fp64", algorithm="tiled",
                       schedule="pins")
-    x = gp.pos.clone()
-    ref = x.cpu().numpy().copy()
+    x = gp.pos.clone()  # fillers in tile order (sorted at set-up)
+    ref = gp.dd.generation_order(x).cpu().numpy()
     gp.entropy_injection(x, 0.996, 47.5)
     OP.entropy_injection(gp.d, ref, 0.996, 47.5, OP.injection_noise(d.num_nodes, 0, 0))
-    assert np.allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-9)
+    # each filler keeps the draw of its generated index
+    assert np.allclose(gp.dd.generation_order(x).cpu().numpy(), ref, rtol=1e-13, atol=1e-9)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
@@ -130,3 +132,46 @@ def test_fused_preconditioner_equals_separate(gpu, dtype):
            N.stream_ptr(gpu))
     _, g2 = ob.evaluate(dd.pos, precondition=(pws, areas, 1.0))
     assert torch.equal(g2, sep)
+
+
+def test_filler_sorting_keeps_the_loop(gpu):
+    """Relabelling the fillers into tile order every 3 iterations (with the
+    optimizer state and the best positions) leaves the FP64 loop on config 0
+    where it is without relabelling: only FP64 summation orders change."""
+    from dreamplace_amd.placer import GlobalPlacer
+    print("case placer filler sorting", flush=True)
+    d = design(0)
+    runs = []
+    for interval in (0, 3):
+        gp = GlobalPlacer(d, gpu, params=dict(filler_sort_interval=interval, early_stop=False),
+                          dtype=torch.float64, precision="fp64", algorithm="tiled", schedule="pins")
+        _, hist = gp.run(10)
+        runs.append((gp, hist))
+    assert runs[0][0].filler_sorts == 0 and runs[1][0].filler_sorts == 4
+    for h0, h1 in zip(runs[0][1], runs[1][1]):
+        for k in ("hpwl", "overflow", "density_weight"):
+            assert h1[k] == pytest.approx(h0[k], rel=1e-9), k
+    p0, p1 = (gp.positions().cpu().numpy() for gp, _ in runs)
+    assert rel_l2(p1, p0) <= 1e-9
+    # the relabelled run's fillers are in tile order (up to the moves since the last sort)
+    assert not torch.equal(runs[1][0].pos, runs[1][0].positions())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_fused_move_boundary_is_bit_identical(gpu, dtype):
+    """The optimizer's extrapolation with the die clamp fused in gives the
+    same bits as extrapolation followed by move_boundary: identical metric
+    histories and positions over 12 iterations on config 0."""
+    from dreamplace_amd.placer import GlobalPlacer
+    print("case placer fused move_boundary %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype)
+    runs = []
+    for fuse in (False, True):
+        gp = GlobalPlacer(d, gpu, params=dict(fuse_move_boundary=fuse, early_stop=False),
+                          precision="fp64" if dtype == np.float64 else "bf16x3", algorithm="tiled",
+                          schedule="pins")
+        _, hist = gp.run(12)
+        runs.append((gp.pos.clone(), hist))
+    assert torch.equal(runs[0][0], runs[1][0])
+    for h0, h1 in zip(runs[0][1], runs[1][1]):
+        assert h0 == h1

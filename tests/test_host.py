@@ -226,3 +226,28 @@ def test_placer_control_logic_matches_oracle():
         assert p["upper_pcof"] * p["lower_pcof"] <= mu <= p["upper_pcof"] ** 2
     assert P.density_weight_factor(1e9, 0.0, 1, p) == pytest.approx(p["upper_pcof"] * p["lower_pcof"])
     assert P.density_weight_factor(0.0, 1.0, 1, p) == pytest.approx(p["upper_pcof"] * 0.9999)
+
+
+def test_filler_tile_order_on_host():
+    """DeviceDesign.reorder_fillers on CPU tensors: a permutation of the
+    fillers only, tile keys non-decreasing afterwards, and the generation
+    order recovered through any number of relabellings."""
+    import torch
+    from dreamplace_amd.engine import DeviceDesign
+    d = synth.make_design(0)
+    dd = DeviceDesign(d, "cpu")
+    n, nf = d.num_nodes, d.num_filler_nodes
+    pos0 = dd.pos.clone()
+    perm = dd.reorder_fillers()
+    assert sorted(perm.tolist()) == list(range(nf))
+    f0 = n - nf
+    assert torch.equal(dd.pos[:f0], pos0[:f0]) and torch.equal(dd.pos[n:n + f0], pos0[n:n + f0])
+    bx = torch.floor((dd.pos[f0:n] - d.xl) / d.bin_size_x).clamp(0, d.num_bins_x - 1).long() // 64
+    by = torch.floor((dd.pos[n + f0:] - d.yl) / d.bin_size_y).clamp(0, d.num_bins_y - 1).long() // 64
+    key = bx * ((d.num_bins_y + 63) // 64) + by
+    assert bool((key[1:] >= key[:-1]).all())
+    assert torch.equal(dd.generation_order(dd.pos), pos0)
+    assert torch.equal(dd.current_order(pos0), dd.pos)
+    dd.reorder_fillers(torch.randperm(nf, generator=torch.Generator().manual_seed(3)))
+    assert torch.equal(dd.generation_order(dd.pos), pos0)
+    assert torch.equal(dd.current_order(pos0), dd.pos)
