@@ -480,7 +480,8 @@ def main():
 
     if args.summarize_traffic:
         wl = ("config1: 2200000 movable cells, 25000 fixed, 10924750 fillers, 2200000 nets, "
-              "9623921 pins, 1024x1024 bins; FP32 + bf16x3 solve, pins schedule, tiled density")
+              "9623921 pins, 1024x1024 bins; FP32 + bf16x3 solve, pins schedule, tiled density; "
+              "fillers in tile order")
         print(json.dumps(summarize_traffic(*args.summarize_traffic, workload=wl), indent=1))
         return
     if args.summarize_trace:
