@@ -37,6 +37,8 @@ _SIGNATURES = {
     "dp_density_map_workspace_bytes": (_S, [_L, _L, _L, _L, _I, _I]),
     "dp_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L,
                             _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
+    "dp_density_map_movable": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D,
+                                    _L, _L, _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
@@ -70,13 +72,14 @@ _SIGNATURES = {
                                 _P, _P]),
     "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
     "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
-                                _P, _P, _P, _P, _P, _P]),
+                                _P, _P, _P, _P, _P, _P, _P]),
     "dp_lse_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
-                                 _P, _P, _P, _P, _P, _P]),
+                                 _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_pins_slot_weights": (_I, [_I, _L, _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
     "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
+    "dp_hpwl_from_nets": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_optimizer_workspace_bytes": (_S, []),

@@ -133,14 +133,19 @@ __global__ void k_quantize(const T* init, int64_t n, T scale, unsigned long long
 
 // CPU-path conversion: out = T(double(init) + double(sum) * (1.0/scale))
 // (atomic_ops.h:48-54 scaleAdd on a clone of the initial map).
+// buf2 (nullable): a second int64 map (the fillers') added to buf for `out`;
+// out_part (nullable) receives the conversion of buf alone.
 template <typename T>
-__global__ void k_convert_cpu(const unsigned long long* buf, const T* init, int64_t n,
-                              double inv_scale, T* out, long long* raw) {
+__global__ void k_convert_cpu(const unsigned long long* buf, const unsigned long long* buf2,
+                              const T* init, int64_t n, double inv_scale, T* out, T* out_part,
+                              long long* raw) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    long long s = (long long)buf[i];
+    const unsigned long long b = buf[i];
+    long long s = (long long)(buf2 ? b + buf2[i] : b);
     double base = init ? (double)init[i] : 0.0;
     if (out) out[i] = (T)(base + (double)s * inv_scale);
+    if (out_part) out_part[i] = (T)(base + (double)(long long)b * inv_scale);
     if (raw) raw[i] = s;
   }
 }
@@ -148,12 +153,15 @@ __global__ void k_convert_cpu(const unsigned long long* buf, const T* init, int6
 // CUDA-path conversion: out = T(u64) * T(1.0/scale).  raw receives the sum
 // without the quantised initial map.
 template <typename T>
-__global__ void k_convert_gpu(const unsigned long long* buf, const T* init, int64_t n, T scale,
-                              T inv_scale, T* out, long long* raw) {
+__global__ void k_convert_gpu(const unsigned long long* buf, const unsigned long long* buf2,
+                              const T* init, int64_t n, T scale, T inv_scale, T* out,
+                              T* out_part, long long* raw) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    unsigned long long s = buf[i];
+    const unsigned long long b = buf[i];
+    unsigned long long s = buf2 ? b + buf2[i] : b;
     if (out) out[i] = (T)s * inv_scale;
+    if (out_part) out_part[i] = (T)b * inv_scale;
     if (raw) {
       unsigned long long q = init ? (unsigned long long)(init[i] * scale) : 0ull;
       raw[i] = (long long)(s - q);
@@ -205,6 +213,7 @@ __global__ void __launch_bounds__(kReduceBlock) k_overflow_partial(const T* d, i
 // ---------------------------------------------------------------------------
 struct DensityWorkspace {
   unsigned long long* map;
+  unsigned long long* filler_map;
   int* qcount;
   int* qlist;
   char* tiled;
@@ -214,6 +223,7 @@ static DensityWorkspace carve(void* ws, int64_t nb, int64_t qcap) {
   char* p = (char*)ws;
   DensityWorkspace w;
   w.map = (unsigned long long*)p;
+  w.filler_map = (unsigned long long*)(p + density_map_bytes(nb));
   w.qcount = (int*)(p + density_queue_offset(nb));
   w.qlist = (int*)(p + density_queue_offset(nb) + 256);
   w.tiled = p + density_tiled_offset(nb, qcap);
@@ -304,14 +314,19 @@ size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, in
   return b;
 }
 
-int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
-                   const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
-                   const void* ratio, const void* initial_density_map, double xl, double yl,
-                   double xh, double yh, double bin_size_x, double bin_size_y,
-                   int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
-                   int num_bins_y, int convention, int algorithm, int64_t large_capacity,
-                   const double* uniform_filler, void* workspace, void* density_map_out,
-                   int64_t* fixed_map_out, void* stream) {
+// dp_density_map, and with movable_map_out the map of the fixed and movable
+// cells alone (the fillers scattered to their own int64 map) from the same
+// scatter.
+static int density_map_impl(const char* name, int dtype, const void* pos, int64_t num_nodes,
+                            const void* node_size_x_clamped, const void* node_size_y_clamped,
+                            const void* offset_x, const void* offset_y, const void* ratio,
+                            const void* initial_density_map, double xl, double yl, double xh,
+                            double yh, double bin_size_x, double bin_size_y,
+                            int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                            int num_bins_y, int convention, int algorithm,
+                            int64_t large_capacity, const double* uniform_filler,
+                            void* workspace, void* density_map_out, int64_t* fixed_map_out,
+                            void* movable_map_out, void* stream) {
   DP_ARG(pos && node_size_x_clamped && node_size_y_clamped && offset_x && offset_y && ratio,
          "dp_density_map: null input array");
   DP_ARG(workspace, "dp_density_map: null workspace");
@@ -327,7 +342,7 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
          "dp_density_map: bad convention %d", convention);
   DP_ARG(algorithm == DP_SCATTER_GLOBAL || algorithm == DP_SCATTER_TILED,
          "dp_density_map: bad algorithm %d", algorithm);
-  DP_ARG(density_map_out || fixed_map_out, "dp_density_map: no output");
+  DP_ARG(density_map_out || fixed_map_out || movable_map_out, "dp_density_map: no output");
   hipStream_t s = S(stream);
   DP_PROBE_OP(DP_OP_DENSITY_MAP, s);
   const int64_t nb = (int64_t)num_bins_x * num_bins_y;
@@ -346,27 +361,74 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
       hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map, nb, 0ull);
     }
     DP_LAUNCHED("density init");
+    unsigned long long* fmap = nullptr;
+    if (movable_map_out && num_filler_nodes > 0) {
+      // the fillers' contributions in a map of their own: w.map then holds the
+      // fixed and movable cells' alone, the map ElectricOverflow forms
+      fmap = w.filler_map;
+      hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
+                         fmap, nb, 0ull);
+      DP_LAUNCHED("filler map init");
+    }
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y)) {
       DP_CHECK_RC(scatter_tiled<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
-                                   num_filler_nodes, w.tiled, w.map, s));
+                                   num_filler_nodes, w.tiled, w.map, fmap, s));
     } else {
+      DP_REQUIRE(!fmap, DP_ERR_UNSUPPORTED,
+                 "%s: the movable map needs the tiled scatter", name);
       DP_CHECK_RC(scatter_global<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
                                     num_filler_nodes, w, large_capacity, s));
     }
     if (convention == DP_CONVERT_CPU) {
       DP_PROBED(DP_KERNEL_DENSITY_CONVERT, s,
                 hipLaunchKernelGGL(k_convert_cpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
-                                   w.map, init, nb, 1.0 / (double)scale, (T*)density_map_out,
-                                   (long long*)fixed_map_out));
+                                   w.map, (const unsigned long long*)fmap, init, nb,
+                                   1.0 / (double)scale, (T*)density_map_out,
+                                   (T*)movable_map_out, (long long*)fixed_map_out));
     } else {
       DP_PROBED(DP_KERNEL_DENSITY_CONVERT, s,
                 hipLaunchKernelGGL(k_convert_gpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
-                                   w.map, init, nb, (T)scale, (T)(1.0 / (double)scale),
-                                   (T*)density_map_out, (long long*)fixed_map_out));
+                                   w.map, (const unsigned long long*)fmap, init, nb, (T)scale,
+                                   (T)(1.0 / (double)scale), (T*)density_map_out,
+                                   (T*)movable_map_out, (long long*)fixed_map_out));
     }
     DP_LAUNCHED("density convert");
     return DP_OK;
   });
+}
+
+int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
+                   const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
+                   const void* ratio, const void* initial_density_map, double xl, double yl,
+                   double xh, double yh, double bin_size_x, double bin_size_y,
+                   int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                   int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                   const double* uniform_filler, void* workspace, void* density_map_out,
+                   int64_t* fixed_map_out, void* stream) {
+  return density_map_impl("dp_density_map", dtype, pos, num_nodes, node_size_x_clamped,
+                          node_size_y_clamped, offset_x, offset_y, ratio, initial_density_map, xl,
+                          yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
+                          num_bins_x, num_bins_y, convention, algorithm, large_capacity,
+                          uniform_filler, workspace, density_map_out, fixed_map_out, nullptr,
+                          stream);
+}
+
+int dp_density_map_movable(int dtype, const void* pos, int64_t num_nodes,
+                           const void* node_size_x_clamped, const void* node_size_y_clamped,
+                           const void* offset_x, const void* offset_y, const void* ratio,
+                           const void* initial_density_map, double xl, double yl, double xh,
+                           double yh, double bin_size_x, double bin_size_y,
+                           int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                           int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                           const double* uniform_filler, void* workspace, void* density_map_out,
+                           void* movable_map_out, void* stream) {
+  DP_ARG(movable_map_out, "dp_density_map_movable: null movable_map_out");
+  return density_map_impl("dp_density_map_movable", dtype, pos, num_nodes, node_size_x_clamped,
+                          node_size_y_clamped, offset_x, offset_y, ratio, initial_density_map, xl,
+                          yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
+                          num_bins_x, num_bins_y, convention, algorithm, large_capacity,
+                          uniform_filler, workspace, density_map_out, nullptr, movable_map_out,
+                          stream);
 }
 
 int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x,

@@ -469,7 +469,8 @@ __device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& r
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TriArgs<T> a, int nty, const int4* work,
                                                       const int* nwork, Recs<T> rec,
-                                                      unsigned long long* map) {
+                                                      unsigned long long* map,
+                                                      unsigned long long* filler_map) {
   __shared__ unsigned long long acc[kTileH * kTileH];
   const int W = *nwork;
   for (int w = blockIdx.x; w < W; w += gridDim.x) {
@@ -509,11 +510,13 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TriArgs<T> a, int
       }
     }
     __syncthreads();
+    // a work item of uniform fillers goes to the filler map when there is one
+    unsigned long long* dst = (it.w != 0 && filler_map) ? filler_map : map;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) {
       const unsigned long long v = acc[k];
       if (v) {
         const int gx = tx0 + k / kTileH, gy = ty0 + k % kTileH;
-        atomicAdd(&map[(int64_t)gx * a.nby + gy], v);
+        atomicAdd(&dst[(int64_t)gx * a.nby + gy], v);
       }
     }
     __syncthreads();
@@ -595,7 +598,9 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
 template <typename T>
 __global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T> rec,
                                                            int64_t total, const int* ntail,
-                                                           unsigned long long* map) {
+                                                           unsigned long long* map,
+                                                           unsigned long long* filler_map,
+                                                           int64_t fill_begin) {
   const int cnt = *ntail;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
     const int64_t j = total - 1 - q;
@@ -603,12 +608,13 @@ __global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T>
     int bxl, nx, byl, ny;
     rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
     if (nx <= 0 || ny <= 0) continue;
+    unsigned long long* dst = (filler_map && rec.idx[j] >= fill_begin) ? filler_map : map;
     for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
       const int k = bxl + t / ny, h = byl + t % ny;
       T pxr = tri(x, sx, a.xl, k, a.bsx) * r;
       T area = pxr * tri(y, sy, a.yl, h, a.bsy);
       long long v = (long long)(area * a.scale);
-      if (v != 0) atomicAdd(&map[(int64_t)k * a.nby + h], (unsigned long long)v);
+      if (v != 0) atomicAdd(&dst[(int64_t)k * a.nby + h], (unsigned long long)v);
     }
   }
 }
@@ -655,11 +661,14 @@ __global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> 
 
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
-                  char* tiled_ws, unsigned long long* map, hipStream_t s) {
+                  char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
+                  hipStream_t s) {
   const int64_t total = n_mov + n_fill;
   TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled scatter: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
+  DP_REQUIRE(!filler_map || n_fill == 0 || a.uni, DP_ERR_UNSUPPORTED,
+             "tiled scatter: a separate filler map needs fillers of one size");
   int* tile_count = (int*)(tiled_ws + L.off_count);
   int* tile_start = (int*)(tiled_ws + L.off_start);
   int* wg_base = (int*)(tiled_ws + L.off_wgbase);
@@ -687,10 +696,11 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
     DP_LAUNCHED("k_bin_fill");
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
               hipLaunchKernelGGL(k_tile_scatter<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty,
-                                 (const int4*)work, (const int*)nwork, rec, map));
+                                 (const int4*)work, (const int*)nwork, rec, map,
+                                 filler_map));
     DP_LAUNCHED("k_tile_scatter");
     hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, rec, total,
-                       (const int*)ntail, map);
+                       (const int*)ntail, map, filler_map, fill_begin);
     DP_LAUNCHED("k_rec_scatter_large");
   }
   return DP_OK;
@@ -736,7 +746,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
 
 #define DP_TILED_INST(T)                                                                     \
   template int scatter_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, char*,        \
-                                unsigned long long*, hipStream_t);                           \
+                                unsigned long long*, unsigned long long*, hipStream_t);      \
   template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, const T*,      \
                                const T*, const T*, char*, int, T*, hipStream_t);
 DP_TILED_INST(float)

@@ -124,16 +124,21 @@ TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, co
 // Tiled (LDS-binned) scatter and gather, density_tiled.hip.
 bool tiled_supported(int nbx, int nby);
 size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype);
+// filler_map (nullable): uniform fillers' contributions go there instead of
+// `map`, which then holds the other cells' only (requires uniform fillers).
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
-                  char* tiled_ws, unsigned long long* map, hipStream_t s);
+                  char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
+                  hipStream_t s);
 template <typename T>
 int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
                  const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
                  hipStream_t s);
 
-// Density workspace layout: [int64 map][queue count (256 B)][queue][tiled region].
-inline size_t density_queue_offset(int64_t nb) { return align256(8 * (size_t)nb); }
+// Density workspace layout: [int64 map][int64 filler map][queue count (256 B)]
+// [queue][tiled region].
+inline size_t density_map_bytes(int64_t nb) { return align256(8 * (size_t)nb); }
+inline size_t density_queue_offset(int64_t nb) { return 2 * density_map_bytes(nb); }
 inline size_t density_tiled_offset(int64_t nb, int64_t qcap) {
   return density_queue_offset(nb) + 256 + align256(4 * (size_t)(qcap > 0 ? qcap : 1));
 }

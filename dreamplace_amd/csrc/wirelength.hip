@@ -422,6 +422,29 @@ __global__ void k_wa_pin_pos_bwd(const T* __restrict__ gix, const T* __restrict_
   gy[n] = sy;
 }
 
+// k_hpwl's weighted FP64 sum over per-net HPWL values computed elsewhere
+// (the pin-parallel WA pass): the same grid, loop order, mask test and
+// empty-net test, so the total is bit-identical to k_hpwl's on the positions
+// the values came from.
+template <typename T>
+__global__ void __launch_bounds__(256) k_hpwl_nets(const T* __restrict__ h_net,
+                                                   const int* __restrict__ netpin_start, int64_t E,
+                                                   const T* __restrict__ w,
+                                                   const uint8_t* __restrict__ mask,
+                                                   double* partials) {
+  __shared__ double smem[4];
+  double acc = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (!mask[e]) continue;
+    if (netpin_start[e + 1] <= netpin_start[e]) continue;
+    const T h = h_net[e];
+    acc += (double)(w ? h * w[e] : h);
+  }
+  double bs = block_sum<256>(acc, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
 static int reduce_grid(int64_t n) {
   int64_t g = ceil_div(n, 256);
   if (g < 1) g = 1;
@@ -671,6 +694,24 @@ int dp_hpwl(int dtype, const void* pin_pos, int64_t num_pins, const int32_t* fla
                        (const int*)flat_netpin, (const int*)netpin_start, num_nets,
                        (const T*)net_weights, net_mask, partials);
     DP_LAUNCHED("k_hpwl");
+    return finish_reduction(partials, grid, 1, nullptr, hpwl_out, -1, s);
+  });
+}
+
+int dp_hpwl_from_nets(int dtype, const void* net_hpwl, const int32_t* netpin_start,
+                      int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                      void* workspace, double* hpwl_out, void* stream) {
+  DP_ARG(net_hpwl && netpin_start && net_mask && workspace && hpwl_out,
+         "dp_hpwl_from_nets: null pointer");
+  DP_ARG(num_nets >= 0, "dp_hpwl_from_nets: bad size");
+  hipStream_t s = S(stream);
+  double* partials = (double*)workspace;
+  int grid = reduce_grid(num_nets);
+  return DP_DISPATCH(dtype, "dp_hpwl_from_nets", [&] {
+    hipLaunchKernelGGL(k_hpwl_nets<T>, dim3(grid), dim3(256), 0, s, (const T*)net_hpwl,
+                       (const int*)netpin_start, num_nets, (const T*)net_weights, net_mask,
+                       partials);
+    DP_LAUNCHED("k_hpwl_nets");
     return finish_reduction(partials, grid, 1, nullptr, hpwl_out, -1, s);
   });
 }

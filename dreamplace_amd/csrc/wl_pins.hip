@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
     const uint8_t* __restrict__ net_mask, const T* __restrict__ slot_w,
-    const T* __restrict__ g_p, T* partial, T* gnode, double* partials) {
+    const T* __restrict__ g_p, T* partial, T* net_hpwl, T* gnode, double* partials) {
   typedef typename Vec2<T>::type V;
   // per-lane words read by the net's first lane: one LDS read per pin for
   // (x, y) and one for its four exponentials
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     wave_sync();
     const bool is_head = on && lane == head;
     T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
-    if (is_head && live) {
+    if (is_head && (live || net_hpwl)) {
       for (int k = head; k < head + deg; ++k) {
         const V v = s_xy[k];
         const T xx = v.x, yy = v.y;
@@ -224,6 +224,8 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       s_c[1][lane] = xmin;
       s_c[2][lane] = ymax;
       s_c[3][lane] = ymin;
+      // the net's HPWL, in k_hpwl's expression order, for every net
+      if (net_hpwl) net_hpwl[e] = xmax - xmin + ymax - ymin;
     }
     wave_sync();
     T ex = 0, enx = 0, ey = 0, eny = 0;
@@ -316,8 +318,8 @@ template <typename T, bool kLse>
 __global__ void __launch_bounds__(256) k_wa_big(
     const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int4* __restrict__ big, int nbig, const T* __restrict__ w,
-    const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial, T* gnode,
-    double* partials) {
+    const uint8_t* __restrict__ net_mask, const T* __restrict__ g_p, T* partial, T* net_hpwl,
+    T* gnode, double* partials) {
   __shared__ double s_red[4];
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -341,6 +343,7 @@ __global__ void __launch_bounds__(256) k_wa_big(
     ymax = wave_max(ymax);
     xmin = -wave_max(-xmin);
     ymin = -wave_max(-ymin);
+    if (net_hpwl && lane == 0) net_hpwl[e] = xmax - xmin + ymax - ymin;
     T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
     for (int j = pb + lane; j < pe; j += 64) {
       const typename Vec2<T>::type v = pin_xy[j];
@@ -572,8 +575,8 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
                        const uint8_t* net_mask, const void* slot_weights, const void* g_p,
-                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
-                       void* stream) {
+                       void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
+                       void* workspace, void* stream) {
   DP_ARG(pos && slot_node && node_offsets && node_dst && pin_meta &&
              (group_start || num_groups == 0) && (big_nets || num_big == 0) && net_mask &&
              g_p && grad_node_order && wl_out && workspace,
@@ -609,12 +612,13 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
               hipLaunchKernelGGL(packed, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
                                  (const PinMeta*)pin_meta, (const int*)group_start, (int)num_groups,
                                  (const T*)net_weights, net_mask, (const T*)slot_weights,
-                                 (const T*)g_p, (T*)partial_wl, (T*)grad_node_order, partials));
+                                 (const T*)g_p, (T*)partial_wl, (T*)net_hpwl,
+                                 (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
     hipLaunchKernelGGL(bignet, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
                        (const PinMeta*)pin_meta, (const int4*)big_nets, (int)num_big,
                        (const T*)net_weights, net_mask, (const T*)g_p, (T*)partial_wl,
-                       (T*)grad_node_order, partials + g1);
+                       (T*)net_hpwl, (T*)grad_node_order, partials + g1);
     DP_LAUNCHED("k_wa_big");
     return finish_reduction(partials, g1 + g2, 1, nullptr, wl_out, dtype, s);
   });
@@ -626,12 +630,12 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
                        const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
-                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
-                       void* stream) {
+                       void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
+                       void* workspace, void* stream) {
   return forward_pins(false, "dp_wa_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
                       num_nets, net_weights, net_mask, slot_weights, inv_gamma, partial_wl,
-                      grad_node_order, wl_out, workspace, stream);
+                      net_hpwl, grad_node_order, wl_out, workspace, stream);
 }
 
 int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
@@ -640,12 +644,12 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                         const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                         int64_t num_big, int64_t num_nets, const void* net_weights,
                         const uint8_t* net_mask, const void* slot_weights, const void* gamma,
-                        void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
-                        void* stream) {
+                        void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
+                        void* workspace, void* stream) {
   return forward_pins(true, "dp_lse_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
                       num_nets, net_weights, net_mask, slot_weights, gamma, partial_wl,
-                      grad_node_order, wl_out, workspace, stream);
+                      net_hpwl, grad_node_order, wl_out, workspace, stream);
 }
 
 int dp_wa_pins_slot_weights(int dtype, int64_t num_pins, int32_t* pin_meta,

@@ -217,14 +217,23 @@ class PinPlan:
 class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
                  energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
-                 keep_partial_wl=False, wirelength="weighted_average", overlap=True):
+                 keep_partial_wl=False, wirelength="weighted_average", overlap=True,
+                 track_metrics=False):
         """keep_partial_wl: also store the per-net (pre-weight) wirelength in
         `partial_wl` (what the net-weight adjoint needs); the pins schedule
         skips that store otherwise.  wirelength: "weighted_average" or
         "logsumexp" (PlaceObj.py:467-503); the log-sum-exp model runs on the
         "pins" and "unfused" schedules.  overlap (pins schedule): run the
         wirelength forward on a second stream, concurrently with the density
-        chain it does not depend on; the node-gradient pass joins them."""
+        chain it does not depend on; the node-gradient pass joins them.
+        track_metrics (pins schedule, tiled density, fillers of one size, no
+        padding): every evaluation also leaves the placer's metrics at its
+        positions in `metric_out` (FP64 {HPWL, overflow cost, max bin
+        density}): the HPWL from the wirelength pass's per-net extremes, the
+        overflow from the same scatter's fixed + movable map.  Both are
+        bit-identical to dp_hpwl and to ElectricOverflow's own scatter.
+        `metric_ptr` is the address of those positions; a caller that changes
+        them in place clears it."""
         if wirelength not in ("weighted_average", "logsumexp"):
             raise ValueError("wirelength must be weighted_average or logsumexp, got %r" % (wirelength,))
         self.lse = wirelength == "logsumexp"
@@ -287,6 +296,21 @@ class PlacementObjective:
         self.side = None
         self._side_stream = None
         self.set_overlap(overlap)
+        self.track_metrics = False
+        self.metric_ptr = None
+        if track_metrics:
+            ep = self.density
+            tiles = ((M + 63) // 64) * ((Nb + 63) // 64)
+            if not (self.schedule == "pins" and ep.algorithm == "tiled" and tiles <= 2048
+                    and ep.padding == 0 and (dd.num_filler_nodes == 0 or ep.use_uniform_filler)):
+                raise ValueError("track_metrics needs the pins schedule, the tiled density "
+                                 "algorithm (at most 2048 tiles), no padding and fillers of one size")
+            self.track_metrics = True
+            self.net_hpwl = z(E)
+            self.movable_map = z(M, Nb)
+            self.metric_out = torch.empty(3, dtype=torch.float64, device=dev)
+            self.metric_ws = torch.empty(N.query("dp_wa_workspace_bytes", E), dtype=torch.uint8,
+                                         device=dev)
 
     def set_overlap(self, on):
         """Run the pins schedule's wirelength forward on a second stream
@@ -342,7 +366,8 @@ class PlacementObjective:
     # -- stages ---------------------------------------------------------------
     def _density_forward(self, pos):
         ep = self.density
-        N.call("dp_density_map", N.dtype_code(pos), N.ptr(pos), pos.numel() // 2,
+        N.call("dp_density_map_movable" if self.track_metrics else "dp_density_map",
+               N.dtype_code(pos), N.ptr(pos), pos.numel() // 2,
                N.ptr(ep.node_size_x_clamped), N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x),
                N.ptr(ep.offset_y), N.ptr(ep.ratio), N.ptr(ep.initial_density_map), float(ep.xl),
                float(ep.yl), float(ep.xh), float(ep.yh), float(ep.bin_size_x),
@@ -350,7 +375,8 @@ class PlacementObjective:
                ep.num_bins_x, ep.num_bins_y,
                N.DP_CONVERT_CPU if ep.convention == "cpu" else N.DP_CONVERT_GPU,
                ep.algorithm_code, ep.large_capacity, ep.uniform_filler_arg(),
-               N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map), None,
+               N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map),
+               N.ptr(self.movable_map) if self.track_metrics else None,
                N.stream_ptr(pos.device))
         dmap = self.density_map
         if dmap.dtype != self.plan.dtype:
@@ -360,6 +386,20 @@ class PlacementObjective:
                           compute_potential=self.potential is not None and want_energy,
                           compute_energy=want_energy,
                           out=(self.field_x, self.field_y, self.potential, self.energy))
+
+    def _reduce_metrics(self, pos, s):
+        """metric_out = {HPWL (design net weights, all nets of two or more
+        pins), overflow cost, max bin density} at `pos`, from this
+        evaluation's per-net extremes and fixed + movable map."""
+        dd, ep = self.dd, self.density
+        N.call("dp_hpwl_from_nets", N.dtype_code(self.net_hpwl), N.ptr(self.net_hpwl),
+               N.ptr(dd.flat_net2pin_start_map), dd.num_nets, N.ptr(dd.net_weights),
+               N.ptr(dd.net_mask_all), N.ptr(self.metric_ws), N.ptr(self.metric_out[0:1]), s)
+        bin_area = float(ep.bin_size_x) * float(ep.bin_size_y)
+        N.call("dp_density_overflow", N.dtype_code(self.movable_map), N.ptr(self.movable_map),
+               self.movable_map.numel(), float(ep.target_density) * bin_area,
+               N.ptr(ep.reduce_workspace(pos.device)), N.ptr(self.metric_out[1:3]), s)
+        self.metric_ptr = pos.data_ptr()
 
     def _energy(self, pos):
         e = self.energy
@@ -417,6 +457,7 @@ class PlacementObjective:
                    pp.num_big, E, N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.slot_w),
                    N.ptr(self.gamma_t if self.lse else self.inv_gamma),
                    N.ptr(self.partial_wl) if self.keep_partial_wl else None,
+                   N.ptr(self.net_hpwl) if self.track_metrics else None,
                    N.ptr(pp.grad_node_order), N.ptr(self.wl), N.ptr(pp.ws), s_wl)
             if self.side is not None:
                 self._ev_join.record(self.side)
@@ -425,6 +466,8 @@ class PlacementObjective:
             if self.side is not None:
                 # join: the node gradient and the objective read the wirelength outputs
                 torch.cuda.current_stream(pos.device).wait_event(self._ev_join)
+            if self.track_metrics:
+                self._reduce_metrics(pos, s)
             if precondition is not None:
                 pws, areas, alpha = precondition
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),

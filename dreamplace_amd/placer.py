@@ -31,7 +31,7 @@ DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
                 density_factor=1.0, early_stop=True, filler_sort_interval=50,
-                fuse_move_boundary=True)
+                fuse_move_boundary=True, fused_metrics=True)
 
 
 def noise(d, ratio, seed):
@@ -105,7 +105,12 @@ class GlobalPlacer:
     are then in that order; `positions()` gives them in the design's.
     fuse_move_boundary: the optimizer clamps its extrapolated positions to
     the die in the extrapolation pass instead of a move_boundary pass after
-    it (the same bits)."""
+    it (the same bits).
+    fused_metrics: HPWL and overflow at positions the objective has just
+    evaluated come from that evaluation (PlacementObjective track_metrics)
+    instead of a pin-position pass, an HPWL pass and a second scatter (the
+    same bits); where the objective cannot track them, the separate passes
+    run."""
 
     def __init__(self, d, device, params=None, dtype=None, seed=0, **objective_kw):
         p = dict(DEFAULTS)
@@ -122,7 +127,13 @@ class GlobalPlacer:
         dd = self.dd
         self.base_gamma = p["gamma"] * (d["bin_size_x"] + d["bin_size_y"])
         self.gamma = 10 * self.base_gamma
-        self.objective = PlacementObjective(dd, self.gamma, density_weight=0.0, **objective_kw)
+        track = False
+        if p["fused_metrics"]:
+            kw = dict(objective_kw)
+            track = (kw.get("schedule") == "pins" and kw.get("algorithm") == "tiled"
+                     and (dd.num_filler_nodes == 0 or dd.uniform_fillers()))
+        self.objective = PlacementObjective(dd, self.gamma, density_weight=0.0, track_metrics=track,
+                                            **objective_kw)
         self.lam = 0.0
         self.quad_penalty = bool(p["quad_penalty"])
         self.density_factor = float(p["density_factor"])
@@ -189,6 +200,7 @@ class GlobalPlacer:
 
     # -- pieces of the reference loop ----------------------------------------
     def constraint(self, x):
+        self.objective.metric_ptr = None  # positions change outside an evaluation
         dd = self.dd
         N.call("dp_move_boundary", self.code, N.ptr(x), dd.num_nodes, N.ptr(dd.node_size_x),
                N.ptr(dd.node_size_y), float(dd.xl), float(dd.yl), float(dd.xh), float(dd.yh),
@@ -238,6 +250,16 @@ class GlobalPlacer:
 
     def metrics(self, x):
         dd, dev = self.dd, x.device
+        ob = self.objective
+        ep = ob.density
+        bin_area = ep.bin_size_x * ep.bin_size_y
+        T = np.float32 if x.dtype == torch.float32 else np.float64
+        if ob.track_metrics and ob.metric_ptr is not None and ob.metric_ptr == x.data_ptr():
+            # the objective's last evaluation was at x: its metrics, one read-back
+            h, cost, mx = ob.metric_out.tolist()
+            return dict(hpwl=h, overflow=float(T(cost)) / self.movable_area,
+                        max_density=float(T(mx / bin_area)), density_weight=self.lam,
+                        gamma=self.gamma)
         N.call("dp_pin_pos_forward", self.code, N.ptr(x), dd.num_nodes, N.ptr(dd.pin_offset_x),
                N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, dd.num_pins,
                N.ptr(self.pin_pos), N.stream_ptr(dev))
@@ -245,15 +267,12 @@ class GlobalPlacer:
         N.call("dp_hpwl", self.code, N.ptr(self.pin_pos), dd.num_pins, N.ptr(dd.flat_net2pin_map),
                N.ptr(dd.flat_net2pin_start_map), dd.num_nets, N.ptr(dd.net_weights),
                N.ptr(dd.net_mask_all), N.ptr(self.hpwl_ws), N.ptr(buf[0:1]), N.stream_ptr(dev))
-        ep = self.objective.density
         m = ep.density_map(x, num_filler_nodes=0)
-        bin_area = ep.bin_size_x * ep.bin_size_y
         N.call("dp_density_overflow", N.dtype_code(m), N.ptr(m), m.numel(),
                float(ep.target_density) * bin_area, N.ptr(ep.reduce_workspace(dev)), N.ptr(buf[1:3]),
                N.stream_ptr(dev))
         h, cost, mx = buf.tolist()
         # ElectricOverflow returns the cost and the peak in the map's dtype
-        T = np.float32 if m.dtype == torch.float32 else np.float64
         return dict(hpwl=h, overflow=float(T(cost)) / self.movable_area,
                     max_density=float(T(mx / bin_area)), density_weight=self.lam, gamma=self.gamma)
 
@@ -273,6 +292,7 @@ class GlobalPlacer:
         shrink every node towards the movables' centroid, add Gaussian noise,
         then put the fixed nodes back."""
         dd = self.dd
+        self.objective.metric_ptr = None
         n, m0 = dd.num_nodes, dd.num_movable_nodes
         m1 = n - dd.num_filler_nodes
         xc = x[:m0].mean()
@@ -312,6 +332,7 @@ class GlobalPlacer:
                     and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
                                          threshold=overflow_list[-1])):
                 x.copy_(self.best_pos)
+                self.objective.metric_ptr = None
                 constrained = False
                 stop = True
                 self.rolled_back = True

@@ -121,6 +121,24 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
                    const double* uniform_filler, void* workspace, void* density_map_out,
                    int64_t* fixed_map_out, void* stream);
 
+/* New: dp_density_map that also writes movable_map_out[num_bins] (`dtype`),
+ * the map of the fixed and movable cells alone, converted as density_map_out
+ * is: the map ElectricOverflow's density_map forms with num_filler_nodes = 0
+ * (electric_overflow.py:260-278, the second scatter NonLinearPlace.py:414
+ * runs for the overflow metric), bit-identical, from this one scatter (the
+ * fillers go to an int64 map of their own and are added for
+ * density_map_out).  Needs DP_SCATTER_TILED with fillers of one size
+ * (uniform_filler) or none; else DP_ERR_UNSUPPORTED. */
+int dp_density_map_movable(int dtype, const void* pos, int64_t num_nodes,
+                           const void* node_size_x_clamped, const void* node_size_y_clamped,
+                           const void* offset_x, const void* offset_y, const void* ratio,
+                           const void* initial_density_map, double xl, double yl, double xh,
+                           double yh, double bin_size_x, double bin_size_y,
+                           int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                           int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                           const double* uniform_filler, void* workspace, void* density_map_out,
+                           void* movable_map_out, void* stream);
+
 /* Replaces electric_potential_cuda.fixed_density_map
  * (electric_density_map_cuda.cpp:139-182, electric_density_map.cpp:155-213)
  * followed by the Python-side scaling of electric_overflow.py:258:
@@ -350,15 +368,17 @@ size_t dp_wa_pins_workspace_bytes(int64_t num_pins, int dtype);
  * partial_wl[num_nets] (nullable; 0 for masked nets) and *wl_out as
  * dp_wa_merged_forward, and grad_node_order[2P]: each pin's derivative
  * (x, y interleaved), times its net weight, 0 for masked nets and pins,
- * stored at the pin's node-order slot. */
+ * stored at the pin's node-order slot.  net_hpwl[num_nets] (nullable, new):
+ * every net's HPWL (max - min per axis, unweighted, in hpwl_cuda_kernel.cu's
+ * expression) from the same pass, masked or not, for dp_hpwl_from_nets. */
 int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
                        const int32_t* slot_node, const void* node_offsets,
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                        int64_t num_big, int64_t num_nets, const void* net_weights,
                        const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
-                       void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
-                       void* stream);
+                       void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
+                       void* workspace, void* stream);
 
 /* The log-sum-exp model on the same plan and skeleton: the chain
  * LSE(PinPos(pos)) of logsumexp_wirelength_merged.cpp:14-85 (CUDA twin
@@ -371,8 +391,8 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                         const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
                         int64_t num_big, int64_t num_nets, const void* net_weights,
                         const uint8_t* net_mask, const void* slot_weights, const void* gamma,
-                        void* partial_wl, void* grad_node_order, void* wl_out, void* workspace,
-                        void* stream);
+                        void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
+                        void* workspace, void* stream);
 
 /* Optional companion of the two entry points above, with no counterpart in
  * the reference: for callers whose net weights and net mask stay fixed
@@ -412,6 +432,17 @@ int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_no
 int dp_hpwl(int dtype, const void* pin_pos, int64_t num_pins, const int32_t* flat_netpin,
             const int32_t* netpin_start, int64_t num_nets, const void* net_weights,
             const uint8_t* net_mask, void* workspace, double* hpwl_out, void* stream);
+
+/* New: the HPWL metric from per-net values another pass already formed
+ * (dp_wa_forward_pins' net_hpwl), for a loop that evaluates the objective
+ * and the metric at the same positions (NonLinearPlace.py:414 evaluates both
+ * there): *hpwl_out = sum over nets with net_mask and at least one pin of
+ * net_hpwl[n] * net_weights[n], with dp_hpwl's grid and summation order, so
+ * the result is bit-identical to dp_hpwl on those positions.  workspace:
+ * dp_wa_workspace_bytes(num_nets). */
+int dp_hpwl_from_nets(int dtype, const void* net_hpwl, const int32_t* netpin_start,
+                      int64_t num_nets, const void* net_weights, const uint8_t* net_mask,
+                      void* workspace, double* hpwl_out, void* stream);
 
 /* Replaces move_boundary_cuda.forward (dreamplace/ops/move_boundary/src/
  * move_boundary_cuda_kernel.cu:10-75): in-place clamp of movables and fillers. */

@@ -192,3 +192,47 @@ def test_engine_filler_tile_order(gpu, dtype):
         out.append((f.item(), dd.generation_order(g)))
     assert out[0][0] == out[1][0]
     assert torch.equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_tracked_metrics_equal_the_separate_passes(gpu, dtype):
+    """HPWL from the wirelength pass's per-net extremes and overflow from the
+    objective's own scatter give the same bits as dp_hpwl on pin positions
+    and ElectricOverflow's movables-only scatter, with nets above 64 pins and
+    nets the WA mask drops (HPWL counts them)."""
+    from dreamplace_amd import _native as N
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case engine tracked metrics %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype, big_nets=20)
+    assert (np.diff(d.flat_net2pin_start_map) >= 100).any()
+    dd = DeviceDesign(d, gpu)
+    dd.reorder_fillers()
+    ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
+                            else "fp64", algorithm="tiled", schedule="pins", track_metrics=True)
+    ob.evaluate(dd.pos)
+    assert ob.metric_ptr == dd.pos.data_ptr()
+    got = ob.metric_out.clone()
+    code = N.dtype_code(dd.pos)
+    pin_pos = torch.empty(2 * d.num_pins, dtype=dd.dtype, device=gpu)
+    N.call("dp_pin_pos_forward", code, N.ptr(dd.pos), d.num_nodes, N.ptr(dd.pin_offset_x),
+           N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, d.num_pins, N.ptr(pin_pos),
+           N.stream_ptr(gpu))
+    ref = torch.empty(3, dtype=torch.float64, device=gpu)
+    ws = torch.empty(N.query("dp_wa_workspace_bytes", d.num_nets), dtype=torch.uint8, device=gpu)
+    N.call("dp_hpwl", code, N.ptr(pin_pos), d.num_pins, N.ptr(dd.flat_net2pin_map),
+           N.ptr(dd.flat_net2pin_start_map), d.num_nets, N.ptr(dd.net_weights), N.ptr(dd.net_mask_all),
+           N.ptr(ws), N.ptr(ref[0:1]), N.stream_ptr(gpu))
+    ep = ob.density
+    m = ep.density_map(dd.pos, num_filler_nodes=0)
+    assert torch.equal(m, ob.movable_map)
+    N.call("dp_density_overflow", N.dtype_code(m), N.ptr(m), m.numel(),
+           float(ep.target_density) * ep.bin_size_x * ep.bin_size_y, N.ptr(ep.reduce_workspace(gpu)),
+           N.ptr(ref[1:3]), N.stream_ptr(gpu))
+    assert torch.equal(got, ref), (got, ref)
+    # the objective's full map is unchanged by the split scatter
+    ob2 = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
+                             else "fp64", algorithm="tiled", schedule="pins")
+    f2, g2 = ob2.evaluate(dd.pos)
+    f1, g1 = ob.evaluate(dd.pos)
+    assert torch.equal(ob.density_map, ob2.density_map)
+    assert f1.item() == f2.item() and torch.equal(g1, g2)

@@ -175,3 +175,23 @@ def test_fused_move_boundary_is_bit_identical(gpu, dtype):
     assert torch.equal(runs[0][0], runs[1][0])
     for h0, h1 in zip(runs[0][1], runs[1][1]):
         assert h0 == h1
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_fused_metrics_keep_the_loop(gpu, dtype):
+    """Metrics taken from the objective's last evaluation (when it was at the
+    same positions) give the same loop, bit for bit, as the separate passes."""
+    from dreamplace_amd.placer import GlobalPlacer
+    print("case placer fused metrics %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype)
+    runs = []
+    for fused in (False, True):
+        gp = GlobalPlacer(d, gpu, params=dict(fused_metrics=fused, early_stop=False),
+                          precision="fp64" if dtype == np.float64 else "bf16x3", algorithm="tiled",
+                          schedule="pins")
+        assert gp.objective.track_metrics == fused
+        _, hist = gp.run(12)
+        runs.append((gp.pos.clone(), hist))
+    assert torch.equal(runs[0][0], runs[1][0])
+    for h0, h1 in zip(runs[0][1], runs[1][1]):
+        assert h0 == h1
