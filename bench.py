@@ -48,10 +48,10 @@ DP_OP_DENSITY_MAP, DP_OP_ELECTRIC_FORCE, DP_OP_WA_FORWARD, DP_OP_NODE_GRAD, DP_O
 # kernels of each op in the pins schedule with tiled density (rocprof names)
 OP_KERNELS = {
     "wa_forward": ["k_pin_push", "k_wa_pins", "k_wa_big", "k_finish"],
-    "density_map": ["k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill", "k_tile_scatter",
-                    "k_rec_scatter_large", "k_convert_cpu"],
+    "density_map": ["k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill", "k_tile_scatter_f32",
+                    "k_drift_scatter", "k_rec_scatter_large", "k_convert_cpu"],
     "poisson": ["k_sum_partial", "k_scale_centre", "k_gemm_bf16x3", "k_spectral_weights"],
-    "electric_force": ["k_fill", "k_tile_gather", "k_rec_gather_large"],
+    "electric_force": ["k_fill", "k_tile_gather", "k_drift_gather", "k_rec_gather_large"],
     "node_grad": ["k_node_grad"],
 }
 
@@ -281,9 +281,10 @@ def summarize_trace(trace_dir, warmup, steps):
 TRAFFIC_KERNELS = {
     "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<", "dp::k_finish"],
     "density_map": ["dp::k_fill<unsigned long long>", "dp::k_bin_count<", "dp::k_bin_scan",
-                    "dp::k_bin_fill<", "dp::k_tile_scatter<", "dp::k_rec_scatter_large<",
-                    "dp::k_convert_cpu<"],
-    "electric_force": ["dp::k_fill<float>", "dp::k_tile_gather<", "dp::k_rec_gather_large<"],
+                    "dp::k_bin_fill<", "dp::k_tile_scatter", "dp::k_drift_scatter<",
+                    "dp::k_rec_scatter_large<", "dp::k_convert_cpu<"],
+    "electric_force": ["dp::k_fill<float>", "dp::k_tile_gather<", "dp::k_drift_gather<",
+                       "dp::k_rec_gather_large<"],
     "node_grad": ["dp::k_node_grad<"],
 }
 
@@ -455,7 +456,8 @@ def main():
                          "from the materialised potential map as the reference does")
     ap.add_argument("--filler-order", default="tile", choices=["tile", "generation"],
                     help="tile: fillers relabelled into the density kernels' tile order before the "
-                         "timed steps (as the placer keeps them); generation: as generated")
+                         "timed steps and taken by tile range (as the placer keeps them); "
+                         "generation: as generated, binned every step")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -513,17 +515,17 @@ def main():
     gamma = d.wa_gamma
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
-    perm = None
-    if args.filler_order == "tile":
-        # fillers are interchangeable (one size, no pins): the engine keeps
-        # them in the density kernels' tile order; the FP64 form and the CPU
-        # baseline get the same order
-        perm = dd32.reorder_fillers()
-        d = synth.Design(d)
-        d.pos = dd32.pos.cpu().numpy()
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
                                algorithm=args.algorithm, schedule=args.schedule,
                                energy_mode=args.energy_mode)
+    if args.filler_order == "tile":
+        # fillers are interchangeable (one size, no pins): the engine keeps
+        # them in the density kernels' tile order and takes them by tile range;
+        # the FP64 form and the CPU baseline get the same order
+        dd32.reorder_fillers(origin_offset=obj32.filler_origin_offset())
+        obj32.update_filler_ranges(dd32.pos)
+        d = synth.Design(d)
+        d.pos = dd32.pos.cpu().numpy()
     pos32 = dd32.pos
     ms32, per32 = time_steps(torch, obj32, pos32, args.steps, args.warmup)
     log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32,
@@ -573,6 +575,8 @@ def main():
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
                                    algorithm=args.algorithm, schedule=args.schedule,
                                    energy_mode=args.energy_mode)
+        if args.filler_order == "tile":
+            obj64.update_filler_ranges(dd64.pos)
         ms64, _ = time_steps(torch, obj64, dd64.pos, max(args.steps // 2, 5), 2)
         log("[bench] fp64: %.3f ms/step" % ms64)
         obj64.evaluate(dd64.pos)

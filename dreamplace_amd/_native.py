@@ -37,8 +37,8 @@ _SIGNATURES = {
     "dp_density_map_workspace_bytes": (_S, [_L, _L, _L, _L, _I, _I]),
     "dp_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L,
                             _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
-    "dp_density_map_movable": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D,
-                                    _L, _L, _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
+    "dp_density_map_ex": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D,
+                               _L, _L, _I, _I, _I, _I, _L, _P, _P, _P, _P, _P, _P]),
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,

@@ -325,7 +325,8 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
                             int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
                             int num_bins_y, int convention, int algorithm,
                             int64_t large_capacity, const double* uniform_filler,
-                            void* workspace, void* density_map_out, int64_t* fixed_map_out,
+                            const int32_t* filler_tile_start, void* workspace,
+                            void* density_map_out, int64_t* fixed_map_out,
                             void* movable_map_out, void* stream) {
   DP_ARG(pos && node_size_x_clamped && node_size_y_clamped && offset_x && offset_y && ratio,
          "dp_density_map: null input array");
@@ -372,7 +373,8 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
     }
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y)) {
       DP_CHECK_RC(scatter_tiled<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
-                                   num_filler_nodes, w.tiled, w.map, fmap, s));
+                                   num_filler_nodes, w.tiled, w.map, fmap,
+                                   (const int*)filler_tile_start, s));
     } else {
       DP_REQUIRE(!fmap, DP_ERR_UNSUPPORTED,
                  "%s: the movable map needs the tiled scatter", name);
@@ -409,26 +411,27 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
                           node_size_y_clamped, offset_x, offset_y, ratio, initial_density_map, xl,
                           yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
                           num_bins_x, num_bins_y, convention, algorithm, large_capacity,
-                          uniform_filler, workspace, density_map_out, fixed_map_out, nullptr,
-                          stream);
+                          uniform_filler, nullptr, workspace, density_map_out, fixed_map_out,
+                          nullptr, stream);
 }
 
-int dp_density_map_movable(int dtype, const void* pos, int64_t num_nodes,
-                           const void* node_size_x_clamped, const void* node_size_y_clamped,
-                           const void* offset_x, const void* offset_y, const void* ratio,
-                           const void* initial_density_map, double xl, double yl, double xh,
-                           double yh, double bin_size_x, double bin_size_y,
-                           int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
-                           int num_bins_y, int convention, int algorithm, int64_t large_capacity,
-                           const double* uniform_filler, void* workspace, void* density_map_out,
-                           void* movable_map_out, void* stream) {
-  DP_ARG(movable_map_out, "dp_density_map_movable: null movable_map_out");
-  return density_map_impl("dp_density_map_movable", dtype, pos, num_nodes, node_size_x_clamped,
+int dp_density_map_ex(int dtype, const void* pos, int64_t num_nodes,
+                      const void* node_size_x_clamped, const void* node_size_y_clamped,
+                      const void* offset_x, const void* offset_y, const void* ratio,
+                      const void* initial_density_map, double xl, double yl, double xh,
+                      double yh, double bin_size_x, double bin_size_y,
+                      int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                      int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                      const double* uniform_filler, const int32_t* filler_tile_start,
+                      void* workspace, void* density_map_out, void* movable_map_out,
+                      void* stream) {
+  return density_map_impl("dp_density_map_ex", dtype, pos, num_nodes, node_size_x_clamped,
                           node_size_y_clamped, offset_x, offset_y, ratio, initial_density_map, xl,
                           yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
                           num_bins_x, num_bins_y, convention, algorithm, large_capacity,
-                          uniform_filler, workspace, density_map_out, nullptr, movable_map_out,
-                          stream);
+                          uniform_filler,
+                          algorithm == DP_SCATTER_TILED ? filler_tile_start : nullptr, workspace,
+                          density_map_out, nullptr, movable_map_out, stream);
 }
 
 int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const void* node_size_x,

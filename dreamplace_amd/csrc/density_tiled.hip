@@ -45,7 +45,13 @@
 namespace dp {
 
 constexpr int kTile = 64;                 // bins per tile side
-constexpr int kTileH = kTile + 2;         // with the two-bin halo of a 3x3 span
+// The LDS window of a tile: kHalo bins on every side plus the two above and
+// right that a 3x3 span from the tile's last bin reaches.  Record cells start
+// inside the tile; filler-range cells may have moved since they were sorted,
+// and those whose span leaves the window go to the drift list, done by
+// k_drift_scatter / k_drift_gather through global memory.
+constexpr int kHalo = 2;
+constexpr int kTileH = kTile + 2 + 2 * kHalo;
 constexpr int kMaxTiles = 2048;           // LDS histogram limit
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
 constexpr int kFillThreads = 1024;        // threads of a fill workgroup
@@ -124,7 +130,8 @@ static ClassSplit class_split(int64_t n0, int64_t total, int groups) {
 struct TiledLayout {
   int ntx, nty, ntiles;
   int64_t max_work;
-  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec[5], off_idx, total;
+  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec[5], off_idx, off_drift,
+      total;
 };
 
 static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) {
@@ -150,6 +157,8 @@ static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) 
     o += align256(tsize * n);
   }
   L.off_idx = o;
+  o += align256(sizeof(int) * n);
+  L.off_drift = o;  // filler-range cells outside their tile's window
   o += align256(sizeof(int) * n);
   L.total = o;
   return L;
@@ -258,17 +267,34 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
 // its class-1 run; tile_start[c * ntiles + k] = first record of (class c,
 // tile k) (exclusive scan over tiles of both counts); work items (tile,
 // begin, end, class) of at most kChunk cells.
+// frange (nullable): filler-relative index ranges per tile, [frange[k],
+// frange[k+1]) for tile k, of fillers kept in tile order; each range becomes
+// class-2 work items.  Ranges that are not a monotone partition of [0, nf)
+// are replaced by one range of all fillers on tile 0, which the scatter and
+// gather still do exactly (their out-of-window path), only slower.
 __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int ntiles,
-                                                   int* tile_start, int4* work, int* nwork) {
+                                                   int* tile_start, int4* work, int* nwork,
+                                                   const int* frange, int nf) {
   __shared__ int s_cells[1024];
   __shared__ int s_items[1024];
+  __shared__ int s_bad;
+  if (threadIdx.x == 0) s_bad = frange && (frange[0] != 0 || frange[ntiles] != nf) ? 1 : 0;
+  __syncthreads();
+  if (frange)
+    for (int k = threadIdx.x; k < ntiles; k += 1024)
+      if (frange[k] > frange[k + 1]) atomicOr(&s_bad, 1);
+  __syncthreads();
+  const bool bad = s_bad != 0;
+  auto f_lo = [&](int k) { return !frange ? 0 : bad ? (k == 0 ? 0 : nf) : frange[k]; };
+  auto f_hi = [&](int k) { return !frange ? 0 : bad ? nf : frange[k + 1]; };
   const int per = (ntiles + 1023) / 1024;
   const int k0 = threadIdx.x * per;
   int cells = 0, items = 0;
   for (int k = k0; k < k0 + per && k < ntiles; ++k) {
     const int c0 = tile_count[k], c1 = tile_count[ntiles + k];
+    const int c2 = f_hi(k) - f_lo(k);
     cells += c0 + c1;
-    items += (c0 + kChunk - 1) / kChunk + (c1 + kChunk - 1) / kChunk;
+    items += (c0 + kChunk - 1) / kChunk + (c1 + kChunk - 1) / kChunk + (c2 + kChunk - 1) / kChunk;
   }
   s_cells[threadIdx.x] = cells;
   s_items[threadIdx.x] = items;
@@ -294,6 +320,9 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
       }
       cbase += c;
     }
+    const int lo = f_lo(k), hi = f_hi(k);
+    for (int j = lo; j < hi; j += kChunk)
+      work[ibase++] = make_int4(k, j, j + kChunk < hi ? j + kChunk : hi, 2);
   }
   if (threadIdx.x == 1023) *nwork = s_items[1023];
 }
@@ -466,34 +495,83 @@ __device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& r
   }
 }
 
+// list[next slots] = v for every lane with `put`: the lanes' slots come from
+// one atomic on `count` per wavefront, ranked by lane (a list many cells of a
+// wavefront append to would otherwise serialise on one counter).  Lanes that
+// are not active take no part.
+__device__ __forceinline__ void wave_append(int* list, int* count, bool put, int v) {
+  const unsigned long long m = __ballot(put);
+  if (m == 0ull) return;  // the same for every lane
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, (int)__popcll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (put) list[base + (int)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// A work item of class 2: a range [j0, end) of uniform fillers (filler-
+// relative), read from the positions directly.  The stretched origin is the
+// same T sum the fill pass stores in a record.
 template <typename T>
-__global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TriArgs<T> a, int nty, const int4* work,
-                                                      const int* nwork, Recs<T> rec,
-                                                      unsigned long long* map,
-                                                      unsigned long long* filler_map) {
+__device__ __forceinline__ void load_range(const TriArgs<T>& a, int j0, int stride, int end,
+                                           RecBatch<T>& b) {
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) {
+    const int j = j0 + u * stride;
+    b.ok[u] = j < end;
+    if (b.ok[u]) {
+      const int64_t i = a.fill_begin + j;
+      b.x[u] = a.x[i] + a.uox;
+      b.y[u] = a.y[i] + a.uoy;
+      b.sx[u] = a.uwx;
+      b.sy[u] = a.uwy;
+      b.r[u] = a.ur;
+      b.idx[u] = (int)i;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const int4* work,
+                                             const int* nwork, const Recs<T>& rec,
+                                             unsigned long long* map,
+                                             unsigned long long* filler_map, int* drift,
+                                             int* ndrift) {
   __shared__ unsigned long long acc[kTileH * kTileH];
   const int W = *nwork;
   for (int w = blockIdx.x; w < W; w += gridDim.x) {
     const int4 it = work[w];
-    const int tx0 = (it.x / nty) * kTile, ty0 = (it.x % nty) * kTile;
+    const int wx0 = (it.x / nty) * kTile - kHalo, wy0 = (it.x % nty) * kTile - kHalo;
+    // uniform fillers (classes 1, 2) go to the filler map when there is one
+    unsigned long long* dst = (it.w != 0 && filler_map) ? filler_map : map;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) acc[k] = 0ull;
     __syncthreads();
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
-      load_batch(a, rec, j0, blockDim.x, it.z, false, it.w != 0, b);
+      if (it.w == 2)
+        load_range(a, j0, blockDim.x, it.z, b);
+      else
+        load_batch(a, rec, j0, blockDim.x, it.z, false, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        if (!b.ok[u]) continue;
         const T x = b.x[u], y = b.y[u], sx = b.sx[u], sy = b.sy[u], r = b.r[u];
         int bxl, nx, byl, ny;
         rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+        const int lx = bxl - wx0, ly = byl - wy0;
+        // a range cell whose span left its tile's window goes to the drift
+        // list (record cells start inside their tile: always in the window);
+        // every lane reaches the append, which takes one atomic per wavefront
+        const bool drifting = b.ok[u] && it.w == 2 && nx > 0 && ny > 0 &&
+                              (lx < 0 || ly < 0 || lx + nx > kTileH || ly + ny > kTileH);
+        wave_append(drift, ndrift, drifting, drifting ? b.idx[u] : 0);
+        if (!b.ok[u] || drifting) continue;
         T pxr[3], py[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           pxr[q] = tri(x, sx, a.xl, bxl + q, a.bsx) * r;
           py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
         }
-        const int lx = bxl - tx0, ly = byl - ty0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           if (q < nx) {
@@ -510,18 +588,45 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_scatter(TriArgs<T> a, int
       }
     }
     __syncthreads();
-    // a work item of uniform fillers goes to the filler map when there is one
-    unsigned long long* dst = (it.w != 0 && filler_map) ? filler_map : map;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) {
       const unsigned long long v = acc[k];
-      if (v) {
-        const int gx = tx0 + k / kTileH, gy = ty0 + k % kTileH;
+      if (v) {  // only bins of the die receive contributions
+        const int gx = wx0 + k / kTileH, gy = wy0 + k % kTileH;
         atomicAdd(&dst[(int64_t)gx * a.nby + gy], v);
       }
     }
     __syncthreads();
   }
 }
+
+// FP32 asks for 8 waves per SIMD explicitly: the extra arguments of the
+// filler map and the drift list would otherwise cost a wave to SGPR pressure
+// (a few SGPRs spill to VGPR lanes).  FP64 keeps the compiler's choice: there
+// the same request would spill VGPRs to scratch.
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_tile_scatter_f32(TriArgs<float> a, int nty, const int4* work, const int* nwork,
+                   Recs<float> rec, unsigned long long* map, unsigned long long* filler_map,
+                   int* drift, int* ndrift) {
+  tile_scatter(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
+}
+
+__global__ void __launch_bounds__(kTileThreads)
+k_tile_scatter_f64(TriArgs<double> a, int nty, const int4* work, const int* nwork,
+                   Recs<double> rec, unsigned long long* map, unsigned long long* filler_map,
+                   int* drift, int* ndrift) {
+  tile_scatter(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
+}
+
+template <typename T>
+struct TileScatterKernel;
+template <>
+struct TileScatterKernel<float> {
+  static constexpr auto fn = k_tile_scatter_f32;
+};
+template <>
+struct TileScatterKernel<double> {
+  static constexpr auto fn = k_tile_scatter_f64;
+};
 
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int nty, const int4* work,
@@ -543,11 +648,11 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
     const int hi = it.y + (int)((int64_t)len * (part + 1) / kGatherSplit);
     it.y = lo;
     it.z = hi;
-    const int tx0 = (it.x / nty) * kTile, ty0 = (it.x % nty) * kTile;
+    const int wx0 = (it.x / nty) * kTile - kHalo, wy0 = (it.x % nty) * kTile - kHalo;
     for (int k = threadIdx.x; k < kTileH * kTileH; k += blockDim.x) {
-      const int gx = tx0 + k / kTileH, gy = ty0 + k % kTileH;
-      const bool in = gx < a.nbx && gy < a.nby;
-      const int64_t idx = (int64_t)gx * a.nby + gy;
+      const int gx = wx0 + k / kTileH, gy = wy0 + k % kTileH;
+      const bool in = gx >= 0 && gy >= 0 && gx < a.nbx && gy < a.nby;
+      const int64_t idx = in ? (int64_t)gx * a.nby + gy : 0;
       V v;
       v.x = in ? fx[idx] : T(0);
       v.y = in ? fy[idx] : T(0);
@@ -556,7 +661,10 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
     __syncthreads();
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
-      load_batch(a, rec, j0, blockDim.x, it.z, true, it.w != 0, b);
+      if (it.w == 2)
+        load_range(a, j0, blockDim.x, it.z, b);
+      else
+        load_batch(a, rec, j0, blockDim.x, it.z, true, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (!b.ok[u]) continue;
@@ -569,7 +677,12 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
           px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
           py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
         }
-        const int lx = bxl - tx0, ly = byl - ty0;
+        const int lx = bxl - wx0, ly = byl - wy0;
+        // a range cell outside its tile's window with a non-empty span is on
+        // the drift list (k_drift_gather); an empty span gives 0 here
+        if (it.w == 2 && (lx < 0 || ly < 0 || lx + nx > kTileH || ly + ny > kTileH) && nx > 0 &&
+            ny > 0)
+          continue;
         T gx = 0, gy = 0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -595,6 +708,79 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
 
 // Tail records: one workgroup per cell, lanes over its bins.  Records with an
 // empty span contribute nothing.
+// The drift list: filler-range cells whose span left their tile's window,
+// done through global memory with the tile kernels' per-cell arithmetic, so
+// the results are the bits the window path would give.
+template <typename T>
+__global__ void __launch_bounds__(256) k_drift_scatter(TriArgs<T> a, const int* drift,
+                                                       const int* ndrift,
+                                                       unsigned long long* dst) {
+  const int cnt = *ndrift;
+  for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < cnt; q0 += gridDim.x * blockDim.x) {
+    const int64_t i = drift[q0];
+    const T x = a.x[i] + a.uox, y = a.y[i] + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    int bxl, nx, byl, ny;
+    rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+    T pxr[3], py[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pxr[q] = tri(x, sx, a.xl, bxl + q, a.bsx) * r;
+      py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (q < nx) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          if (l < ny) {
+            T area = pxr[q] * py[l];
+            long long v = (long long)(area * a.scale);
+            if (v != 0)
+              atomicAdd(&dst[(int64_t)(bxl + q) * a.nby + (byl + l)], (unsigned long long)v);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_drift_gather(TriArgs<T> a, const int* drift,
+                                                      const int* ndrift, const T* __restrict__ fx,
+                                                      const T* __restrict__ fy, const T* grad,
+                                                      int64_t num_nodes, int interleaved, T* out) {
+  const int cnt = *ndrift;
+  const T g = grad ? *grad : T(1);
+  for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < cnt; q0 += gridDim.x * blockDim.x) {
+    const int64_t i = drift[q0];
+    const T x = a.x[i] + a.uox, y = a.y[i] + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    int bxl, nx, byl, ny;
+    rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
+    T px[3], py[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
+      py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
+    }
+    T gx = 0, gy = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (q < nx) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          if (l < ny) {
+            T area = px[q] * py[l];
+            const int64_t k = (int64_t)(bxl + q) * a.nby + (byl + l);
+            gx += area * fx[k];
+            gy += area * fy[k];
+          }
+        }
+      }
+    }
+    put_grad(out, num_nodes, i, -((gx * r) * g), -((gy * r) * g), interleaved);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T> rec,
                                                            int64_t total, const int* ntail,
@@ -662,13 +848,20 @@ __global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> 
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
                   char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
-                  hipStream_t s) {
+                  const int* filler_ranges, hipStream_t s) {
   const int64_t total = n_mov + n_fill;
   TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled scatter: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
   DP_REQUIRE(!filler_map || n_fill == 0 || a.uni, DP_ERR_UNSUPPORTED,
              "tiled scatter: a separate filler map needs fillers of one size");
+  // filler ranges: uniform fillers whose stretched size is at most 2 bins, so
+  // every span is at most 3 bins (the tile kernels' loops)
+  const bool ranges = filler_ranges != nullptr && n_fill > 0;
+  DP_REQUIRE(!ranges || (a.uni && (double)a.uwx * (double)a.inv_x <= 2.0 &&
+                         (double)a.uwy * (double)a.inv_y <= 2.0),
+             DP_ERR_UNSUPPORTED,
+             "tiled scatter: filler ranges need fillers of one size, at most 2 bins stretched");
   int* tile_count = (int*)(tiled_ws + L.off_count);
   int* tile_start = (int*)(tiled_ws + L.off_start);
   int* wg_base = (int*)(tiled_ws + L.off_wgbase);
@@ -678,16 +871,20 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
   Recs<T> rec = recs_of<T>(tiled_ws, L);
   hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(2 * L.ntiles, 256)), dim3(256), 0, s, tile_count,
                      2 * (int64_t)L.ntiles, 0);
-  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, nwork, (int64_t)2, 0);
+  int* ndrift = nwork + 2;
+  int* drift = (int*)(tiled_ws + L.off_drift);
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, nwork, (int64_t)3, 0);
   DP_LAUNCHED("tiled init");
   if (total > 0) {
-    // uniform fillers are class 1 (their own workgroups and record runs)
-    const ClassSplit cs = class_split(a.uni ? n_mov : total, total, kBinGroups);
+    // uniform fillers are class 1 (their own workgroups and record runs);
+    // with filler ranges they are not binned at all (class 2 work items)
+    const ClassSplit cs = ranges ? class_split(n_mov, n_mov, kBinGroups)
+                                 : class_split(a.uni ? n_mov : total, total, kBinGroups);
     hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256), 0, s, a, n_mov, fill_begin,
                        total, cs, L.nty, L.ntiles, tile_count, wg_base, ntail, rec);
     DP_LAUNCHED("k_bin_count");
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, (const int*)tile_count, L.ntiles,
-                       tile_start, work, nwork);
+                       tile_start, work, nwork, ranges ? filler_ranges : nullptr, (int)n_fill);
     DP_LAUNCHED("k_bin_scan");
     DP_PROBED(DP_KERNEL_DENSITY_BIN, s,
               hipLaunchKernelGGL(k_bin_fill<T>, dim3(kBinGroups), dim3(kFillThreads), 0, s, a,
@@ -695,10 +892,15 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
                                  (const int*)tile_start, (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
-              hipLaunchKernelGGL(k_tile_scatter<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty,
-                                 (const int4*)work, (const int*)nwork, rec, map,
-                                 filler_map));
+              hipLaunchKernelGGL(TileScatterKernel<T>::fn, dim3(2048), dim3(kTileThreads), 0, s,
+                                 a, L.nty, (const int4*)work, (const int*)nwork, rec, map,
+                                 filler_map, drift, ndrift));
     DP_LAUNCHED("k_tile_scatter");
+    if (ranges) {
+      hipLaunchKernelGGL(k_drift_scatter<T>, dim3(256), dim3(256), 0, s, a, (const int*)drift,
+                         (const int*)ndrift, filler_map ? filler_map : map);
+      DP_LAUNCHED("k_drift_scatter");
+    }
     hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, rec, total,
                        (const int*)ntail, map, filler_map, fill_begin);
     DP_LAUNCHED("k_rec_scatter_large");
@@ -737,6 +939,12 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
               hipLaunchKernelGGL(k_tile_gather<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty, work,
                                  nwork, rec, fx, fy, grad, num_nodes, interleaved, out));
     DP_LAUNCHED("k_tile_gather");
+    // fillers taken by range whose span left their tile's window (the list is
+    // empty unless the last density map used filler ranges)
+    hipLaunchKernelGGL(k_drift_gather<T>, dim3(256), dim3(256), 0, s, a,
+                       (const int*)(tiled_ws + L.off_drift), nwork + 2, fx, fy, grad, num_nodes,
+                       interleaved, out);
+    DP_LAUNCHED("k_drift_gather");
     hipLaunchKernelGGL(k_rec_gather_large<T>, dim3(256), dim3(256), 0, s, a, rec, total, ntail,
                        fx, fy, grad, num_nodes, interleaved, out);
     DP_LAUNCHED("k_rec_gather_large");
@@ -746,7 +954,8 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
 
 #define DP_TILED_INST(T)                                                                     \
   template int scatter_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, char*,        \
-                                unsigned long long*, unsigned long long*, hipStream_t);      \
+                                unsigned long long*, unsigned long long*, const int*,        \
+                                hipStream_t);                                                \
   template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, const T*,      \
                                const T*, const T*, char*, int, T*, hipStream_t);
 DP_TILED_INST(float)

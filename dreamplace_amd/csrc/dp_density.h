@@ -126,10 +126,13 @@ bool tiled_supported(int nbx, int nby);
 size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype);
 // filler_map (nullable): uniform fillers' contributions go there instead of
 // `map`, which then holds the other cells' only (requires uniform fillers).
+// filler_ranges (nullable, device int[ntiles + 1]): fillers kept in tile
+// order, scattered (and later gathered) from their positions by tile range
+// instead of being binned.
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
                   char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
-                  hipStream_t s);
+                  const int* filler_ranges, hipStream_t s);
 template <typename T>
 int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
                  const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,

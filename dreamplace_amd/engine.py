@@ -67,14 +67,15 @@ class DeviceDesign:
         sx, sy = self.node_size_x[n - nf:], self.node_size_y[n - nf:]
         return bool((sx == sx[0]).all().item() and (sy == sy[0]).all().item())
 
-    def reorder_fillers(self, perm=None):
+    def reorder_fillers(self, perm=None, origin_offset=(0.0, 0.0)):
         """Relabel the fillers so that those in one 64x64-bin tile of the
         density kernels are consecutive: the tiled scatter's records and the
         field gather's gradient stores then run in node order instead of
         scattering.  Fillers are interchangeable (one size, no pins), so the
         objective is unchanged and each filler's gradient moves with it.
         `perm` (filler-relative, new slot -> old slot) defaults to the tile
-        order of the current positions.  Applies it to `pos`, keeps the
+        order of the current positions (plus origin_offset, the density
+        kernels' stretch of the fillers' origin).  Applies it to `pos`, keeps the
         accumulated relabelling in `filler_perm` (slot -> generated filler)
         and returns `perm` for the caller's other per-node vectors
         (permute_fillers_)."""
@@ -85,7 +86,8 @@ class DeviceDesign:
             raise ValueError("reorder_fillers: fillers of different sizes are not interchangeable")
         if perm is None:
             perm = filler_tile_order(self.pos, n, nf, self.xl, self.yl, self.bin_size_x,
-                                     self.bin_size_y, self.num_bins_x, self.num_bins_y)
+                                     self.bin_size_y, self.num_bins_x, self.num_bins_y,
+                                     origin_offset)
         perm = perm.to(device=self.device, dtype=torch.int64)
         permute_fillers_(self.t["pos"], n, nf, perm)
         fp = self.__dict__.get("filler_perm")
@@ -122,18 +124,28 @@ class DeviceDesign:
         raise AttributeError(k)
 
 
-def filler_tile_order(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_y, num_bins_x,
-                      num_bins_y, tile=64):
-    """Filler-relative permutation (new slot -> old slot) that sorts the
-    fillers of a [2N] position vector by the tile of their lower-left bin,
-    tile-major as the tiled density kernels number tiles; stable, so fillers
-    within a tile keep their relative order."""
+def filler_tile_keys(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_y, num_bins_x,
+                     num_bins_y, origin_offset=(0.0, 0.0), tile=64):
+    """Tile of each filler's lower-left bin, tile-major as the tiled density
+    kernels number tiles, for the origin pos + origin_offset (the stretched
+    origin the kernels bin, in their operation order: (x + ox - xl) / bin)."""
     n, f0 = int(num_nodes), int(num_nodes) - int(num_fillers)
     x, y = pos[f0:n], pos[n + f0:2 * n]
-    bx = torch.floor((x - xl) / bin_size_x).clamp_(0, num_bins_x - 1).to(torch.int64) // tile
-    by = torch.floor((y - yl) / bin_size_y).clamp_(0, num_bins_y - 1).to(torch.int64) // tile
+    ox, oy = origin_offset
+    bx = torch.floor((x + ox - xl) * (1.0 / bin_size_x)).clamp_(0, num_bins_x - 1).to(torch.int64)
+    by = torch.floor((y + oy - yl) * (1.0 / bin_size_y)).clamp_(0, num_bins_y - 1).to(torch.int64)
     nty = (int(num_bins_y) + tile - 1) // tile
-    return torch.argsort(bx * nty + by, stable=True)
+    return (bx // tile) * nty + by // tile
+
+
+def filler_tile_order(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_y, num_bins_x,
+                      num_bins_y, origin_offset=(0.0, 0.0), tile=64):
+    """Filler-relative permutation (new slot -> old slot) that sorts the
+    fillers by filler_tile_keys; stable, so fillers within a tile keep their
+    relative order."""
+    return torch.argsort(filler_tile_keys(pos, num_nodes, num_fillers, xl, yl, bin_size_x,
+                                          bin_size_y, num_bins_x, num_bins_y, origin_offset, tile),
+                         stable=True)
 
 
 def permute_fillers_(v, num_nodes, num_fillers, perm):
@@ -296,6 +308,7 @@ class PlacementObjective:
         self.side = None
         self._side_stream = None
         self.set_overlap(overlap)
+        self.filler_ranges = None
         self.track_metrics = False
         self.metric_ptr = None
         if track_metrics:
@@ -311,6 +324,37 @@ class PlacementObjective:
             self.metric_out = torch.empty(3, dtype=torch.float64, device=dev)
             self.metric_ws = torch.empty(N.query("dp_wa_workspace_bytes", E), dtype=torch.uint8,
                                          device=dev)
+
+    def filler_origin_offset(self):
+        """The stretch offset of the fillers' origin in the density kernels
+        (one value for fillers of one size), for sorting them by tile."""
+        ep = self.density
+        uf = ep.uniform_filler if ep.use_uniform_filler else None
+        return (uf[2], uf[3]) if uf else (0.0, 0.0)
+
+    def update_filler_ranges(self, pos):
+        """After the caller has put the fillers in tile order
+        (DeviceDesign.reorder_fillers with filler_origin_offset()): the
+        per-tile filler ranges the tiled scatter and gather then take instead
+        of binning the fillers (dp_density_map_ex).  Positions may move
+        afterwards; results stay exact, speed falls as fillers leave their
+        tiles.  Returns False, using none, where they do not apply (not the
+        tiled algorithm, fillers of several sizes or stretched past 2 bins)."""
+        ep, dd = self.density, self.dd
+        uf = ep.uniform_filler if ep.use_uniform_filler else None
+        nf = dd.num_filler_nodes
+        self.filler_ranges = None
+        if (ep.algorithm != "tiled" or nf == 0 or uf is None or uf[0] > 1.99 * ep.bin_size_x
+                or uf[1] > 1.99 * ep.bin_size_y):
+            return False
+        M, Nb = ep.num_bins_x, ep.num_bins_y
+        ntiles = ((M + 63) // 64) * ((Nb + 63) // 64)
+        keys = filler_tile_keys(pos, dd.num_nodes, nf, float(ep.xl), float(ep.yl),
+                                float(ep.bin_size_x), float(ep.bin_size_y), M, Nb, (uf[2], uf[3]))
+        fr = torch.zeros(ntiles + 1, dtype=torch.int32, device=pos.device)
+        fr[1:] = torch.cumsum(torch.bincount(keys, minlength=ntiles), 0)
+        self.filler_ranges = fr
+        return True
 
     def set_overlap(self, on):
         """Run the pins schedule's wirelength forward on a second stream
@@ -366,8 +410,7 @@ class PlacementObjective:
     # -- stages ---------------------------------------------------------------
     def _density_forward(self, pos):
         ep = self.density
-        N.call("dp_density_map_movable" if self.track_metrics else "dp_density_map",
-               N.dtype_code(pos), N.ptr(pos), pos.numel() // 2,
+        N.call("dp_density_map_ex", N.dtype_code(pos), N.ptr(pos), pos.numel() // 2,
                N.ptr(ep.node_size_x_clamped), N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x),
                N.ptr(ep.offset_y), N.ptr(ep.ratio), N.ptr(ep.initial_density_map), float(ep.xl),
                float(ep.yl), float(ep.xh), float(ep.yh), float(ep.bin_size_x),
@@ -375,8 +418,8 @@ class PlacementObjective:
                ep.num_bins_x, ep.num_bins_y,
                N.DP_CONVERT_CPU if ep.convention == "cpu" else N.DP_CONVERT_GPU,
                ep.algorithm_code, ep.large_capacity, ep.uniform_filler_arg(),
-               N.ptr(ep.workspace(pos.device)), N.ptr(self.density_map),
-               N.ptr(self.movable_map) if self.track_metrics else None,
+               N.ptr(self.filler_ranges), N.ptr(ep.workspace(pos.device)),
+               N.ptr(self.density_map), N.ptr(self.movable_map) if self.track_metrics else None,
                N.stream_ptr(pos.device))
         dmap = self.density_map
         if dmap.dtype != self.plan.dtype:

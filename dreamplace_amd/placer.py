@@ -30,7 +30,7 @@ from .nesterov import NesterovAcceleratedGradientOptimizer
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
-                density_factor=1.0, early_stop=True, filler_sort_interval=50,
+                density_factor=1.0, early_stop=True, filler_sort_interval=25,
                 fuse_move_boundary=True, fused_metrics=True)
 
 
@@ -175,9 +175,13 @@ class GlobalPlacer:
     def sort_fillers(self):
         """Relabel the fillers into the density kernels' tile order: the
         positions (the optimizer's parameter), the optimizer's per-node state
-        and the best positions take the same permutation."""
+        and the best positions take the same permutation, and the objective
+        takes the fillers by tile range from then on (no binning of them)."""
         dd = self.dd
-        perm = dd.reorder_fillers()  # relabels dd.pos, which is self.pos
+        ob = self.objective
+        # relabels dd.pos, which is self.pos
+        perm = dd.reorder_fillers(origin_offset=ob.filler_origin_offset())
+        ob.update_filler_ranges(dd.pos)
         n, nf = dd.num_nodes, dd.num_filler_nodes
         fn = lambda v: permute_fillers_(v, n, nf, perm)
         if self.optimizer is not None:

@@ -121,23 +121,36 @@ int dp_density_map(int dtype, const void* pos, int64_t num_nodes, const void* no
                    const double* uniform_filler, void* workspace, void* density_map_out,
                    int64_t* fixed_map_out, void* stream);
 
-/* New: dp_density_map that also writes movable_map_out[num_bins] (`dtype`),
- * the map of the fixed and movable cells alone, converted as density_map_out
- * is: the map ElectricOverflow's density_map forms with num_filler_nodes = 0
- * (electric_overflow.py:260-278, the second scatter NonLinearPlace.py:414
- * runs for the overflow metric), bit-identical, from this one scatter (the
- * fillers go to an int64 map of their own and are added for
- * density_map_out).  Needs DP_SCATTER_TILED with fillers of one size
- * (uniform_filler) or none; else DP_ERR_UNSUPPORTED. */
-int dp_density_map_movable(int dtype, const void* pos, int64_t num_nodes,
-                           const void* node_size_x_clamped, const void* node_size_y_clamped,
-                           const void* offset_x, const void* offset_y, const void* ratio,
-                           const void* initial_density_map, double xl, double yl, double xh,
-                           double yh, double bin_size_x, double bin_size_y,
-                           int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
-                           int num_bins_y, int convention, int algorithm, int64_t large_capacity,
-                           const double* uniform_filler, void* workspace, void* density_map_out,
-                           void* movable_map_out, void* stream);
+/* New: dp_density_map with two options for a placement loop, both nullable.
+ * movable_map_out[num_bins] (`dtype`): the map of the fixed and movable
+ * cells alone, converted as density_map_out is: the map ElectricOverflow's
+ * density_map forms with num_filler_nodes = 0 (electric_overflow.py:260-278,
+ * the second scatter NonLinearPlace.py:414 runs for the overflow metric),
+ * bit-identical, from this one scatter (the fillers go to an int64 map of
+ * their own and are added for density_map_out).
+ * filler_tile_start (device int32[ntiles + 1], ntiles = ceil(M/64) *
+ * ceil(N/64), tile t = (bx/64) * ceil(N/64) + by/64 of a cell's lower-left
+ * bin): the fillers are kept in tile order and [start[t], start[t+1]) are
+ * the (filler-relative) fillers of tile t.  They are then scattered, and
+ * gathered by the next dp_electric_force on this workspace, from their
+ * positions by tile instead of being binned.  The maps and gradients are
+ * bit-identical to the binned path for any ranges: a filler outside its
+ * tile's LDS window (moved since it was sorted) is done through global
+ * memory, and ranges that are not a monotone partition of the fillers are
+ * replaced by one range; only speed depends on how well they fit.
+ * Both need DP_SCATTER_TILED (filler_tile_start is ignored otherwise) and
+ * fillers of one size (uniform_filler) or none; filler_tile_start also needs
+ * their stretched size at most 2 bins.  Else DP_ERR_UNSUPPORTED. */
+int dp_density_map_ex(int dtype, const void* pos, int64_t num_nodes,
+                      const void* node_size_x_clamped, const void* node_size_y_clamped,
+                      const void* offset_x, const void* offset_y, const void* ratio,
+                      const void* initial_density_map, double xl, double yl, double xh,
+                      double yh, double bin_size_x, double bin_size_y,
+                      int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                      int num_bins_y, int convention, int algorithm, int64_t large_capacity,
+                      const double* uniform_filler, const int32_t* filler_tile_start,
+                      void* workspace, void* density_map_out, void* movable_map_out,
+                      void* stream);
 
 /* Replaces electric_potential_cuda.fixed_density_map
  * (electric_density_map_cuda.cpp:139-182, electric_density_map.cpp:155-213)

@@ -236,3 +236,45 @@ def test_tracked_metrics_equal_the_separate_passes(gpu, dtype):
     f1, g1 = ob.evaluate(dd.pos)
     assert torch.equal(ob.density_map, ob2.density_map)
     assert f1.item() == f2.item() and torch.equal(g1, g2)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_filler_ranges_are_exact(gpu, dtype):
+    """Fillers taken by tile range instead of binned: the same density map,
+    objective and gradient bits with freshly sorted fillers, after the fillers
+    have moved (up to 3 bins: some leave their tile's window and take the
+    global path), and with ranges that are not a partition (replaced by one
+    range); with the movable-only map of the tracked metrics too."""
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case engine filler ranges %s" % np.dtype(dtype).name, flush=True)
+    d = design(1, dtype=dtype, scale=0.01)  # 102 x 102 bins: 2 x 2 tiles
+    prec = "bf16x3" if dtype == np.float32 else "fp64"
+    dd = DeviceDesign(d, gpu)
+    kw = dict(precision=prec, algorithm="tiled", schedule="pins", track_metrics=True)
+    ref = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, **kw)
+    rng = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, **kw)
+    dd.reorder_fillers(origin_offset=rng.filler_origin_offset())
+    assert rng.update_filler_ranges(dd.pos)
+    assert ref.filler_ranges is None
+
+    def same():
+        f0, g0 = ref.evaluate(dd.pos)
+        f0, g0, m0, mm0 = f0.item(), g0.clone(), ref.density_map.clone(), ref.movable_map.clone()
+        f1, g1 = rng.evaluate(dd.pos)
+        assert torch.equal(m0, rng.density_map)
+        assert torch.equal(mm0, rng.movable_map)
+        assert f0 == f1.item()
+        assert torch.equal(g0, g1)
+
+    same()
+    n, nf = d.num_nodes, d.num_filler_nodes
+    b = float(d.bin_size_x)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    step = ((torch.rand(2, nf, generator=gen, dtype=torch.float64) - 0.5) * 6 * b).to(gpu, dd.dtype)
+    with torch.no_grad():
+        dd.pos.view(2, n)[:, n - nf:] += step
+    same()
+    bad = rng.filler_ranges.clone()
+    rng.filler_ranges[1] = bad[-1] + 5  # not monotone, past the end
+    same()
+    rng.filler_ranges.copy_(bad)
