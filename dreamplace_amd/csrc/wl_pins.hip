@@ -148,10 +148,46 @@ __device__ __forceinline__ void store_grad(T* gnode, int q, T gx, T gy) {
   ((typename Vec2<T>::type*)gnode)[q] = gv;
 }
 
+// One lane's inputs of one group: the group's slot range and the lane's pin
+// metadata, slot weight and position (zeros past the group's end).
+template <typename T>
+struct GroupLoad {
+  int pb, pe;
+  PinMeta m;
+  T sw;
+  typename Vec2<T>::type v;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_group(GroupLoad<T>& L, int g, int ngroups, int lane,
+                                           const int* __restrict__ group_start,
+                                           const PinMeta* __restrict__ meta,
+                                           const T* __restrict__ slot_w,
+                                           const typename Vec2<T>::type* __restrict__ pin_xy) {
+  L.pb = 0;
+  L.pe = 0;
+  L.m = make_int4(0, 0, 0, 0);
+  L.sw = T(1);
+  L.v.x = 0;
+  L.v.y = 0;
+  if (g < ngroups) {
+    L.pb = group_start[g];
+    L.pe = group_start[g + 1];
+    const int j = L.pb + lane;
+    if (j < L.pe) {
+      L.m = meta[j];
+      if (slot_w) L.sw = slot_w[j];
+      L.v = pin_xy[j];
+    }
+  }
+}
+
 // Four independent wavefronts per workgroup, each with its own LDS slice and
 // its own grid-stride loop over groups; they exchange data only within the
 // wavefront (wave_sync), so no wavefront waits for another until the final
-// workgroup sum.
+// workgroup sum.  Each wavefront issues the loads of its next group before it
+// evaluates the current one, so that their latency overlaps the evaluation
+// (244 -> 206 us per call at 9.6M pins, the same bits).
 // kLse: the log-sum-exp model (logsumexp_wirelength_merged.cpp:14-85) on the
 // same skeleton; g_p then points at gamma instead of 1/gamma.
 template <typename T, bool kLse>
@@ -177,15 +213,20 @@ __global__ void __launch_bounds__(256) k_wa_pins(
   // (contiguous), else gathered per net
   const bool weighted = slot_w != nullptr || w != nullptr;
   double contrib = 0;
-  for (int g = blockIdx.x * kWaves + wv; g < ngroups; g += gridDim.x * kWaves) {
-    const int pb = group_start[g], pe = group_start[g + 1];
-    const int j = pb + lane;
-    const bool on = j < pe;
+  const int gstride = gridDim.x * kWaves;
+  int g = blockIdx.x * kWaves + wv;
+  GroupLoad<T> nxt;
+  load_group(nxt, g, ngroups, lane, group_start, meta, slot_w, pin_xy);
+  for (; g < ngroups; g += gstride) {
+    const GroupLoad<T> cur = nxt;
+    load_group(nxt, g + gstride, ngroups, lane, group_start, meta, slot_w, pin_xy);
+    const int j = cur.pb + lane;
+    const bool on = j < cur.pe;
     int e = 0, head = 0, deg = 0, dst = 0;
     bool live = false, pin_masked = false;
     T x = 0, y = 0, wn = T(1);
     if (on) {
-      const PinMeta m = meta[j];
+      const PinMeta m = cur.m;
       e = m.x & ~kMaskBit;
       pin_masked = m.x < 0;
       dst = m.y;
@@ -193,14 +234,13 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       deg = m.z >> 8;
       if (slot_w) {
         live = m.w != 0;
-        wn = slot_w[j];
+        wn = cur.sw;
       } else {
         live = net_mask[e] != 0;
         if (w) wn = w[e];
       }
-      const typename Vec2<T>::type v = pin_xy[j];
-      x = v.x;
-      y = v.y;
+      x = cur.v.x;
+      y = cur.v.y;
     }
     {
       V v;
