@@ -94,6 +94,7 @@ struct GemmBatch {
 constexpr int BT = 64;          // block tile (M and N)
 constexpr int kXcds = 8;        // accelerator dies; workgroup ids go round-robin over them
 constexpr int kGroupRows = 4;   // tile rows per raster band
+constexpr int kLargeGridTiles = 256;  // 128 x 128 tiles per launch from which the BF16x3 GEMM uses them
 
 struct TileIdx {
   int g, ty, tx;
@@ -340,6 +341,136 @@ __global__ void __launch_bounds__(256 * SPLIT) k_gemm_bf16x3(GemmBatch bt) {
   }
 }
 
+// BF16x3 on 128 x 128 block tiles, for large grids.  The 64 x 64 tiles
+// above read each A row panel and B column panel once per tile, through L2:
+// at 2048^2 that is 1 GB per GEMM and L2 bandwidth sets the time.  Here each
+// of 4 wavefronts owns a 64 x 64 tile (2 x 2 MFMA tiles), which halves the
+// L2 traffic and reads each LDS fragment for two MFMA tiles.  The two halves
+// of the 512-thread workgroup take the two halves of K as above.  Every
+// output element sees the same MFMA sequence (the same K order, lo*hi, hi*lo,
+// hi*hi per K step, the same K-half addition) as in k_gemm_bf16x3<2>, so
+// the two kernels give the same bits.
+constexpr int BT2 = 128;
+
+__global__ void __launch_bounds__(512) k_gemm_bf16x3_t128(GemmBatch bt) {
+  const TileIdx ti = map_tile(bt);
+  if (!ti.ok) return;  // whole workgroup: before any barrier
+  const GemmDesc& gd = desc_of(bt, ti.g);
+  const float* A = (const float*)gd.A;
+  const float* B = (const float*)gd.B;
+  float* C = (float*)gd.C;
+  const int N = bt.N, K = bt.K;
+  const int m0 = ti.ty * BT2, n0 = ti.tx * BT2;
+  constexpr int kTileElems = BT2 * BF_LD;
+  // [half][Ah, Al, Bh, Bl]; one array so the K-half reduction can reuse it
+  __shared__ __attribute__((aligned(16))) __bf16 lds[2][4][kTileElems];
+  const int half = threadIdx.x >> 8;
+  const int t = threadIdx.x & 255, lane = t & 63, wid = t >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  __bf16* Ah = lds[half][0];
+  __bf16* Al = lds[half][1];
+  __bf16* Bh = lds[half][2];
+  __bf16* Bl = lds[half][3];
+  const int kspan = K / 2, kbase = half * kspan;
+  f32x16 acc[2][2] = {};
+  // A: 32 consecutive k of one row; B: 32 consecutive k of one column (each
+  // B load coalesced across lanes).  The next K tile is loaded into
+  // registers while the current one is computed.
+  const int a_row = t >> 1, a_kc = (t & 1) * 32;
+  const int b_col = t & 127, b_kc = (t >> 7) * 32;
+  float av[32], bv[32];
+  auto load = [&](int k0) {
+    const float4* ap = (const float4*)(A + (size_t)(m0 + a_row) * K + k0 + a_kc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float4 v = ap[j];
+      av[4 * j] = v.x;
+      av[4 * j + 1] = v.y;
+      av[4 * j + 2] = v.z;
+      av[4 * j + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 32; ++j) bv[j] = B[(size_t)(k0 + b_kc + j) * N + n0 + b_col];
+  };
+  load(kbase);
+  for (int k0 = kbase; k0 < kbase + kspan; k0 += BF_BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bf16x8 hi, lo;
+      split8(av + 8 * q, hi, lo);
+      *(bf16x8*)(Ah + a_row * BF_LD + a_kc + 8 * q) = hi;
+      *(bf16x8*)(Al + a_row * BF_LD + a_kc + 8 * q) = lo;
+      split8(bv + 8 * q, hi, lo);
+      *(bf16x8*)(Bh + b_col * BF_LD + b_kc + 8 * q) = hi;
+      *(bf16x8*)(Bl + b_col * BF_LD + b_kc + 8 * q) = lo;
+    }
+    __syncthreads();
+    if (k0 + BF_BK < kbase + kspan) load(k0 + BF_BK);
+#pragma unroll
+    for (int s = 0; s < BF_BK / 16; ++s) {
+      const int ko = 16 * s + 8 * h;
+      bf16x8 xh[2], xl[2], yh[2], yl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int ra = (wm * 64 + 32 * i + l32) * BF_LD + ko;
+        const int rb = (wn * 64 + 32 * i + l32) * BF_LD + ko;
+        xh[i] = *(const bf16x8*)(Ah + ra);
+        xl[i] = *(const bf16x8*)(Al + ra);
+        yh[i] = *(const bf16x8*)(Bh + rb);
+        yl[i] = *(const bf16x8*)(Bl + rb);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl[i], yh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh[i], yl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh[i], yh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+  // the upper half's accumulators (4 waves x 64 x 64 floats = 64 KB) through
+  // the tile arrays (no longer read), added to the lower half's
+  static_assert(sizeof(lds) >= 4 * 64 * 64 * sizeof(float), "reduction buffer");
+  float* red = (float*)&lds[0][0][0];
+  __syncthreads();
+  if (half == 1) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) red[(((i * 2 + j) * 16 + r) * 4 + wid) * 64 + lane] = acc[i][j][r];
+  }
+  __syncthreads();
+  const float alpha = (float)bt.alpha;
+  const float* R = (const float*)gd.R;
+  double e = 0;
+  if (half == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 64 + 32 * j + l32;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float a = acc[i][j][r] + red[(((i * 2 + j) * 16 + r) * 4 + wid) * 64 + lane];
+          const int row = m0 + wm * 64 + 32 * i + 8 * (r >> 2) + 4 * h + (r & 3);
+          const float v = alpha * a;
+          C[(size_t)row * N + col] = v;
+          if (R) e += (double)v * (double)R[(size_t)row * N + col];
+        }
+      }
+  }
+  if (R) {
+    __shared__ double smem[8];
+    double bs = block_sum<512>(e, smem);
+    if (threadIdx.x == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
+  }
+}
+
 // ---- FP64: v_mfma_f64_16x16x4_f64, K tile 16 --------------------------------
 // Each wave owns a 32x32 tile = 2x2 MFMA tiles.  LDS rows hold 16 k values
 // permuted as pos(k) = (k&3)*4 + (k>>2), so lane quarter q (= lane/16) reads
@@ -465,7 +596,18 @@ static bool mfma_shape(int M, int N) { return M % BT == 0 && N % BT == 0; }
 // per-GEMM energy partial blocks.
 static int launch_gemm(int precision, GemmBatch& bt, int count, hipStream_t s, int* nblocks) {
   dim3 block(256);
-  if (mfma_shape(bt.M, bt.N) && bt.K % BT == 0) {
+  // 128 x 128 tiles where they still give every CU a workgroup
+  if (precision == DP_SPECTRAL_BF16X3 && bt.M % BT2 == 0 && bt.N % BT2 == 0 &&
+      bt.K % (2 * BF_BK) == 0 &&
+      (int64_t)(bt.M / BT2) * (bt.N / BT2) * count >= kLargeGridTiles) {
+    bt.TY = bt.M / BT2;
+    bt.TX = bt.N / BT2;
+    bt.count = count;
+    const int64_t tiles = (int64_t)bt.TY * bt.TX * count;
+    if (nblocks) *nblocks = bt.TY * bt.TX;
+    hipLaunchKernelGGL(k_gemm_bf16x3_t128, dim3((unsigned)(kXcds * ceil_div(tiles, kXcds))),
+                       dim3(512), 0, s, bt);
+  } else if (mfma_shape(bt.M, bt.N) && bt.K % BT == 0) {
     bt.TY = bt.M / BT;
     bt.TX = bt.N / BT;
     bt.count = count;

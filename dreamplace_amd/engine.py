@@ -487,11 +487,22 @@ class PlacementObjective:
                self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
                N.ptr(self._density_grad), N.stream_ptr(pos.device))
 
-    def evaluate(self, pos, precondition=None):
+    def evaluate(self, pos, precondition=None, grad_out=None):
         """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers.
         precondition = (pin_weight_sum, node_areas, alpha) also applies
         PlaceObj's preconditioner to the gradient (with the current density
-        weight), fused into the node-gradient pass on the pins schedule."""
+        weight), fused into the node-gradient pass on the pins schedule.
+        grad_out: a contiguous [2N] tensor of the positions' type and device
+        that receives the gradient (and is returned) instead of the
+        objective's own buffer, so a caller that keeps gradients from step to
+        step needs no copy."""
+        grad = self.grad
+        if grad_out is not None:
+            if (grad_out.dtype != self.grad.dtype or grad_out.device != self.grad.device
+                    or grad_out.numel() != self.grad.numel() or not grad_out.is_contiguous()):
+                raise ValueError("evaluate: grad_out must be a contiguous %s tensor of %d elements on %s"
+                                 % (self.grad.dtype, self.grad.numel(), self.grad.device))
+            grad = grad_out
         dd = self.dd
         code = N.dtype_code(pos)
         s = N.stream_ptr(pos.device)
@@ -541,12 +552,12 @@ class PlacementObjective:
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),
                        N.ptr(pp.grad_node_order), N.ptr(dd.flat_node2pin_start_map),
                        N.ptr(self._density_grad), self.grad_layout, dd.num_physical_nodes, n,
-                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(self.grad), s)
+                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(grad), s)
                 precondition = None  # done
             else:
                 N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
                        N.ptr(dd.flat_node2pin_start_map), N.ptr(self._density_grad),
-                       self.grad_layout, dd.num_physical_nodes, n, N.ptr(self.grad), s)
+                       self.grad_layout, dd.num_physical_nodes, n, N.ptr(grad), s)
         elif self.schedule == "fused":
             N.call("dp_wa_forward_from_nodes", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), P, N.ptr(dd.flat_net2pin_map),
@@ -558,7 +569,7 @@ class PlacementObjective:
             N.call("dp_wa_backward_to_nodes", code, N.ptr(self.one), N.ptr(self.grad_intermediate),
                    P, N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    N.ptr(dd.pin2net_map), N.ptr(self.w), N.ptr(self.net_mask), N.ptr(self.pin_mask),
-                   N.ptr(self._density_grad), dd.num_physical_nodes, n, N.ptr(self.grad), s)
+                   N.ptr(self._density_grad), dd.num_physical_nodes, n, N.ptr(grad), s)
         else:
             N.call("dp_pin_pos_forward", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), N.DP_INDEX_INT32, P,
@@ -576,13 +587,13 @@ class PlacementObjective:
             N.call("dp_pin_pos_backward", code, N.ptr(self.grad_intermediate), P,
                    N.ptr(dd.flat_node2pin_map), N.ptr(dd.flat_node2pin_start_map),
                    dd.num_physical_nodes, n, N.ptr(self.wl_grad), s)
-            torch.add(self.wl_grad, self._density_grad, out=self.grad)
+            torch.add(self.wl_grad, self._density_grad, out=grad)
         if precondition is not None:
             pws, areas, alpha = precondition
-            N.call("dp_precondition", code, N.ptr(self.grad), n, N.ptr(pws), N.ptr(areas),
+            N.call("dp_precondition", code, N.ptr(grad), n, N.ptr(pws), N.ptr(areas),
                    N.ptr(self.lam), float(alpha), s)
         energy = self._energy(pos)
         if self.quad_coeff is not None:
             energy = energy * (1 + self.quad_coeff * energy)
         torch.add(self.wl, energy, alpha=self.density_factor * self.lam_host, out=self.obj)
-        return self.obj, self.grad
+        return self.obj, grad

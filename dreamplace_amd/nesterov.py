@@ -31,13 +31,17 @@ class _Norms:
 
 class NesterovAcceleratedGradientOptimizer(Optimizer):
     def __init__(self, params, lr=required, obj_and_grad_fn=required, constraint_fn=None,
-                 use_bb=True, bounds=None):
+                 use_bb=True, bounds=None, grad_into=False):
         """bounds (new, optional): the move_boundary box as a dict with
         node_size_x, node_size_y (device), xl, yl, xh, yh, num_movable_nodes,
         num_filler_nodes and filler_size ((w, h) shared by all fillers, or
         None).  The extrapolation then clamps v_kp1 in the same pass
         (dp_nesterov_extrapolate_bounded) and constraint_fn, which must be
-        that same clamp, is not called after it."""
+        that same clamp, is not called after it.
+        grad_into (new, optional): obj_and_grad_fn also takes grad_out=, a
+        buffer that receives the gradient.  step_nobb then evaluates into the
+        storage of g_k_1 (which the line search no longer reads) and keeps
+        it as g_k, with no copy of the gradient."""
         if lr is not required and lr < 0.0:
             raise ValueError("Invalid learning rate: {}".format(lr))
         defaults = dict(lr=lr, u_k=[], v_k=[], g_k=[], obj_k=[], a_k=[], alpha_k=[], v_k_1=[],
@@ -47,6 +51,7 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         self.constraint_fn = constraint_fn
         self.use_bb = use_bb
         self.bounds = bounds
+        self.grad_into = grad_into
         if len(self.param_groups) != 1 or len(self.param_groups[0]["params"]) != 1:
             raise ValueError("Only parameters with single tensor is supported")
         self._norms = None
@@ -117,7 +122,10 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         while True:
             self._extrapolate(v_k.data, g_k, u_k, alpha_k, a_k, v_kp1)
             self._constrain(v_kp1)
-            f_kp1, g_kp1 = f(v_kp1)
+            if self.grad_into:
+                f_kp1, g_kp1 = f(v_kp1, grad_out=g_k_1)
+            else:
+                f_kp1, g_kp1 = f(v_kp1)
             s = self._norms(v_kp1, v_k.data, g_kp1, g_k)
             alpha_kp1 = (s[0] / s[1]).sqrt().to(g_k.dtype).reshape(1)
             cnt += 1
@@ -132,11 +140,13 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         a_k.copy_(self._a_kp1)
         # v_k_1 <- v_k, v_k <- v_kp1, g_k_1 <- g_k, g_k <- g_kp1, u_k <- u_kp1
         # (lines 151-162) by rotating storage: only the objective's reused
-        # gradient buffer is copied.  The parameter keeps its identity and
+        # gradient buffer is copied, and with grad_into the gradient is
+        # already in g_k_1's storage.  The parameter keeps its identity and
         # takes v_kp1's storage.
         self._rotate_positions(group, v_k, v_kp1)
         group["g_k_1"][0], group["g_k"][0] = g_k, g_k_1
-        g_k_1.copy_(g_kp1)
+        if g_kp1.data_ptr() != g_k_1.data_ptr():
+            g_k_1.copy_(g_kp1)
         group["u_k"][0], self._u_kp1 = self._u_kp1, u_k
         return loss
 

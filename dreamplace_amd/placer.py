@@ -92,6 +92,70 @@ def density_weight_factor(cur_hpwl, prev_hpwl, flat_iteration, p):
                                  p["upper_pcof"])
 
 
+class BestPositions:
+    """best_pos of NonLinearPlace.py:350-357 (the positions after the step of
+    the iteration with the lowest overflow so far, restored when the loop
+    diverges), kept without copying them while they are the current ones.
+
+    The reference copies the positions at every improving iteration, which
+    is most of them.  Here an improving iteration only notes that the best
+    positions are the parameter's values (`pending`).  After the next step,
+    step_nobb's storage rotation has moved those values into its v_k_1,
+    which it does not read after its first step; if that step did not
+    improve, this class takes that storage and gives the optimizer its old
+    buffer instead.  Whoever changes the positions outside the optimizer
+    calls `materialize` first.  With the Barzilai-Borwein step (which reads
+    v_k_1 in every step) it copies, as the reference does."""
+
+    def __init__(self, optimizer, defer):
+        self.opt = optimizer
+        self.defer = defer
+        self.buf = None
+        self.pending = False
+
+    def improved(self, x):
+        """The iteration just stepped improved: the best positions are x."""
+        if self.defer:
+            self.pending = True
+        else:
+            self._copy(x)
+
+    def not_improved(self):
+        """The iteration just stepped did not improve."""
+        if self.pending:
+            group = self.opt.param_groups[0]
+            prev = group["v_k_1"][0]  # the parameter's values before this step
+            if self.buf is None:
+                self.buf = torch.empty_like(prev)
+            self.buf, group["v_k_1"][0] = prev, self.buf
+            self.pending = False
+
+    def materialize(self, x):
+        """Before x changes outside the optimizer."""
+        if self.pending:
+            self._copy(x)
+            self.pending = False
+
+    def restore(self, x):
+        """x <- the best positions."""
+        if not self.pending:
+            x.copy_(self.buf)
+
+    def permute_(self, fn):
+        if self.buf is not None and not self.pending:
+            fn(self.buf)
+
+    def value(self, x):
+        """The best positions (x itself while they are the current ones)."""
+        return x if self.pending else self.buf
+
+    def _copy(self, x):
+        if self.buf is None:
+            self.buf = x.clone()
+        else:
+            self.buf.copy_(x)
+
+
 class GlobalPlacer:
     """`d`: a design dict (dreamplace_amd.synth layout).  Objective options
     (precision, algorithm, schedule) go to PlacementObjective.  `params`
@@ -158,7 +222,7 @@ class GlobalPlacer:
         self.pos = dd.pos
         self.history = []
         self.best = None
-        self.best_pos = None
+        self.best_positions = BestPositions(None, defer=False)
         self.rolled_back = False
         self.perturbations = 0
         self.optimizer = None
@@ -186,8 +250,7 @@ class GlobalPlacer:
         fn = lambda v: permute_fillers_(v, n, nf, perm)
         if self.optimizer is not None:
             self.optimizer.permute_state_(fn, include_param=False)
-        if self.best_pos is not None:
-            fn(self.best_pos)
+        self.best_positions.permute_(fn)
         self.filler_sorts += 1
 
     def boundary_box(self):
@@ -220,16 +283,17 @@ class GlobalPlacer:
             c = float(T(self.p["density_quad_coeff"] / 2) * T(inv))
         self.objective.set_density_penalty(self.density_factor, c)
 
-    def obj_and_grad(self, x):
-        """PlaceObj.obj_and_grad_fn: objective, gradient, preconditioner."""
+    def obj_and_grad(self, x, grad_out=None):
+        """PlaceObj.obj_and_grad_fn: objective, gradient, preconditioner
+        (the gradient into grad_out when given)."""
         pre = (self.pws, self.areas, self.p["precondition_alpha"])
-        f, g = self.objective.evaluate(x, precondition=pre)
+        f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out)
         if self.init_density is None:
             # PlaceObj.obj_fn records the density of its first evaluation
             self.init_density = float(self.objective._energy(x).item())
             if self.quad_penalty:
                 self._penalty()
-                f, g = self.objective.evaluate(x, precondition=pre)
+                f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out)
         return f, g
 
     def initialize_density_weight(self, x):
@@ -320,7 +384,12 @@ class GlobalPlacer:
         lr = self.estimate_learning_rate(x)
         self.optimizer = NesterovAcceleratedGradientOptimizer(
             [x], lr=lr, obj_and_grad_fn=self.obj_and_grad, constraint_fn=self.constraint,
-            use_bb=p["use_bb"], bounds=self.boundary_box() if p["fuse_move_boundary"] else None)
+            use_bb=p["use_bb"], bounds=self.boundary_box() if p["fuse_move_boundary"] else None,
+            grad_into=True)
+        # a best kept from an earlier run lives in storage of that run's optimizer
+        self.best_positions.materialize(x)
+        self.best_positions.opt = self.optimizer
+        self.best_positions.defer = not p["use_bb"]
         hist = self.history
         T = np.float32 if self.dd.dtype == torch.float32 else np.float64
         overflow_list, divergence_list = [1], []
@@ -335,7 +404,7 @@ class GlobalPlacer:
             if (p["early_stop"] and p["stop_overflow"] * 1.1 < overflow_list[-1] < p["stop_overflow"] * 4
                     and check_divergence(divergence_list, self.best, p["stop_overflow"], window=50,
                                          threshold=overflow_list[-1])):
-                x.copy_(self.best_pos)
+                self.best_positions.restore(x)
                 self.objective.metric_ptr = None
                 constrained = False
                 stop = True
@@ -354,10 +423,9 @@ class GlobalPlacer:
             constrained = True
             if self.best is None or self.best["overflow"] > cur["overflow"]:
                 self.best = cur
-                if self.best_pos is None:
-                    self.best_pos = x.clone()
-                else:
-                    self.best_pos.copy_(x)
+                self.best_positions.improved(x)
+            else:
+                self.best_positions.not_improved()
             overflow_list.append(cur["overflow"])
             divergence_list.append([cur["hpwl"], cur["overflow"]])
             if it - last_perturb > min_perturb_interval and check_plateau(overflow_list, 15, 0.001):
@@ -367,6 +435,7 @@ class GlobalPlacer:
                     self._penalty()
                     if overflow_list[-1] > 0.95:
                         intensity = min(max(40 + (120 - 40) * (overflow_list[-1] - 0.95) * 10, 40), 90)
+                        self.best_positions.materialize(x)
                         self.entropy_injection(x, 0.996, intensity)
                         constrained = False
                     last_perturb = it

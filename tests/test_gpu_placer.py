@@ -37,6 +37,107 @@ def test_nesterov_matches_oracle_on_quadratic(gpu, use_bb):
         assert opt.backtracks == ref.backtracks
 
 
+def test_nesterov_gradient_into_caller_buffer(gpu):
+    """grad_into: the line search evaluates into g_k_1's storage and keeps
+    it as g_k; the iterates are the same bits as with the copy, and the
+    gradient lands in buffers the optimizer owns."""
+    from dreamplace_amd.nesterov import NesterovAcceleratedGradientOptimizer
+    print("case nesterov grad_into", flush=True)
+    n = 5000
+    a = torch.from_numpy(np.random.default_rng(0).uniform(0.5, 20.0, n)).to(gpu)
+    x0 = torch.from_numpy(np.random.default_rng(1).standard_normal(n)).to(gpu)
+    own = torch.empty(n, dtype=torch.float64, device=gpu)
+    targets = []
+
+    def f_copy(x):
+        torch.mul(a, x, out=own)
+        return 0.5 * torch.sum(a * x * x), own
+
+    def f_into(x, grad_out=None):
+        g = own if grad_out is None else grad_out
+        if grad_out is not None:
+            targets.append(grad_out.data_ptr())
+        torch.mul(a, x, out=g)
+        return 0.5 * torch.sum(a * x * x), g
+
+    xa, xb = x0.clone(), x0.clone()
+    oa = NesterovAcceleratedGradientOptimizer([xa], lr=0.01, obj_and_grad_fn=f_copy, use_bb=False)
+    ob = NesterovAcceleratedGradientOptimizer([xb], lr=0.01, obj_and_grad_fn=f_into, use_bb=False,
+                                              grad_into=True)
+    for _ in range(25):
+        oa.step()
+        ob.step()
+        assert torch.equal(xa, xb)
+        assert torch.equal(oa.param_groups[0]["g_k"][0], ob.param_groups[0]["g_k"][0])
+    assert oa.backtracks == ob.backtracks
+    assert targets and own.data_ptr() not in targets
+    assert set(targets) <= {ob.param_groups[0]["g_k"][0].data_ptr(),
+                            ob.param_groups[0]["g_k_1"][0].data_ptr()}
+
+
+def test_best_positions_without_copies(gpu):
+    """BestPositions with deferred copies (storage taken from the
+    optimizer's v_k_1) holds the same best positions as copying at every
+    improvement, through relabellings, roll-backs and injected noise, and
+    leaves the optimizer's iterates unchanged."""
+    from dreamplace_amd.nesterov import NesterovAcceleratedGradientOptimizer
+    from dreamplace_amd.placer import BestPositions
+    print("case best positions deferred", flush=True)
+    n = 4000
+    a = torch.from_numpy(np.random.default_rng(0).uniform(0.5, 20.0, n)).to(gpu)
+    x0 = torch.from_numpy(np.random.default_rng(1).standard_normal(n)).to(gpu)
+
+    def f(x, grad_out=None):
+        g = torch.empty_like(x) if grad_out is None else grad_out
+        torch.mul(a, x, out=g)
+        return 0.5 * torch.sum(a * x * x), g
+
+    xa, xb = x0.clone(), x0.clone()
+    oa = NesterovAcceleratedGradientOptimizer([xa], lr=0.01, obj_and_grad_fn=f, use_bb=False,
+                                              grad_into=True)
+    ob = NesterovAcceleratedGradientOptimizer([xb], lr=0.01, obj_and_grad_fn=f, use_bb=False,
+                                              grad_into=True)
+    bp = BestPositions(oa, defer=True)
+    ref = None
+    rng = np.random.default_rng(3)
+    gen = torch.Generator().manual_seed(4)
+    events = {"restore": 0, "permute": 0, "inject": 0, "taken": 0}
+    for it in range(80):
+        if ref is not None and rng.random() < 0.1:
+            bp.restore(xa)
+            xb.copy_(ref)
+            assert torch.equal(xa, ref)
+            events["restore"] += 1
+        if rng.random() < 0.1:
+            perm = torch.randperm(n, generator=gen).to(gpu)
+            fn = lambda v: v.copy_(v[perm])
+            for x, o in ((xa, oa), (xb, ob)):
+                fn(x.data)
+                o.permute_state_(fn, include_param=False)
+            bp.permute_(fn)
+            if ref is not None:
+                ref = ref[perm]
+            events["permute"] += 1
+        oa.step()
+        ob.step()
+        assert torch.equal(xa, xb)
+        if it == 0 or rng.random() < 0.6:
+            bp.improved(xa)
+            ref = xa.detach().clone()
+        else:
+            events["taken"] += int(bp.pending)
+            bp.not_improved()
+        if rng.random() < 0.1:
+            bp.materialize(xa)
+            noise = torch.randn(n, generator=gen, dtype=torch.float64).to(gpu) * 1e-3
+            xa.data.add_(noise)
+            xb.data.add_(noise)
+            events["inject"] += 1
+        assert torch.equal(bp.value(xa), ref), it
+    print("  events", events, flush=True)
+    assert all(v > 0 for v in events.values())
+
+
 def test_placer_matches_oracle(gpu):
     """Eight iterations of the placement loop on config 0 (FP64, FP64 solve):
     the same metrics, density weights, gammas, line-search counts and

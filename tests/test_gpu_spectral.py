@@ -50,3 +50,27 @@ def test_poisson_solve(oracle, gpu, precision, tol):
     # spectral-domain energy (no fourth transform) gives the same value
     _, _, _, e2 = plan.poisson(dt, d.bin_size_x, d.bin_size_y, False, True)
     assert abs(e2.item() - ref["energy"]) <= tol * abs(ref["energy"])
+
+
+@pytest.mark.parametrize("shape", [(2048, 2048), (2048, 4096)], ids=["2048x2048", "2048x4096"])
+def test_large_grid_bf16x3_against_fp64(gpu, shape):
+    """Grids with at least 256 tiles of 128 x 128 take the 128-tile BF16x3
+    GEMM: the DCT2 and the Poisson solve (fields, potential and the energy
+    from the GEMM epilogue) stay within the accuracy gate of the FP64 form on
+    the device (whose kernels the oracle pins at the sizes above)."""
+    from dreamplace_amd.spectral import get_plan
+    print("case large grid bf16x3 %s" % (shape,), flush=True)
+    M, N = shape
+    g = torch.Generator().manual_seed(7)
+    x64 = (torch.rand(M, N, generator=g, dtype=torch.float64) * 2.0).to(gpu)
+    p64 = get_plan(M, N, "fp64", gpu)
+    p16 = get_plan(M, N, "bf16x3", gpu)
+    x32 = x64.float()
+    assert rel_l2(p16.dct2(x32).double().cpu().numpy(), p64.dct2(x64).cpu().numpy()) <= 2e-5
+    if M != N:
+        return
+    r64 = p64.poisson(x64, 1.5, 2.0, True, True)
+    r16 = p16.poisson(x32, 1.5, 2.0, True, True)
+    for a, b in zip(r16[:3], r64[:3]):
+        assert rel_l2(a.double().cpu().numpy(), b.cpu().numpy()) <= 1e-5
+    assert abs(r16[3].item() - r64[3].item()) <= 1e-5 * abs(r64[3].item())
