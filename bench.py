@@ -48,8 +48,8 @@ DP_OP_DENSITY_MAP, DP_OP_ELECTRIC_FORCE, DP_OP_WA_FORWARD, DP_OP_NODE_GRAD, DP_O
 # kernels of each op in the pins schedule with tiled density (rocprof names)
 OP_KERNELS = {
     "wa_forward": ["k_pin_push", "k_wa_pins", "k_wa_big", "k_finish"],
-    "density_map": ["k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill", "k_tile_scatter_f32",
-                    "k_drift_scatter", "k_rec_scatter_large", "k_convert_cpu"],
+    "density_map": ["k_zero_words", "k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill",
+                    "k_tile_scatter_f32", "k_drift_scatter", "k_rec_scatter_large", "k_convert_cpu"],
     "poisson": ["k_sum_partial", "k_scale_centre", "k_gemm_bf16x3", "k_spectral_weights"],
     "electric_force": ["k_fill", "k_tile_gather", "k_drift_gather", "k_rec_gather_large"],
     "node_grad": ["k_node_grad"],
@@ -280,7 +280,7 @@ def summarize_trace(trace_dir, warmup, steps):
 # gather, one k_finish to the wirelength reduction)
 TRAFFIC_KERNELS = {
     "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<", "dp::k_finish"],
-    "density_map": ["dp::k_fill<unsigned long long>", "dp::k_bin_count<", "dp::k_bin_scan",
+    "density_map": ["dp::k_zero_words", "dp::k_bin_count<", "dp::k_bin_scan",
                     "dp::k_bin_fill<", "dp::k_tile_scatter", "dp::k_drift_scatter<",
                     "dp::k_rec_scatter_large<", "dp::k_convert_cpu<"],
     "electric_force": ["dp::k_fill<float>", "dp::k_tile_gather<", "dp::k_drift_gather<",

@@ -359,7 +359,7 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
       hipLaunchKernelGGL(k_quantize<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, init, nb,
                          (T)scale, w.map);
     } else {
-      hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map, nb, 0ull);
+      zero_words(w.map, nb, s);
     }
     DP_LAUNCHED("density init");
     unsigned long long* fmap = nullptr;
@@ -367,8 +367,7 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
       // the fillers' contributions in a map of their own: w.map then holds the
       // fixed and movable cells' alone, the map ElectricOverflow forms
       fmap = w.filler_map;
-      hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s,
-                         fmap, nb, 0ull);
+      zero_words(fmap, nb, s);
       DP_LAUNCHED("filler map init");
     }
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y)) {
@@ -455,7 +454,7 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
   unsigned long long* map = (unsigned long long*)workspace;
   return DP_DISPATCH(dtype, "dp_fixed_density_map", [&] {
     const T* P = (const T*)pos;
-    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(nb, 256)), dim3(256), 0, s, map, nb, 0ull);
+    zero_words(map, nb, s);
     if (num_terminals > 0) {
       const int64_t b = num_movable_nodes;
       hipLaunchKernelGGL(k_exact_scatter<T>, dim3((unsigned)ceil_div(num_terminals, 256)),

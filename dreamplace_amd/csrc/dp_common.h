@@ -183,6 +183,25 @@ __global__ void k_fill(V* p, int64_t n, V v) {
     p[i] = v;
 }
 
+// Zero n 8-byte words at a 16-byte aligned p with 16-byte stores (the
+// fixed-point maps: one 8-byte store per thread ran at 1.5 TB/s on 8 MB).
+static __global__ void k_zero_words(unsigned long long* p, int64_t n) {
+  const int64_t pairs = n >> 1;
+  uint4* q = (uint4*)p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs;
+       i += (int64_t)gridDim.x * blockDim.x)
+    q[i] = make_uint4(0u, 0u, 0u, 0u);
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) p[n - 1] = 0ull;
+}
+
+// Launch the zeroing of n words at p on s (p 16-byte aligned, else word stores).
+inline void zero_words(unsigned long long* p, int64_t n, hipStream_t s) {
+  if (((uintptr_t)p & 15) == 0)
+    hipLaunchKernelGGL(k_zero_words, dim3(grid_for((n >> 1) > 0 ? (n >> 1) : 1, 256)), dim3(256), 0, s, p, n);
+  else
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, 0ull);
+}
+
 // Two-stage deterministic reduction helpers (reduce.hip).
 constexpr int kReduceBlock = 256;
 constexpr int kReduceMaxBlocks = 16384;

@@ -131,13 +131,17 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
             cnt += 1
             group["obj_eval_count"] += 1
             done = bool((alpha_kp1 > 0.95 * alpha_k).item()) or cnt >= 10
-            alpha_k.copy_(alpha_kp1)
+            # alpha_k <- alpha_kp1 (a fresh tensor: taken, not copied)
+            alpha_k = alpha_kp1
+            group["alpha_k"][0] = alpha_k
             if done:
                 break
         self.backtracks.append(cnt)
-        group["obj_k_1"][0].copy_(group["obj_k"][0])
+        # obj_k_1 <- obj_k, obj_k <- f_kp1 (the objective's reused buffer),
+        # a_k <- a_kp1: the optimizer's own buffers swap instead of copying
+        group["obj_k_1"][0], group["obj_k"][0] = group["obj_k"][0], group["obj_k_1"][0]
         group["obj_k"][0].copy_(f_kp1)
-        a_k.copy_(self._a_kp1)
+        group["a_k"][0], self._a_kp1 = self._a_kp1, a_k
         # v_k_1 <- v_k, v_k <- v_kp1, g_k_1 <- g_k, g_k <- g_kp1, u_k <- u_kp1
         # (lines 151-162) by rotating storage: only the objective's reused
         # gradient buffer is copied, and with grad_into the gradient is
