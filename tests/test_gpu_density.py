@@ -11,6 +11,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     (0, {}, "config0"),
+    # an odd bin count: the map clear's 16-byte stores leave one 8-byte tail
+    (0, {"bins": 45}, "config0-45x45bins"),
     (1, {"scale": 0.002}, "config1-small"),
     (4, {"scale": 0.004, "max_fillers": 200000}, "config4-small-macros"),
 ]
