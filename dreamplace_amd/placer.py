@@ -93,7 +93,7 @@ def density_weight_factor(cur_hpwl, prev_hpwl, flat_iteration, p):
 
 
 class BestPositions:
-    """best_pos of NonLinearPlace.py:350-357 (the positions after the step of
+    """best_pos of NonLinearPlace.py:518-524,640-647 (the positions after the step of
     the iteration with the lowest overflow so far, restored when the loop
     diverges), kept without copying them while they are the current ones.
 
