@@ -454,6 +454,9 @@ def main():
     ap.add_argument("--energy-mode", default="spectral", choices=["spectral", "potential"],
                     help="energy from the DCT coefficients (Parseval; no potential map) or "
                          "from the materialised potential map as the reference does")
+    ap.add_argument("--movable-order", default="netlist", choices=["netlist", "generation"],
+                    help="movable cells relabelled in the wirelength pass's netlist order "
+                         "(same objective bits) or kept in the generated order")
     ap.add_argument("--filler-order", default="tile", choices=["tile", "generation"],
                     help="tile: fillers relabelled into the density kernels' tile order before the "
                          "timed steps and taken by tile range (as the placer keeps them); "
@@ -505,7 +508,7 @@ def main():
 
     from dreamplace_amd import _native as N
     from dreamplace_amd import synth
-    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective, movable_netlist_order
     dev = torch.device("cuda:0")
     N.lib()
 
@@ -515,6 +518,8 @@ def main():
     gamma = d.wa_gamma
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
+    if args.movable_order == "netlist":
+        dd32.reorder_movables(movable_netlist_order(dd32))
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
                                algorithm=args.algorithm, schedule=args.schedule,
                                energy_mode=args.energy_mode)
@@ -572,6 +577,8 @@ def main():
                       ops_timing="each op alone, every kernel on one stream")
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)
+        if args.movable_order == "netlist":
+            dd64.reorder_movables(movable_netlist_order(dd64))
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
                                    algorithm=args.algorithm, schedule=args.schedule,
                                    energy_mode=args.energy_mode)
@@ -611,6 +618,7 @@ def main():
             "form": "fp32 + %s spectral solve, %s schedule, %s density, %s energy" % (
                 args.precision, args.schedule, args.algorithm, args.energy_mode),
             "filler_order": args.filler_order,
+            "movable_order": args.movable_order,
             "ops": ops,
             **result,
         },

@@ -23,11 +23,50 @@ autograd graph or host synchronisation.  Three schedules:
 Both give the same objective and gradient (the fused kernels keep the
 per-element arithmetic and summation order).
 """
+import ctypes
+
 import numpy as np
 import torch
 
 from . import _native as N
 from .electric_potential import ElectricPotential
+
+
+def pack_nets(netpin_start):
+    """dp_wa_pack_nets on a host CSR start array: (group_start, num_groups,
+    big, num_big, net_slot), net_slot[e] the first group-order slot of net e."""
+    start = np.ascontiguousarray(netpin_start, dtype=np.int32)
+    E = len(start) - 1
+    group_start = np.zeros(E + 1, dtype=np.int32)
+    big = np.zeros(4 * max(E, 1), dtype=np.int32)
+    net_slot = np.zeros(max(E, 1), dtype=np.int32)
+    ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    N.call("dp_wa_pack_nets", vp(start), E, vp(group_start), ctypes.byref(ng), vp(big),
+           ctypes.byref(nb), vp(net_slot))
+    return group_start, ng.value, big, nb.value, net_slot
+
+
+def movable_netlist_order(dd):
+    """An order of the movable cells for DeviceDesign.reorder_movables: by
+    the first slot, in the pin-parallel wirelength pass's group order
+    (dp_wa_pack_nets), among each cell's pins; cells without pins last, in
+    their order.  The pass's two permutations (node order <-> group order)
+    are scattered 8-byte stores, which L2 combines when neighbouring stores
+    hit neighbouring addresses; in this order each cell's first pin lands
+    next to its neighbours' (config 1: wirelength forward 383 -> 345 us)."""
+    dev = dd.device
+    st = dd.flat_net2pin_start_map.long()
+    E, P = st.numel() - 1, dd.num_pins
+    net_slot = pack_nets(st.cpu().numpy())[4]
+    lens = st[1:] - st[:-1]
+    net_of_flat = torch.repeat_interleave(torch.arange(E, device=dev), lens)
+    slot = (torch.from_numpy(net_slot[:E].astype(np.int64)).to(dev)[net_of_flat]
+            + torch.arange(P, device=dev) - st[net_of_flat])
+    node = dd.pin2node_map.long()[dd.flat_net2pin_map.long()]
+    first = torch.full((dd.num_nodes,), P, dtype=torch.int64, device=dev)
+    first.scatter_reduce_(0, node, slot, "amin")
+    return torch.argsort(first[:dd.num_movable_nodes], stable=True)
 
 
 class DeviceDesign:
@@ -56,7 +95,7 @@ class DeviceDesign:
                 x = x.to(torch.bool)
             elif x.is_floating_point():
                 x = x.to(dt)
-            self.t[k] = x.to(self.device)
+            self.t[k] = x.to(self.device, copy=True)  # never aliases the caller's arrays
 
     def uniform_fillers(self):
         """Whether every filler has the same size (PlaceDB.py:917-936 makes
@@ -94,24 +133,70 @@ class DeviceDesign:
         self.filler_perm = perm.clone() if fp is None else fp[perm]
         return perm
 
+    def reorder_movables(self, order):
+        """Relabel the movable cells: cell i becomes the current cell
+        order[i] (a permutation of [0, num_movable_nodes)).  Pins keep their
+        ids and each net its pin order; each cell keeps its pins in their CSR
+        order.  Every sum of the objective (per net in pin order, per node in
+        CSR order, the integer density map) then keeps its terms and their
+        order: the objective is the same bits and each cell's gradient moves
+        with it.  For set-up, before objectives or optimizer state exist.
+        The accumulated relabelling is kept in `movable_perm` (slot ->
+        generated cell)."""
+        m, n, dev = self.num_movable_nodes, self.num_nodes, self.device
+        order = torch.as_tensor(order).to(device=dev, dtype=torch.int64)
+        if order.numel() != m or not torch.equal(torch.sort(order).values,
+                                                 torch.arange(m, device=dev)):
+            raise ValueError("reorder_movables: order must be a permutation of the %d movable cells" % m)
+        inv = torch.empty_like(order)
+        inv[order] = torch.arange(m, device=dev)
+        p = self.t["pos"].view(2, n)
+        p[:, :m] = p[:, :m][:, order]
+        for k in ("node_size_x", "node_size_y"):
+            self.t[k][:m] = self.t[k][:m][order]
+        if "movable_macro_mask" in self.t:
+            self.t["movable_macro_mask"] = self.t["movable_macro_mask"][order].contiguous()
+        remap = lambda a: torch.where(a < m, inv[a.clamp(max=m - 1)], a).to(torch.int32)
+        if "sorted_node_map" in self.t:
+            self.t["sorted_node_map"] = remap(self.t["sorted_node_map"].long())
+        self.t["pin2node_map"] = remap(self.t["pin2node_map"].long())
+        st = self.t["flat_node2pin_start_map"].long()
+        nphys = st.numel() - 1
+        old_of_new = torch.cat([order, torch.arange(m, nphys, device=dev)])
+        lens = st[old_of_new + 1] - st[old_of_new]
+        nst = torch.zeros(nphys + 1, dtype=torch.int64, device=dev)
+        nst[1:] = torch.cumsum(lens, 0)
+        idx = (torch.repeat_interleave(st[old_of_new] - nst[:-1], lens)
+               + torch.arange(int(nst[-1].item()), device=dev))
+        self.t["flat_node2pin_map"] = self.t["flat_node2pin_map"][idx].contiguous()
+        self.t["flat_node2pin_start_map"] = nst.to(torch.int32)
+        mp = self.__dict__.get("movable_perm")
+        self.movable_perm = order.clone() if mp is None else mp[order]
+
     def current_order(self, v):
         """A copy of a [2N] per-node vector given in the generation order,
-        with the fillers in the current order."""
+        with the movable cells and the fillers in the current order."""
         out = v.clone()
+        mp = self.__dict__.get("movable_perm")
+        if mp is not None:
+            n, m = self.num_nodes, self.num_movable_nodes
+            out.view(2, n)[:, :m] = v.view(2, n)[:, :m][:, mp]
         fp = self.__dict__.get("filler_perm")
         if fp is not None:
             permute_fillers_(out, self.num_nodes, self.num_filler_nodes, fp)
         return out
 
     def generation_order(self, v):
-        """A copy of a [2N] per-node vector with the fillers back in the
-        order the design was generated in."""
+        """A copy of a [2N] per-node vector with the movable cells and the
+        fillers back in the order the design was generated in."""
         fp = self.__dict__.get("filler_perm")
-        if fp is None:
-            return v.clone()
-        n, nf = self.num_nodes, self.num_filler_nodes
+        mp = self.__dict__.get("movable_perm")
+        n, nf, m = self.num_nodes, self.num_filler_nodes, self.num_movable_nodes
         out = v.clone()
-        out.view(2, n)[:, n - nf:][:, fp] = v.view(2, n)[:, n - nf:]
+        if fp is not None:
+            out.view(2, n)[:, n - nf:][:, fp] = v.view(2, n)[:, n - nf:]
+        if mp is not None:
+            out.view(2, n)[:, :m][:, mp] = v.view(2, n)[:, :m]
         return out
 
     def __getattr__(self, k):
@@ -193,14 +278,8 @@ class PinPlan:
         check_netlist(dd.flat_net2pin_map.cpu().numpy(), start, dd.pin2net_map.cpu().numpy(),
                       dd.flat_node2pin_map.cpu().numpy(), dd.flat_node2pin_start_map.cpu().numpy(),
                       P, dd.num_physical_nodes)
-        group_start = np.zeros(E + 1, dtype=np.int32)
-        big = np.zeros(4 * max(E, 1), dtype=np.int32)
-        net_slot = np.zeros(max(E, 1), dtype=np.int32)
-        ng, nb = ctypes.c_int64(0), ctypes.c_int64(0)
-        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-        N.call("dp_wa_pack_nets", vp(start), E, vp(group_start), ctypes.byref(ng), vp(big),
-               ctypes.byref(nb), vp(net_slot))
-        self.num_groups, self.num_big, self.num_nets = ng.value, nb.value, E
+        group_start, ngv, big, nbv, net_slot = pack_nets(start)
+        self.num_groups, self.num_big, self.num_nets = ngv, nbv, E
         dev = dd.device
         self.group_start = torch.from_numpy(group_start[:self.num_groups + 1]).to(dev)
         self.big = torch.from_numpy(big[:4 * max(self.num_big, 1)]).to(dev)

@@ -24,13 +24,13 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .engine import DeviceDesign, PlacementObjective, permute_fillers_
+from .engine import DeviceDesign, PlacementObjective, movable_netlist_order, permute_fillers_
 from .nesterov import NesterovAcceleratedGradientOptimizer
 
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
-                density_factor=1.0, early_stop=True, filler_sort_interval=25,
+                density_factor=1.0, early_stop=True, filler_sort_interval=25, movable_order="netlist",
                 fuse_move_boundary=True, fused_metrics=True)
 
 
@@ -189,6 +189,9 @@ class GlobalPlacer:
         self.d = d
         self.dd = DeviceDesign(d, device, dtype=dtype)
         dd = self.dd
+        if p["movable_order"] == "netlist":
+            # same objective bits; positions() returns the generated order
+            dd.reorder_movables(movable_netlist_order(dd))
         self.base_gamma = p["gamma"] * (d["bin_size_x"] + d["bin_size_y"])
         self.gamma = 10 * self.base_gamma
         track = False

@@ -278,3 +278,35 @@ def test_filler_ranges_are_exact(gpu, dtype):
     rng.filler_ranges[1] = bad[-1] + 5  # not monotone, past the end
     same()
     rng.filler_ranges.copy_(bad)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_engine_movable_relabelling(gpu, dtype):
+    """Movable cells relabelled in the wirelength pass's netlist order
+    (reorder_movables) give the same objective bits, and each cell's
+    gradient moves with it; generation_order / current_order undo and redo
+    the relabelling (with fillers reordered too)."""
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective, movable_netlist_order
+    print("case movable relabelling %s" % np.dtype(dtype).name, flush=True)
+    d = design(0, dtype=dtype, big_nets=20)
+    res = []
+    for relabel in (False, True):
+        dd = DeviceDesign(d, gpu)
+        if relabel:
+            order = movable_netlist_order(dd)
+            assert not torch.equal(order.cpu(), torch.arange(d.num_movable_nodes))
+            dd.reorder_movables(order)
+        ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="fp64",
+                                algorithm="tiled", schedule="pins")
+        perm = dd.reorder_fillers(origin_offset=ob.filler_origin_offset())
+        ob.update_filler_ranges(dd.pos)
+        f, g = ob.evaluate(dd.pos)
+        res.append((f.item(), dd.generation_order(g), dd.generation_order(dd.pos), dd))
+    assert res[0][0] == res[1][0]
+    assert torch.equal(res[0][1], res[1][1])
+    assert torch.equal(res[0][2], res[1][2])
+    assert torch.equal(res[0][2].cpu(), torch.from_numpy(d.pos).to(res[0][2].dtype))
+    dd = res[1][3]
+    assert torch.equal(dd.current_order(res[1][2]), dd.pos)
+    with pytest.raises(ValueError):
+        dd.reorder_movables(torch.zeros(d.num_movable_nodes, dtype=torch.int64))
