@@ -91,6 +91,20 @@ PROBES = {"wa_forward": DP_OP_WA_FORWARD, "density_map": DP_OP_DENSITY_MAP,
           "node_grad": DP_OP_NODE_GRAD}
 
 
+def relabel(dd, movable_order, filler_origin=None):
+    """The timed form's node order: movable cells in the wirelength pass's
+    netlist order ("netlist") and, when `filler_origin` (the density kernels'
+    filler origin offset) is given, fillers in tile order.  Neither changes
+    the objective's bits.  Every other form of the same placement (FP64, the
+    CPU baseline) is made from dd.host_design(...) afterwards, so that all of
+    them hold the same positions, sizes and netlist in the same order."""
+    from dreamplace_amd.engine import movable_netlist_order
+    if movable_order == "netlist":
+        dd.reorder_movables(movable_netlist_order(dd))
+    if filler_origin is not None:
+        dd.reorder_fillers(origin_offset=filler_origin)
+
+
 def time_steps(torch, obj, pos, steps, warmup):
     for _ in range(warmup):
         obj.evaluate(pos)
@@ -508,7 +522,7 @@ def main():
 
     from dreamplace_amd import _native as N
     from dreamplace_amd import synth
-    from dreamplace_amd.engine import DeviceDesign, PlacementObjective, movable_netlist_order
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
     dev = torch.device("cuda:0")
     N.lib()
 
@@ -518,19 +532,17 @@ def main():
     gamma = d.wa_gamma
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
-    if args.movable_order == "netlist":
-        dd32.reorder_movables(movable_netlist_order(dd32))
+    relabel(dd32, args.movable_order)
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
                                algorithm=args.algorithm, schedule=args.schedule,
                                energy_mode=args.energy_mode)
     if args.filler_order == "tile":
         # fillers are interchangeable (one size, no pins): the engine keeps
-        # them in the density kernels' tile order and takes them by tile range;
-        # the FP64 form and the CPU baseline get the same order
-        dd32.reorder_fillers(origin_offset=obj32.filler_origin_offset())
+        # them in the density kernels' tile order and takes them by tile range
+        relabel(dd32, None, obj32.filler_origin_offset())
         obj32.update_filler_ranges(dd32.pos)
-        d = synth.Design(d)
-        d.pos = dd32.pos.cpu().numpy()
+    # the FP64 form and the CPU baseline: the same arrays in the same order
+    d = dd32.host_design(d)
     pos32 = dd32.pos
     ms32, per32 = time_steps(torch, obj32, pos32, args.steps, args.warmup)
     log("[bench] fp32/%s: %.3f ms/step (event median %.3f)" % (args.precision, ms32,
@@ -576,9 +588,7 @@ def main():
         result.update(ms_per_step_one_stream=round(ms_serial, 4),
                       ops_timing="each op alone, every kernel on one stream")
     if not args.skip_fp64:
-        dd64 = DeviceDesign(d, dev, dtype=torch.float64)
-        if args.movable_order == "netlist":
-            dd64.reorder_movables(movable_netlist_order(dd64))
+        dd64 = DeviceDesign(d, dev, dtype=torch.float64)  # already in dd32's order
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
                                    algorithm=args.algorithm, schedule=args.schedule,
                                    energy_mode=args.energy_mode)

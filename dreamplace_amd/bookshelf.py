@@ -11,8 +11,9 @@ output.
 * `write_pl(path, db, x, y)` writes PlaceDB.write_pl's format
   (dreamplace/PlaceDB.py:1039-1074).
 
-Orientation-dependent pin flips of the reference (PyPlaceDB.cpp:680-735)
-are not applied: every node is read as placed, offsets as given.
+Fixed nodes and IO pins placed in an orientation other than N have their
+sizes and pin offsets restated for N by the reader, as PyPlaceDB does
+(PyPlaceDB.cpp:669-740); `node_orient` keeps the orientation read.
 """
 import ctypes
 import math

@@ -16,7 +16,9 @@
 //   * nets ordered by degree, ties by file order, then pins by net, ties by
 //     creation order (PlaceDB.cpp:1854-1990);
 //   * rows sorted bottom-up; die = bounding box of the rows
-//     (PlaceDB.cpp:819-820, PyPlaceDB.cpp:434-440).
+//     (PlaceDB.cpp:819-820, PyPlaceDB.cpp:434-440);
+//   * fixed nodes and IO pins in another orientation than N: size and pin
+//     offsets restated for N (PyPlaceDB.cpp:669-740).
 // The Bookshelf grammar itself follows the published format (the reference
 // parses it with the Limbo library, a submodule that is not vendored).
 #include <algorithm>
@@ -370,6 +372,47 @@ bool parse_pl(Design& d, const std::string& text) {
 }
 
 // -- orders -----------------------------------------------------------------
+// Orientation as (clockwise degrees, mirrored about the y axis); UNKNOWN
+// counts as N (PyPlaceDB.cpp:545-594).
+void orient_degree_flip(int orient, int* degree, int* flip) {
+  static const int deg[8] = {0, 180, 90, 270, 0, 180, 90, 270};
+  *degree = orient >= 0 && orient < 8 ? deg[orient] : 0;
+  *flip = orient >= DP_ORIENT_FN && orient <= DP_ORIENT_FE ? 1 : 0;
+}
+
+// Fixed nodes and IO pins placed in an orientation other than N get their
+// size and their pins' offsets restated for N (PyPlaceDB::convertOrient,
+// PyPlaceDB.cpp:669-740, with the rotations of :596-667): rotate by
+// (360 - degree) % 360, then mirror about the y axis if the orientation is
+// flipped.  The recorded orientation is kept.  Movable nodes are placed in
+// N anyway and are left alone.
+void convert_orient(Design& d) {
+  for (auto& nd : d.nodes) {
+    if (nd.status != FIXED || nd.orient == DP_ORIENT_N || nd.orient == DP_ORIENT_UNKNOWN) continue;
+    int degree, flip;
+    orient_degree_flip(nd.orient, &degree, &flip);
+    const int rot = (360 - degree) % 360;
+    const double w = (double)nd.w, h = (double)nd.h;
+    for (int64_t p : nd.pins) {
+      BPin& pin = d.pins[p];
+      const double x = pin.ox, y = pin.oy;
+      if (rot == 180) {
+        pin.ox = w - x;
+        pin.oy = h - y;
+      } else if (rot == 270) {
+        pin.ox = h - y;
+        pin.oy = x;
+      } else if (rot == 90) {
+        pin.ox = y;
+        pin.oy = w - x;
+      }
+    }
+    if (rot == 90 || rot == 270) std::swap(nd.w, nd.h);
+    if (flip)
+      for (int64_t p : nd.pins) d.pins[p].ox = (double)nd.w - d.pins[p].ox;
+  }
+}
+
 void sort_nodes(Design& d) {
   const int64_t n = (int64_t)d.nodes.size();
   auto key = [&](int64_t i) {
@@ -496,6 +539,7 @@ int dp_bookshelf_read(const char* aux_path, int sort_nets_by_degree, void** hand
   }
   sort_nodes(*d);
   if (sort_nets_by_degree) sort_nets(*d);
+  convert_orient(*d);
   *handle = d;
   return DP_OK;
 }

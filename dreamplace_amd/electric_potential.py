@@ -86,7 +86,7 @@ class ElectricPotential(ElectricOverflow):
 
     def reset(self):
         super(ElectricPotential, self).reset()
-        logger.info("regard %d cells as movable macros in global placement" % self.num_movable_macros)
+        logger.info("%d movable cells are treated as macros by the density model" % self.num_movable_macros)
 
     def plan(self, dtype, device):
         prec = self.precision or default_precision(dtype)
@@ -128,7 +128,7 @@ class ElectricPotential(ElectricOverflow):
         return out
 
     def forward(self, pos, mode="density"):
-        assert mode in {"density", "overflow"}, "Only support density mode or overflow mode"
+        assert mode in {"density", "overflow"}, "forward mode must be 'density' or 'overflow'"
         N.require_gpu(pos, name="ElectricPotential")
         if self.initial_density_map is None:
             self.compute_initial_density_map(pos)

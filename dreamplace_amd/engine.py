@@ -144,6 +144,8 @@ class DeviceDesign:
         The accumulated relabelling is kept in `movable_perm` (slot ->
         generated cell)."""
         m, n, dev = self.num_movable_nodes, self.num_nodes, self.device
+        if m == 0:
+            return  # nothing to relabel
         order = torch.as_tensor(order).to(device=dev, dtype=torch.int64)
         if order.numel() != m or not torch.equal(torch.sort(order).values,
                                                  torch.arange(m, device=dev)):
@@ -197,6 +199,18 @@ class DeviceDesign:
             out.view(2, n)[:, n - nf:][:, fp] = v.view(2, n)[:, n - nf:]
         if mp is not None:
             out.view(2, n)[:, :m][:, mp] = v.view(2, n)[:, :m]
+        return out
+
+    def host_design(self, like):
+        """A host design dict (the layout of `like`, the dict this design was
+        made from) holding this design's arrays in their current order: the
+        relabelled positions, sizes and netlist maps together, each array in
+        the dtype it has in `like`.  What another form of the same placement
+        (another precision, a CPU run) is built from."""
+        out = type(like)(like)
+        for k, v in like.items():
+            if isinstance(v, np.ndarray):
+                out[k] = self.t[k].cpu().numpy().astype(v.dtype, copy=False)
         return out
 
     def __getattr__(self, k):
@@ -262,6 +276,19 @@ def check_netlist(flat_netpin, netpin_start, pin2net, flat_node2pin, node2pin_st
     owner = np.repeat(np.arange(len(deg)), deg)
     if not np.array_equal(np.asarray(pin2net)[np.asarray(flat_netpin)], owner):
         raise ValueError("pin2net_map disagrees with flat_net2pin_map / flat_net2pin_start_map")
+
+
+# tiles of 64x64 bins the tiled density kernels take (their LDS histograms)
+MAX_TILES = 2048
+
+
+def can_track_metrics(schedule, algorithm, num_bins_x, num_bins_y, padding, uniform_fillers):
+    """Whether PlacementObjective(track_metrics=True) applies: the pins
+    schedule, the tiled density algorithm within its tile limit, no padding
+    and fillers of one size (or none)."""
+    tiles = ((int(num_bins_x) + 63) // 64) * ((int(num_bins_y) + 63) // 64)
+    return (schedule == "pins" and algorithm == "tiled" and tiles <= MAX_TILES
+            and int(padding) == 0 and bool(uniform_fillers))
 
 
 class PinPlan:
@@ -396,11 +423,11 @@ class PlacementObjective:
         self.metric_ptr = None
         if track_metrics:
             ep = self.density
-            tiles = ((M + 63) // 64) * ((Nb + 63) // 64)
-            if not (self.schedule == "pins" and ep.algorithm == "tiled" and tiles <= 2048
-                    and ep.padding == 0 and (dd.num_filler_nodes == 0 or ep.use_uniform_filler)):
+            if not can_track_metrics(self.schedule, ep.algorithm, M, Nb, ep.padding,
+                                     dd.num_filler_nodes == 0 or ep.use_uniform_filler):
                 raise ValueError("track_metrics needs the pins schedule, the tiled density "
-                                 "algorithm (at most 2048 tiles), no padding and fillers of one size")
+                                 "algorithm (at most %d tiles), no padding and fillers of one size"
+                                 % MAX_TILES)
             self.track_metrics = True
             self.net_hpwl = z(E)
             self.movable_map = z(M, Nb)

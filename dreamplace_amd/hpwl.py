@@ -28,7 +28,7 @@ class HPWL(nn.Module):
     def __init__(self, flat_netpin=None, netpin_start=None, pin2net_map=None, net_weights=None,
                  net_mask=None, algorithm='atomic'):
         super(HPWL, self).__init__()
-        assert net_mask is not None, "net_mask is a requried parameter"
+        assert net_mask is not None, "HPWL needs a net mask (net_mask=None given)"
         assert flat_netpin is not None and netpin_start is not None, \
             "flat_netpin, netpin_start are required by the net-parallel kernel"
         self.flat_netpin = N.as_i32(flat_netpin)

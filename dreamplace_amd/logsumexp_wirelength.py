@@ -81,11 +81,11 @@ class LogSumExpWirelength(nn.Module):
                  net_mask=None, pin_mask=None, gamma=None, algorithm='merged'):
         super(LogSumExpWirelength, self).__init__()
         if net_weights is None or net_mask is None or pin_mask is None or gamma is None:
-            raise AssertionError("net_weights, net_mask, pin_mask, gamma are requried parameters")
+            raise AssertionError("log-sum-exp wirelength: net_weights, net_mask, pin_mask and gamma must all be given")
         if algorithm not in ("merged", "atomic"):
-            raise AssertionError("algorithm must be merged | atomic, got %s" % algorithm)
+            raise AssertionError("unknown wirelength algorithm %r (expected 'merged' or 'atomic')" % algorithm)
         if flat_netpin is None or netpin_start is None or pin2net_map is None:
-            raise AssertionError("flat_netpin, netpin_start, pin2net_map are requried parameters")
+            raise AssertionError("log-sum-exp wirelength: the net CSR (flat_netpin, netpin_start) and pin2net_map must all be given")
         self.flat_netpin = flat_netpin
         self.netpin_start = netpin_start
         self.pin2net_map = pin2net_map

@@ -43,7 +43,7 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         storage of g_k_1 (which the line search no longer reads) and keeps
         it as g_k, with no copy of the gradient."""
         if lr is not required and lr < 0.0:
-            raise ValueError("Invalid learning rate: {}".format(lr))
+            raise ValueError("the learning rate must be non-negative, got {}".format(lr))
         defaults = dict(lr=lr, u_k=[], v_k=[], g_k=[], obj_k=[], a_k=[], alpha_k=[], v_k_1=[],
                         g_k_1=[], obj_k_1=[], v_kp1=[None], obj_eval_count=0)
         super(NesterovAcceleratedGradientOptimizer, self).__init__(params, defaults)
@@ -53,7 +53,7 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         self.bounds = bounds
         self.grad_into = grad_into
         if len(self.param_groups) != 1 or len(self.param_groups[0]["params"]) != 1:
-            raise ValueError("Only parameters with single tensor is supported")
+            raise ValueError("the optimizer takes exactly one parameter tensor (the [2N] positions)")
         self._norms = None
         self.backtracks = []
 

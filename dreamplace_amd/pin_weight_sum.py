@@ -12,7 +12,7 @@ class PinWeightSum(nn.Module):
                  algorithm='node-by-node'):
         super(PinWeightSum, self).__init__()
         assert flat_nodepin is not None and nodepin_start is not None, \
-            "flat_nodepin, nodepin_start are requried parameters for algorithm node-by-node"
+            "the node-by-node sum needs the node-to-pin CSR (flat_nodepin and nodepin_start)"
         self.flat_nodepin = N.as_i32(flat_nodepin).contiguous()
         self.nodepin_start = N.as_i32(nodepin_start).contiguous()
         self.pin2net_map = N.as_i32(pin2net_map).contiguous()

@@ -24,7 +24,8 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .engine import DeviceDesign, PlacementObjective, movable_netlist_order, permute_fillers_
+from .engine import (DeviceDesign, PlacementObjective, can_track_metrics, movable_netlist_order,
+                     permute_fillers_)
 from .nesterov import NesterovAcceleratedGradientOptimizer
 
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
@@ -196,9 +197,12 @@ class GlobalPlacer:
         self.gamma = 10 * self.base_gamma
         track = False
         if p["fused_metrics"]:
+            # where the objective cannot track the metrics (another schedule
+            # or algorithm, too many tiles), the separate passes run instead
             kw = dict(objective_kw)
-            track = (kw.get("schedule") == "pins" and kw.get("algorithm") == "tiled"
-                     and (dd.num_filler_nodes == 0 or dd.uniform_fillers()))
+            track = can_track_metrics(kw.get("schedule"), kw.get("algorithm"), d["num_bins_x"],
+                                      d["num_bins_y"], 0,
+                                      dd.num_filler_nodes == 0 or dd.uniform_fillers())
         self.objective = PlacementObjective(dd, self.gamma, density_weight=0.0, track_metrics=track,
                                             **objective_kw)
         self.lam = 0.0
@@ -380,8 +384,10 @@ class GlobalPlacer:
 
     # -- the loop -------------------------------------------------------------
     def run(self, iterations, log_every=0):
-        """At most `iterations` steps; returns (positions, metrics before each
-        step).  `log_every` > 0 prints every that many iterations' metrics."""
+        """At most `iterations` steps; returns (positions in the design's
+        node order, as positions() gives them; metrics before each step).
+        The loop's own vector, in its relabelled order, stays `self.pos`.
+        `log_every` > 0 prints every that many iterations' metrics."""
         p, x = self.p, self.pos
         interval, sorting = int(p["filler_sort_interval"]), self.sorting
         lr = self.estimate_learning_rate(x)
@@ -460,7 +466,7 @@ class GlobalPlacer:
                 break
             if stop:
                 break
-        return x, hist
+        return self.positions(), hist
 
 
 def main(argv=None):
@@ -509,9 +515,10 @@ def main(argv=None):
                       energy_mode="spectral", seed=a.seed)
     torch.cuda.synchronize()
     t1 = time.time()
-    x, hist = gp.run(a.iterations, log_every=a.log_every)
+    _, hist = gp.run(a.iterations, log_every=a.log_every)
     torch.cuda.synchronize()
     t_run = time.time() - t1
+    x = gp.pos
     gp.constraint(x)
     fin = gp.metrics(x)
     out = dict(config=a.config, scale=a.scale, precision=a.precision, dtype=str(np.dtype(dt)),

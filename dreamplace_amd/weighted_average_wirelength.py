@@ -109,9 +109,10 @@ class WeightedAverageWirelength(nn.Module):
                  store_limit_bytes=None):
         super(WeightedAverageWirelength, self).__init__()
         assert net_weights is not None and net_mask is not None and pin_mask is not None \
-            and gamma is not None, "net_weights, net_mask, pin_mask, gamma are requried parameters"
+            and gamma is not None, \
+            "weighted-average wirelength: net_weights, net_mask, pin_mask and gamma must all be given"
         assert flat_netpin is not None and netpin_start is not None and pin2net_map is not None, \
-            "flat_netpin, netpin_start, pin2net_map are required by the merged algorithm"
+            "the merged weighted-average kernels need the net CSR and pin2net_map"
         if algorithm not in ("net-by-net", "atomic", "merged"):
             raise ValueError("unknown algorithm %s" % algorithm)
         self.flat_netpin = flat_netpin

@@ -155,3 +155,60 @@ def test_missing_file_raises(tmp_path):
     (tmp_path / "x.aux").write_text("RowBasedPlacement : x.nodes x.nets x.pl x.scl\n")
     with pytest.raises(RuntimeError, match="cannot read"):
         B.read(str(tmp_path / "x.aux"))
+
+
+ORIENT_CASES = {
+    # orientation: (size in N, lower-left offset of the pin in N), derived by
+    # hand from PyPlaceDB.cpp:545-740 for a 6x4 node whose pin sits at (+2, -1)
+    # from its centre, i.e. (5, 1) from its lower-left corner in the file
+    "N": ((6, 4), (5, 1)),
+    "S": ((6, 4), (1, 3)),    # rotate 180: (w - x, h - y)
+    "W": ((4, 6), (3, 5)),    # rotate 270: (h - y, x), sizes swapped
+    "E": ((4, 6), (1, 1)),    # rotate 90: (y, w - x), sizes swapped
+    "FN": ((6, 4), (1, 1)),   # mirror: (w - x, y)
+    "FS": ((6, 4), (5, 3)),   # rotate 180, then mirror
+    "FW": ((4, 6), (1, 5)),   # rotate 270, then mirror with the new width 4
+    "FE": ((4, 6), (3, 1)),   # rotate 90, then mirror with the new width 4
+}
+
+
+def _orient_design(tmp_path):
+    names = list(ORIENT_CASES)
+    nodes = ["UCLA nodes 1.0", "NumNodes : %d" % (len(names) + 1),
+             "NumTerminals : %d" % len(names), "c0 2 4"]
+    nodes += ["f_%s 6 4 terminal" % o for o in names]
+    nets = ["UCLA nets 1.0", "NumNets : %d" % len(names), "NumPins : %d" % (2 * len(names))]
+    for k, o in enumerate(names):
+        nets += ["NetDegree : 2 n%d" % k, "c0 I : 0 0", "f_%s O : 2 -1" % o]
+    pl = ["UCLA pl 1.0", "c0 40 40 : FS"]
+    pl += ["f_%s %d %d : %s /FIXED" % (o, 10 * k, 80, o) for k, o in enumerate(names)]
+    scl = ["UCLA scl 1.0", "NumRows : 25"]
+    for r in range(25):
+        scl += ["CoreRow Horizontal", " Coordinate : %d" % (4 * r), " Height : 4",
+                " Sitewidth : 1", " Sitespacing : 1", " Siteorient : 1", " Sitesymmetry : 1",
+                " SubrowOrigin : 0 NumSites : 100", "End"]
+    for ext, lines in (("nodes", nodes), ("nets", nets), ("pl", pl), ("scl", scl)):
+        (tmp_path / ("o.%s" % ext)).write_text("\n".join(lines) + "\n")
+    (tmp_path / "o.aux").write_text("RowBasedPlacement : o.nodes o.nets o.pl o.scl\n")
+    return B.read(str(tmp_path / "o.aux"))
+
+
+def test_fixed_node_orientations(tmp_path):
+    """Fixed nodes placed in each of the eight orientations: sizes and pin
+    offsets restated for N (PyPlaceDB::convertOrient); the recorded
+    orientation is kept; a movable node's orientation changes nothing; every
+    pin stays inside its (rotated) node."""
+    db = _orient_design(tmp_path)
+    for o, (size, off) in ORIENT_CASES.items():
+        i = db.node_name2id_map["f_" + o]
+        assert db.node_orient[i] == o
+        assert (db.node_size_x[i], db.node_size_y[i]) == size, o
+        (p,) = db.node2pin_map[i]
+        assert (db.pin_offset_x[p], db.pin_offset_y[p]) == off, o
+        assert 0 <= off[0] <= size[0] and 0 <= off[1] <= size[1]
+    c = db.node_name2id_map["c0"]
+    assert db.node_orient[c] == "FS"
+    assert (db.node_size_x[c], db.node_size_y[c]) == (2, 4)
+    assert all((db.pin_offset_x[p], db.pin_offset_y[p]) == (1, 2) for p in db.node2pin_map[c])
+    # the space area uses the rotated sizes: 4 nodes of 6x4 and 4 of 4x6, apart
+    assert db.total_space_area == pytest.approx(100 * 100 - 8 * 24)

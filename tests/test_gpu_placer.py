@@ -296,3 +296,42 @@ def test_fused_metrics_keep_the_loop(gpu, dtype):
     assert torch.equal(runs[0][0], runs[1][0])
     for h0, h1 in zip(runs[0][1], runs[1][1]):
         assert h0 == h1
+
+
+def test_bookshelf_design_through_the_loop(gpu, tmp_path):
+    """The reference's `simple` Bookshelf fixture, read and converted
+    (bookshelf.to_design), placed for 6 iterations (FP64): the same metrics
+    and positions as the restated loop.  run() returns the positions in the
+    design's node order, so the .pl written from them puts every movable
+    node where the restated loop puts it."""
+    import os
+
+    from dreamplace_amd import bookshelf as B
+    from dreamplace_amd.placer import GlobalPlacer
+    from oracle import placer as OP
+    print("case bookshelf simple through the loop", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    db = B.read(os.path.join(here, "golden", "bookshelf_simple", "simple.aux"))
+    d = B.to_design(db, target_density=1.0)
+    gp = GlobalPlacer(d, gpu, dtype=torch.float64, precision="fp64", algorithm="tiled",
+                      schedule="pins")
+    pos, hist = gp.run(6)
+    xr, hr, _ = OP.place(d, 6)
+    assert len(hist) == len(hr)
+    for h, r in zip(hist, hr):
+        for k in ("hpwl", "overflow", "density_weight", "gamma"):
+            assert h[k] == pytest.approx(r[k], rel=1e-9), k
+    assert torch.equal(pos, gp.dd.generation_order(gp.pos))
+    assert rel_l2(pos.cpu().numpy(), xr) <= 1e-9
+    x, y = B.positions_in_file_units(d, pos.cpu().numpy())
+    out = tmp_path / "simple.gp.pl"
+    B.write_pl(str(out), db, x, y)
+    rows = {}
+    for line in out.read_text().splitlines()[1:]:
+        t = line.split()
+        if t:
+            rows[t[0]] = (float(t[1]), float(t[2]))
+    xe, ye = B.positions_in_file_units(d, xr)
+    for i in range(db.num_movable_nodes):
+        got = rows[db.node_names[i]]
+        assert got[0] == pytest.approx(xe[i], rel=1e-5) and got[1] == pytest.approx(ye[i], rel=1e-5)

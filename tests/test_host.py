@@ -251,3 +251,63 @@ def test_filler_tile_order_on_host():
     dd.reorder_fillers(torch.randperm(nf, generator=torch.Generator().manual_seed(3)))
     assert torch.equal(dd.generation_order(dd.pos), pos0)
     assert torch.equal(dd.current_order(pos0), dd.pos)
+
+
+def test_bench_forms_share_one_placement(oracle):
+    """bench.py's set-up: the timed FP32 design is relabelled (movable cells
+    in netlist order, fillers in tile order) and the FP64 form and the CPU
+    baseline are built from its host_design.  All three must be one
+    placement: the same positions, sizes and netlist in the same order, so
+    the objective is the same number and each cell's gradient moves with it.
+    (Relabelling the positions alone, or relabelling twice, fails here.)"""
+    import bench
+    from dreamplace_amd.engine import DeviceDesign
+    d = synth.make_design(0, dtype=np.float32)
+    dd = DeviceDesign(d, "cpu")
+    bench.relabel(dd, "netlist")
+    bench.relabel(dd, None, (0.3, -0.2))
+    h = dd.host_design(d)
+    dd64 = DeviceDesign(h, "cpu", dtype=torch.float64)
+    assert torch.equal(dd64.pos, dd.pos.double())
+    for k in ("node_size_x", "node_size_y", "pin_offset_x", "pin_offset_y"):
+        assert torch.equal(dd64.t[k], dd.t[k].double()), k
+    for k in ("pin2node_map", "flat_node2pin_map", "flat_node2pin_start_map", "flat_net2pin_map",
+              "pin2net_map", "sorted_node_map"):
+        assert torch.equal(dd64.t[k], dd.t[k]), k
+        assert h[k].dtype == d[k].dtype, k
+    # the relabelling is real: positions, pin owners and node CSR all moved
+    assert not np.array_equal(h["pos"], d["pos"])
+    assert not np.array_equal(h["pin2node_map"], d["pin2node_map"])
+    f0, g0, p0 = oracle.objective_and_gradient(d, 0.5, d.wa_gamma)
+    f1, g1, p1 = oracle.objective_and_gradient(h, 0.5, d.wa_gamma)
+    assert f1 == f0
+    assert np.array_equal(p1["density_map"], p0["density_map"])
+    assert np.array_equal(g1, dd.current_order(torch.from_numpy(g0)).numpy())
+    assert np.array_equal(dd.generation_order(torch.from_numpy(g1)).numpy(), g0)
+
+
+def test_reorder_movables_without_movable_cells():
+    """A design with no movable cells: relabelling them is a no-op."""
+    from dreamplace_amd.engine import DeviceDesign
+    d = synth.make_design(0)
+    dd = DeviceDesign(d, "cpu")
+    before = {k: v.clone() for k, v in dd.t.items()}
+    dd.scalars["num_movable_nodes"] = 0
+    dd.reorder_movables(torch.empty(0, dtype=torch.int64))
+    for k, v in before.items():
+        assert torch.equal(dd.t[k], v), k
+    assert dd.__dict__.get("movable_perm") is None
+
+
+def test_metric_tracking_gate():
+    """The placer tracks its metrics inside the objective only where the
+    objective can (pins schedule, tiled density within its tile limit, no
+    padding, fillers of one size); otherwise the separate passes run."""
+    from dreamplace_amd.engine import MAX_TILES, can_track_metrics
+    assert can_track_metrics("pins", "tiled", 2048, 2048, 0, True)  # 32 x 32 tiles
+    assert can_track_metrics("pins", "tiled", 64 * MAX_TILES, 64, 0, True)  # the limit itself
+    assert not can_track_metrics("pins", "tiled", 64 * MAX_TILES + 1, 64, 0, True)  # one more
+    assert not can_track_metrics("pins", "global", 1024, 1024, 0, True)
+    assert not can_track_metrics("fused", "tiled", 1024, 1024, 0, True)
+    assert not can_track_metrics("pins", "tiled", 1024, 1024, 2, True)
+    assert not can_track_metrics("pins", "tiled", 1024, 1024, 0, False)

@@ -324,6 +324,68 @@ def test_fixed_point_scale(oracle):
     assert oracle.fixed_scale(0, 0, 2.0 ** 20, 2.0 ** 20) == 2 ** 23
 
 
+def test_gpu_conversion_known_answers(oracle):
+    """The CUDA deterministic path's conversion (atomic_ops.cuh:53-59 as
+    electric_density_map_cuda_kernel.cu:447-461 uses it): the initial map is
+    quantised with u64(init * T(scale)) (truncation), the integer scatter sum
+    is added, and the map is T(u64) * T(1/scale).  Expected values are
+    derived by hand (scale 2^32) and chosen so that rounding instead of
+    truncating, adding init after the conversion, or the CPU path's double
+    sum each give another answer (noted per entry)."""
+    s32 = 2 ** 32
+    init = np.array([3 * 2.0 ** -33,   # init*2^32 = 1.5 -> 1 (round: 2; init after / cpu: 3*2^-33)
+                     0.25,             # 2^30 + 3 -> T rounds to 2^30 (ulp 128)
+                     2.0 ** -9,        # 2^23 + (2^23 + 3) = 2^24 + 3 -> tie to even: 2^24 + 4
+                     1 - 2.0 ** -24,   # 2^32 - 256 + 5 -> nearest float is 2^32 - 256
+                     5 * 2.0 ** -35],  # 0.625 -> 0 (round: 1), + 7
+                    dtype=np.float32)
+    sums = np.array([0, 3, 2 ** 23 + 3, 5, 7], dtype=np.int64)
+    expect = np.array([2.0 ** -32, 0.25, 2.0 ** -8 + 2.0 ** -30, 1 - 2.0 ** -24, 7 * 2.0 ** -32],
+                      dtype=np.float32)
+    q = oracle.fixed_quantize_gpu(init, s32)
+    assert q.tolist() == [1, 2 ** 30, 2 ** 23, 2 ** 32 - 256, 0]
+    got = oracle.fixed_to_map_gpu(q + sums, s32, np.float32)
+    assert np.array_equal(got, expect)
+    # the planted mistakes give other maps
+    rounded = np.floor(init.astype(np.float64) * s32 + 0.5).astype(np.int64)
+    assert not np.array_equal(oracle.fixed_to_map_gpu(rounded + sums, s32, np.float32), expect)
+    after = (oracle.fixed_to_map_gpu(sums, s32, np.float32) + init).astype(np.float32)
+    assert not np.array_equal(after, expect)
+    cpu = oracle.fixed_add_cpu(sums, s32, init.copy())
+    assert not np.array_equal(cpu, expect)
+    # FP64: 0.1*2^32 = 429496729.6... -> 429496729 (round: ...730; cpu: 0.1);
+    # 0.75*2^32 + 2^33 + 1 = 11*2^30 + 1, exact in double
+    init64 = np.array([0.1, 0.75])
+    q64 = oracle.fixed_quantize_gpu(init64, s32)
+    assert q64.tolist() == [429496729, 3 * 2 ** 30]
+    got64 = oracle.fixed_to_map_gpu(q64 + np.array([0, 2 ** 33 + 1]), s32, np.float64)
+    assert got64.tolist() == [429496729 / 2.0 ** 32, 2.75 + 2.0 ** -32]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gpu_and_cpu_conversions_agree_within_a_quantum(oracle, dtype):
+    """On a whole config-0 map the two conventions differ only by what
+    quantising the initial map drops (< 2^-32 per bin) plus each side's final
+    rounding to T; and T(u64)*T(1/scale) equals rounding the exact double
+    product, since the scale is a power of two and every sum is below 2^53.
+    With target density 0.9 the FP64 initial map has fractions below 2^-32,
+    so there the two maps differ."""
+    d = design(0, dtype=dtype)
+    d.target_density = 0.9
+    dp = density_params(oracle, d)
+    assert dp.scale & (dp.scale - 1) == 0
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+    cpu = oracle.density_map(dp, d.pos, init, convention="cpu").reshape(-1)
+    gpu = oracle.density_map(dp, d.pos, init, convention="gpu").reshape(-1)
+    ulp = np.spacing(np.maximum(np.abs(cpu), np.abs(gpu)))
+    assert np.all(np.abs(gpu.astype(np.float64) - cpu) <= 1.0 / dp.scale + ulp)
+    if dtype == np.float64:
+        assert not np.array_equal(gpu, cpu)
+    buf = oracle.fixed_quantize_gpu(init.reshape(-1), dp.scale) + oracle.density_fixed_sum(dp, d.pos)
+    assert buf.max() < 2 ** 53
+    assert np.array_equal(gpu, (buf.astype(np.float64) / dp.scale).astype(dtype))
+
+
 def test_dct_brute_force(oracle):
     """The four transforms against their defining double sums (4x8)."""
     M, N = 4, 8
