@@ -213,20 +213,57 @@ def cpu_baseline(d, budget_s):
         return _cpu_baseline(d, budget_s)
 
 
-def _cpu_baseline(d, budget_s):
-    """The reference's own CPU ops (oracle/_ref) on the same design, one
-    warm-up then timed iterations until the budget is used (at least one)."""
+def host_cpu():
+    """The host CPU: model name, logical CPUs, whether SMT is on, and the
+    CPUs this process may run on (its affinity set)."""
+    model, smt = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return dict(model=model, logical_cpus=os.cpu_count(), smt=smt, affinity_cpus=affinity)
+
+
+def baseline_threads(cpu):
+    """Threads for the CPU baseline: the CPU share this process is given
+    (OMP_NUM_THREADS where the host sets it, as it does to apportion a shared
+    machine's cores per GPU), else every CPU of its affinity set."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return min(int(env), cpu["affinity_cpus"])
+    return cpu["affinity_cpus"]
+
+
+def _cpu_baseline(d, budget_s, warmup=3):
+    """The reference's own CPU ops (oracle/_ref) on the same design: `warmup`
+    untimed iterations, then timed ones until the budget is used (at least
+    one, at most 10); the median."""
     import torch
     from oracle import ref_iteration as R
     if not R.available():
         return None
-    threads = min(os.cpu_count() or 1, 16)
+    cpu = host_cpu()
+    threads = baseline_threads(cpu)
     it = R.RefIteration(d, dtype=torch.float32, num_threads=threads, density_weight=1e-3)
     t0 = time.perf_counter()
-    it.iterate()
-    first = time.perf_counter() - t0
+    for _ in range(warmup):
+        it.iterate()
+    first = (time.perf_counter() - t0) / max(warmup, 1)
     times = []
-    t_end = time.perf_counter() + max(budget_s - first, 0)
+    t_end = time.perf_counter() + max(budget_s - first * warmup, 0)
     while True:
         t1 = time.perf_counter()
         it.iterate()
@@ -234,9 +271,11 @@ def _cpu_baseline(d, budget_s):
         if time.perf_counter() > t_end or len(times) >= 10:
             break
     return dict(value=round(statistics.median(times) * 1e3, 3), unit="ms", cores=threads,
-                kind="reference",
+                kind="reference", cpu_model=cpu["model"], logical_cpus=cpu["logical_cpus"],
+                smt=cpu["smt"], affinity_cpus=cpu["affinity_cpus"],
                 sample="full config-1 iteration through oracle/_ref (the reference's CPU ops, FP32, "
-                       "%d threads): 1 warm-up + %d timed, median" % (threads, len(times)))
+                       "%d threads = this process's CPU share): %d warm-up + %d timed, median"
+                       % (threads, warmup, len(times)))
 
 
 def summarize_trace(trace_dir, warmup, steps):
@@ -453,6 +492,103 @@ def wa_terms_study(torch, args):
                 del wl, op, x, w
 
 
+def with_bins(d, nb):
+    """A copy of design `d` on an nb x nb bin grid (bin sizes, centres and
+    the WA gamma of 4 bin widths follow; the cells are the same)."""
+    from dreamplace_amd import synth
+    d2 = synth.Design(d)
+    bsx, bsy = (d.xh - d.xl) / nb, (d.yh - d.yl) / nb
+    d2.update(num_bins_x=nb, num_bins_y=nb, bin_size_x=bsx, bin_size_y=bsy, wa_gamma=4.0 * bsx)
+    dt = d.pos.dtype
+    d2.bin_center_x = synth._bin_centers(d.xl, d.xh, bsx, nb).astype(dt)
+    d2.bin_center_y = synth._bin_centers(d.yl, d.yh, bsy, nb).astype(dt)
+    return d2
+
+
+def precision_study(torch, args):
+    """The precision study on config-1 shapes: the Poisson solve at 1024^2
+    and 2048^2 bins in three forms -- FP64, FP32 on the matrix cores, and
+    BF16x3 (bf16 hi/lo splits, FP32 accumulation) -- all on the same FP32
+    density map, the one that is bit-exact with the reference's
+    deterministic FP32 map (tests/test_gpu_density.py holds that).  Against
+    the FP64 form: relative L2 of the potential, of the field (x and y
+    together), of the energy, and of the per-cell density gradient, which the
+    FP32 field gather forms from each form's field (so the FP32 gather is in
+    the chain; the FP64 row isolates the gather).  The reference gradient is
+    the FP64 gather of the FP64 field at the same positions.  ms per solve:
+    the step's solve (fields + energy, no potential map), median of HIP-event
+    timings.  Prints one JSON line per (grid, form)."""
+    from dreamplace_amd import _native as N
+    from dreamplace_amd import synth
+    from dreamplace_amd.electric_potential import ElectricPotential
+    from dreamplace_amd.spectral import get_plan
+    dev = torch.device("cuda:0")
+    N.lib()
+    t0 = time.perf_counter()
+    base = synth.make_design(1, seed=0, dtype=np.float32, scale=args.scale)
+    log("[study] design built in %.1fs: %s" % (time.perf_counter() - t0, synth.summary(base)))
+    reps = 20
+
+    def module(d, t, dtype, algorithm):
+        return ElectricPotential(
+            t["node_size_x"].to(dtype), t["node_size_y"].to(dtype), t["bin_center_x"].to(dtype),
+            t["bin_center_y"].to(dtype), d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
+            d.bin_size_y, d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+            t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"], algorithm=algorithm)
+
+    for nb in ([1024, 2048] if args.scale == 1.0 else [base.num_bins_x, 2 * base.num_bins_x]):
+        d = with_bins(base, nb)
+        t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()
+             if isinstance(v, np.ndarray)}
+        pos32, pos64 = t["pos"], t["pos"].double()
+        ep32 = module(d, t, torch.float32, "tiled")
+        ep32.compute_initial_density_map(pos32)
+        m32 = ep32.density_map(pos32)  # also bins the cells for the FP32 gather
+        ep64 = module(d, t, torch.float64, "global")
+        one32 = torch.ones((), dtype=torch.float32, device=dev)
+        one64 = torch.ones((), dtype=torch.float64, device=dev)
+        bx, by = d.bin_size_x, d.bin_size_y
+        ref = get_plan(nb, nb, "fp64", dev).poisson(m32.double(), bx, by, True, True)
+        g_ref = ep64.electric_force(pos64, ref[0], ref[1], one64)
+        both = lambda a, b: torch.cat([a.reshape(-1), b.reshape(-1)])
+        for prec in ("fp64", "fp32", "bf16x3"):
+            plan = get_plan(nb, nb, prec, dev)
+            m = m32.double() if prec == "fp64" else m32
+            fx, fy, phi, e = plan.poisson(m, bx, by, True, True)
+            g = ep32.electric_force(pos32, fx.float(), fy.float(), one32)
+            ms = []
+            for _ in range(3):
+                plan.poisson(m, bx, by, False, True)
+            for _ in range(reps):
+                a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a_.record()
+                plan.poisson(m, bx, by, False, True)
+                b_.record()
+                torch.cuda.synchronize()
+                ms.append(a_.elapsed_time(b_))
+            med = statistics.median(ms)
+            fl = poisson_flops(nb, nb, 3)
+            tf = fl / (med * 1e-3) / 1e12
+            rec = dict(study="precision", workload="config1 cells (%d cells, %d fillers) on %dx%d bins"
+                       % (d.num_movable_nodes, d.num_filler_nodes, nb, nb),
+                       form=prec, ms_per_solve=round(med, 4), flops_per_solve=fl,
+                       achieved_tflops=round(tf, 1),
+                       l2err_potential=rel(torch, phi, ref[2]),
+                       l2err_field=rel(torch, both(fx, fy), both(ref[0], ref[1])),
+                       l2err_energy=abs(float(e.double().item()) - float(ref[3].item()))
+                       / abs(float(ref[3].item())),
+                       l2err_density_grad_fp32_gather=rel(torch, g, g_ref))
+            if prec != "fp64":
+                rec["frac_of_fp32_peak"] = round(tf / MFMA_PEAK_TFLOPS["fp32"], 4)
+            if prec == "bf16x3":
+                rec["frac_of_bf16_peak"] = round(3 * tf / MFMA_PEAK_TFLOPS["bf16x3"], 4)
+            if prec == "fp64":
+                rec["frac_of_fp64_peak"] = round(tf / MFMA_PEAK_TFLOPS["fp64"], 4)
+            print(json.dumps(rec), flush=True)
+        del ep32, ep64, t
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,9 +614,11 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--study", choices=["wa-terms"],
+    ap.add_argument("--study", choices=["wa-terms", "precision"],
                     help="wa-terms: the config-3 differentiation study (store vs recompute of the WA "
-                         "exp/max terms, FD checks) instead of the step benchmark")
+                         "exp/max terms, FD checks); precision: the config-2 precision study (FP64, "
+                         "FP32 and BF16x3 solves at 1024^2 and 2048^2 bins); instead of the step "
+                         "benchmark")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print the per-phase kernel summary of a rocprofv3 csv kernel trace of "
                          "`bench.py --steps K --warmup W --skip-cpu` (pass the same K and W) and exit")
@@ -509,6 +647,9 @@ def main():
     import torch
     if args.study == "wa-terms":
         wa_terms_study(torch, args)
+        return
+    if args.study == "precision":
+        precision_study(torch, args)
         return
     if args.calibrate_copy:
         n = (1 << 30) // 4
@@ -587,6 +728,15 @@ def main():
     if ms_serial is not None:
         result.update(ms_per_step_one_stream=round(ms_serial, 4),
                       ops_timing="each op alone, every kernel on one stream")
+    # the same step with the net-weight adjoint: d obj / d w_n = WL_n, the
+    # per-net pre-weight wirelength, which the wirelength pass then also stores
+    if args.schedule in ("pins", "fused"):
+        obj32.set_overlap(True)
+        obj32.keep_partial_wl = True
+        ms_nw, _ = time_steps(torch, obj32, pos32, max(args.steps // 2, 5), 2)
+        obj32.keep_partial_wl = False
+        log("[bench] fp32/%s with the net-weight gradient: %.3f ms/step" % (args.precision, ms_nw))
+        result.update(ms_per_step_with_net_weight_grad=round(ms_nw, 4))
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)  # already in dd32's order
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",

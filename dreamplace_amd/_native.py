@@ -39,10 +39,14 @@ _SIGNATURES = {
                             _I, _I, _I, _I, _L, _P, _P, _P, _P, _P]),
     "dp_density_map_ex": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D,
                                _L, _L, _I, _I, _I, _I, _L, _P, _P, _P, _P, _P, _P]),
+    "dp_density_map_noalloc": (_I, [_I, _P, _L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D,
+                                    _L, _L, _I, _I, _I, _P, _P, _P, _P]),
     "dp_fixed_density_map": (_I, [_I, _P, _L, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D, _L, _L, _I,
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
                                _I, _I, _I, _L, _I, _P, _P, _P, _P]),
+    "dp_energy_gradient": (_I, [_I, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
+                                _I, _I, _I, _P, _P, _P]),
     "dp_reduce_workspace_bytes": (_S, [_L]),
     "dp_density_overflow": (_I, [_I, _P, _L, _D, _P, _P, _P]),
     "dp_spectral_plan_bytes": (_S, [_I, _I, _I]),

@@ -497,3 +497,123 @@ int dp_density_overflow(int dtype, const void* density_map, int64_t num_bins, do
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free density map (dp_density_map_noalloc), for callers that
+// cannot allocate (code under HIP graph capture, serving loops): the
+// per-cell global-atomic scatter of dp_density_map with its int64 map and
+// large-cell queue in a file-scope device array, `scratch`, instead of a
+// caller workspace.  Each device holds its own copy (8.25 MiB) until the
+// process ends; every call on a device shares it, so calls that overlap on
+// one device race.  Nothing in it is assumed zero at entry: the map and the
+// queue counter are cleared (or the map seeded) by kernels, not by
+// hipMemsetAsync, so the call can be captured into a graph.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace dp {
+constexpr int64_t kNoAllocMaxBins = (int64_t)1 << 20;  // 1024 x 1024 bins
+constexpr int kNoAllocQueue = 1 << 16;                 // queued cells above 3x3 bins
+constexpr size_t kNoAllocCountOffset = 8 * (size_t)kNoAllocMaxBins;  // after the int64 map
+constexpr size_t kNoAllocQueueOffset = kNoAllocCountOffset + 256;
+constexpr size_t kNoAllocBytes = kNoAllocQueueOffset + 4 * (size_t)kNoAllocQueue;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace dp
+
+static __device__ __align__(256) unsigned char scratch[dp::kNoAllocBytes];
+
+namespace dp {
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kNoAllocMaxDevices];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's name in host code is only a stand-in for the symbol).
+static int scratch_address(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("dp_density_map_noalloc: hipGetDevice failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  DP_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, DP_ERR_UNSUPPORTED,
+             "dp_density_map_noalloc: device %d beyond the %d supported", dev,
+             kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("dp_density_map_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s",
+                dev, hipGetErrorString(e));
+      return e != hipSuccess ? (int)e : DP_ERR_WORKSPACE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DP_OK;
+}
+}  // namespace dp
+
+extern "C" int dp_density_map_noalloc(int dtype, const void* pos, int64_t num_nodes,
+                                      const void* node_size_x_clamped,
+                                      const void* node_size_y_clamped, const void* offset_x,
+                                      const void* offset_y, const void* ratio,
+                                      const void* initial_density_map, double xl, double yl,
+                                      double xh, double yh, double bin_size_x, double bin_size_y,
+                                      int64_t num_movable_nodes, int64_t num_filler_nodes,
+                                      int num_bins_x, int num_bins_y, int convention,
+                                      const double* uniform_filler, void* density_map_out,
+                                      int64_t* fixed_map_out, void* stream) {
+  DP_ARG(pos && node_size_x_clamped && node_size_y_clamped && offset_x && offset_y && ratio,
+         "dp_density_map_noalloc: null input array");
+  DP_ARG(num_bins_x > 0 && num_bins_y > 0, "dp_density_map_noalloc: bad bin counts %d x %d",
+         num_bins_x, num_bins_y);
+  DP_REQUIRE((int64_t)num_bins_x * num_bins_y <= kNoAllocMaxBins, DP_ERR_UNSUPPORTED,
+             "dp_density_map_noalloc: %d x %d bins exceed its fixed scratch of %lld bins "
+             "(1024 x 1024); use dp_density_map with a workspace",
+             num_bins_x, num_bins_y, (long long)kNoAllocMaxBins);
+  DP_ARG(num_movable_nodes >= 0 && num_filler_nodes >= 0 &&
+             num_movable_nodes + num_filler_nodes <= num_nodes,
+         "dp_density_map_noalloc: bad node counts");
+  DP_ARG(num_nodes < (int64_t)1 << 31, "dp_density_map_noalloc: more than 2^31 nodes");
+  DP_ARG(convention == DP_CONVERT_CPU || convention == DP_CONVERT_GPU,
+         "dp_density_map_noalloc: bad convention %d", convention);
+  DP_ARG(density_map_out || fixed_map_out, "dp_density_map_noalloc: no output");
+  void* base = nullptr;
+  DP_CHECK_RC(scratch_address(&base));
+  hipStream_t s = S(stream);
+  const int64_t nb = (int64_t)num_bins_x * num_bins_y;
+  const int64_t scale = dp_fixed_point_scale(xl, yl, xh, yh);
+  DensityWorkspace w;
+  w.map = (unsigned long long*)base;
+  w.filler_map = nullptr;
+  w.qcount = (int*)((char*)base + kNoAllocCountOffset);
+  w.qlist = (int*)((char*)base + kNoAllocQueueOffset);
+  w.tiled = nullptr;
+  return DP_DISPATCH(dtype, "dp_density_map_noalloc", [&] {
+    TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
+                                    offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
+                                    num_bins_x, num_bins_y, scale, uniform_filler,
+                                    num_nodes - num_filler_nodes);
+    const T* init = (const T*)initial_density_map;
+    if (convention == DP_CONVERT_GPU && init)
+      hipLaunchKernelGGL(k_quantize<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, init, nb,
+                         (T)scale, w.map);
+    else
+      zero_words(w.map, nb, s);
+    DP_LAUNCHED("density init (noalloc)");
+    DP_CHECK_RC(scatter_global<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
+                                  num_filler_nodes, w, kNoAllocQueue, s));
+    if (convention == DP_CONVERT_CPU)
+      hipLaunchKernelGGL(k_convert_cpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map,
+                         (const unsigned long long*)nullptr, init, nb, 1.0 / (double)scale,
+                         (T*)density_map_out, (T*)nullptr, (long long*)fixed_map_out);
+    else
+      hipLaunchKernelGGL(k_convert_gpu<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, w.map,
+                         (const unsigned long long*)nullptr, init, nb, (T)scale,
+                         (T)(1.0 / (double)scale), (T*)density_map_out, (T*)nullptr,
+                         (long long*)fixed_map_out);
+    DP_LAUNCHED("density convert (noalloc)");
+    return DP_OK;
+  });
+}

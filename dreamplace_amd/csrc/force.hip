@@ -107,9 +107,86 @@ __global__ void __launch_bounds__(256) k_force_large(TriArgs<T> a, const T* __re
   }
 }
 
+// d/dx of tri(x, s, xl, k, b) from the right: [x+s < bin right] - [x >= bin
+// left], the derivatives of the two sides of the overlap.
+template <typename T>
+__device__ __forceinline__ T dtri(T x, T s, T xl, int k, T b) {
+  T bxl = xl + (T)k * b;
+  T e = x + s, be = bxl + b;
+  return ((e < be) ? T(1) : T(0)) - ((x >= bxl) ? T(1) : T(0));
+}
+
+// Exact adjoint of the energy: one thread per node over the cell's scatter
+// bin range (k outer, h inner, accumulation in T, then * ratio * coef).
+// Only the bins holding a cell edge have a non-zero overlap derivative, so
+// a cell touches the potential map at the same bins as the scatter.
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy_grad(TriArgs<T> a, const T* __restrict__ phi,
+                                                     const T* grad, T coef, int64_t num_nodes,
+                                                     int64_t n_mov, int64_t n_fill, int il,
+                                                     T* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_nodes) return;
+  if (i >= n_mov && i < num_nodes - n_fill) {
+    put_grad(out, num_nodes, i, T(0), T(0), il);
+    return;
+  }
+  CellBins<T> c = cell_bins(a, i);
+  T g = grad ? *grad : T(1);
+  T gx = 0, gy = 0;
+  for (int k = c.bxl; k < c.bxh; ++k) {
+    T px = tri(c.x, c.sx, a.xl, k, a.bsx);
+    T dpx = dtri(c.x, c.sx, a.xl, k, a.bsx);
+    const int64_t row = (int64_t)k * a.nby;
+    for (int h = c.byl; h < c.byh; ++h) {
+      T py = tri(c.y, c.sy, a.yl, h, a.bsy);
+      T dpy = dtri(c.y, c.sy, a.yl, h, a.bsy);
+      T p = phi[row + h];
+      gx += (dpx * py) * p;
+      gy += (px * dpy) * p;
+    }
+  }
+  put_grad(out, num_nodes, i, ((gx * c.r) * coef) * g, ((gy * c.r) * coef) * g, il);
+}
+
 }  // namespace dp
 
 using namespace dp;
+
+extern "C" int dp_energy_gradient(int dtype, const void* grad_scale, const void* potential_map,
+                                  const void* pos, int64_t num_nodes,
+                                  const void* node_size_x_clamped, const void* node_size_y_clamped,
+                                  const void* offset_x, const void* offset_y, const void* ratio,
+                                  double xl, double yl, double bin_size_x, double bin_size_y,
+                                  int64_t num_movable_nodes, int64_t num_filler_nodes,
+                                  int num_bins_x, int num_bins_y, int grad_layout,
+                                  const double* uniform_filler, void* grad_out, void* stream) {
+  DP_ARG(potential_map && pos && node_size_x_clamped && node_size_y_clamped && offset_x &&
+             offset_y && ratio && grad_out,
+         "dp_energy_gradient: null pointer");
+  DP_ARG(num_bins_x > 0 && num_bins_y > 0 && bin_size_x > 0 && bin_size_y > 0,
+         "dp_energy_gradient: bad bins");
+  DP_ARG(num_movable_nodes >= 0 && num_filler_nodes >= 0 &&
+             num_movable_nodes + num_filler_nodes <= num_nodes && num_nodes < ((int64_t)1 << 31),
+         "dp_energy_gradient: bad node counts");
+  DP_ARG(grad_layout == DP_LAYOUT_SPLIT || grad_layout == DP_LAYOUT_INTERLEAVED,
+         "dp_energy_gradient: bad grad_layout %d", grad_layout);
+  if (num_nodes == 0) return DP_OK;
+  hipStream_t s = S(stream);
+  const int il = grad_layout == DP_LAYOUT_INTERLEAVED;
+  return DP_DISPATCH(dtype, "dp_energy_gradient", [&] {
+    TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
+                                    offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
+                                    num_bins_x, num_bins_y, 1, uniform_filler,
+                                    num_nodes - num_filler_nodes);
+    const T coef = (T)(2.0 / (bin_size_x * bin_size_y));
+    hipLaunchKernelGGL(k_energy_grad<T>, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0,
+                       s, a, (const T*)potential_map, (const T*)grad_scale, coef, num_nodes,
+                       num_movable_nodes, num_filler_nodes, il, (T*)grad_out);
+    DP_LAUNCHED("k_energy_grad");
+    return DP_OK;
+  });
+}
 
 extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
                                  const void* field_map_y, const void* pos, int64_t num_nodes,

@@ -13,7 +13,10 @@ Additions (keyword-only, defaults reproduce the reference):
 ``precision`` ("fp64" | "fp32" | "bf16x3") for the spectral solve,
 ``energy_mode`` ("potential": materialise phi = IDCT2(...) as the reference
 does; "spectral": evaluate the same energy from the DCT coefficients without
-the fourth transform), ``convention`` and ``algorithm`` of the density map.
+the fourth transform), ``convention`` and ``algorithm`` of the density map,
+and ``backward``: "force" (the reference's field gather) or "adjoint" (the
+exact derivative of the returned energy, dp_energy_gradient; the forward
+then also materialises phi).
 Fence regions (``region_id`` / ``fence_regions``) are not supported and raise.
 """
 import logging
@@ -38,6 +41,7 @@ class ElectricPotentialFunction(Function):
         ctx.op = op
         ctx.field_map_x = fx
         ctx.field_map_y = fy
+        ctx.potential_map = op.potential_map
         ctx.binning = op.binning_generation
         ctx.save_for_backward(pos)
         op.last_density_map = density_map
@@ -51,6 +55,8 @@ class ElectricPotentialFunction(Function):
             # another density call has re-binned the cells since this forward:
             # bin these positions again for the gather
             op.density_map(pos)
+        if op.backward == "adjoint":
+            return op.energy_gradient(pos, ctx.potential_map, grad_pos), None
         grad = op.electric_force(pos, ctx.field_map_x, ctx.field_map_y, grad_pos)
         return grad, None
 
@@ -63,7 +69,8 @@ class ElectricPotential(ElectricOverflow):
                  num_filler_nodes, padding, deterministic_flag, sorted_node_map,
                  movable_macro_mask=None, fast_mode=False, region_id=None, fence_regions=None,
                  node2fence_region_map=None, placedb=None, *, precision=None,
-                 energy_mode="potential", convention="cpu", algorithm="global"):
+                 energy_mode="potential", convention="cpu", algorithm="global",
+                 backward="force"):
         if region_id is not None or fence_regions is not None:
             raise NotImplementedError("dreamplace_amd.ElectricPotential: fence regions are not supported")
         super(ElectricPotential, self).__init__(
@@ -79,8 +86,11 @@ class ElectricPotential(ElectricOverflow):
         self.placedb = placedb
         if energy_mode not in ("potential", "spectral"):
             raise ValueError("energy_mode must be 'potential' or 'spectral'")
+        if backward not in ("force", "adjoint"):
+            raise ValueError("backward must be 'force' or 'adjoint'")
         self.precision = precision
         self.energy_mode = energy_mode
+        self.backward = backward
         self.last_density_map = None
         self.potential_map = None
 
@@ -97,9 +107,11 @@ class ElectricPotential(ElectricOverflow):
         plan = self.plan(density_map.dtype, density_map.device)
         d = density_map if density_map.dtype == plan.dtype else density_map.to(plan.dtype)
         want_energy = not self.fast_mode
-        want_phi = want_energy and self.energy_mode == "potential"
+        want_phi = (want_energy and self.energy_mode == "potential") or self.backward == "adjoint"
         fx, fy, phi, energy = plan.poisson(d, self.bin_size_x, self.bin_size_y,
                                            compute_potential=want_phi, compute_energy=want_energy)
+        if phi is not None and phi.dtype != density_map.dtype:
+            phi = phi.to(density_map.dtype)
         self.potential_map = phi
         if fx.dtype != density_map.dtype:
             fx, fy = fx.to(density_map.dtype), fy.to(density_map.dtype)
@@ -124,6 +136,25 @@ class ElectricPotential(ElectricOverflow):
                float(self.bin_size_y), int(self.num_movable_nodes), int(self.num_filler_nodes),
                self.num_bins_x, self.num_bins_y, self.algorithm_code, self.large_capacity,
                layout, self.uniform_filler_arg(), N.ptr(self.workspace(pos.device)), N.ptr(out),
+               N.stream_ptr(pos.device))
+        return out
+
+    def energy_gradient(self, pos, potential_map, grad_pos, grad_layout="split"):
+        """The exact derivative of the energy sum(phi*rho) with respect to
+        the positions, times grad_pos (dp_energy_gradient): 2/(bx*by) *
+        ratio * sum (overlap derivative) * phi for movables and fillers, 0
+        for terminals.  grad_layout as for electric_force."""
+        layout = {"split": N.DP_LAYOUT_SPLIT, "interleaved": N.DP_LAYOUT_INTERLEAVED}[grad_layout]
+        pos = pos.view(-1)
+        grad_scale = grad_pos.reshape(()).to(pos.dtype).contiguous()
+        phi = potential_map.to(pos.dtype).contiguous()
+        out = torch.empty_like(pos)
+        N.call("dp_energy_gradient", N.dtype_code(pos), N.ptr(grad_scale), N.ptr(phi), N.ptr(pos),
+               pos.numel() // 2, N.ptr(self.node_size_x_clamped), N.ptr(self.node_size_y_clamped),
+               N.ptr(self.offset_x), N.ptr(self.offset_y), N.ptr(self.ratio), float(self.xl),
+               float(self.yl), float(self.bin_size_x), float(self.bin_size_y),
+               int(self.num_movable_nodes), int(self.num_filler_nodes), self.num_bins_x,
+               self.num_bins_y, layout, self.uniform_filler_arg(), N.ptr(out),
                N.stream_ptr(pos.device))
         return out
 

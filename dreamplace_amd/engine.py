@@ -336,7 +336,7 @@ class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
                  energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
                  keep_partial_wl=False, wirelength="weighted_average", overlap=True,
-                 track_metrics=False):
+                 track_metrics=False, density_backward="force"):
         """keep_partial_wl: also store the per-net (pre-weight) wirelength in
         `partial_wl` (what the net-weight adjoint needs); the pins schedule
         skips that store otherwise.  wirelength: "weighted_average" or
@@ -351,7 +351,13 @@ class PlacementObjective:
         overflow from the same scatter's fixed + movable map.  Both are
         bit-identical to dp_hpwl and to ElectricOverflow's own scatter.
         `metric_ptr` is the address of those positions; a caller that changes
-        them in place clears it."""
+        them in place clears it.  density_backward: "force" (the reference's
+        field gather) or "adjoint" (the exact derivative of the energy,
+        dp_energy_gradient, from the potential map the solve then writes)."""
+        if density_backward not in ("force", "adjoint"):
+            raise ValueError("density_backward must be 'force' or 'adjoint', got %r"
+                             % (density_backward,))
+        self.density_backward = density_backward
         if wirelength not in ("weighted_average", "logsumexp"):
             raise ValueError("wirelength must be weighted_average or logsumexp, got %r" % (wirelength,))
         self.lse = wirelength == "logsumexp"
@@ -393,7 +399,8 @@ class PlacementObjective:
         pdt = self.plan.dtype
         self.field_x = torch.empty(M, Nb, dtype=pdt, device=dev)
         self.field_y = torch.empty_like(self.field_x)
-        self.potential = torch.empty_like(self.field_x) if energy_mode == "potential" else None
+        self.potential = (torch.empty_like(self.field_x)
+                          if energy_mode == "potential" or density_backward == "adjoint" else None)
         self.energy = torch.empty((), dtype=pdt, device=dev)
         # the pins schedule keeps the density gradient interleaved ((x, y) per
         # node: one scattered word per cell in the gather); others keep [2N]
@@ -543,7 +550,8 @@ class PlacementObjective:
             dmap = dmap.to(self.plan.dtype)
         want_energy = not ep.fast_mode
         self.plan.poisson(dmap, ep.bin_size_x, ep.bin_size_y,
-                          compute_potential=self.potential is not None and want_energy,
+                          compute_potential=self.potential is not None
+                          and (want_energy or self.density_backward == "adjoint"),
                           compute_energy=want_energy,
                           out=(self.field_x, self.field_y, self.potential, self.energy))
 
@@ -580,10 +588,20 @@ class PlacementObjective:
 
     def _density_backward(self, pos):
         ep = self.density
+        w = self._density_grad_weight(pos)
+        if self.density_backward == "adjoint":
+            phi = self.potential if self.potential.dtype == pos.dtype else self.potential.to(pos.dtype)
+            N.call("dp_energy_gradient", N.dtype_code(pos), N.ptr(w), N.ptr(phi), N.ptr(pos),
+                   pos.numel() // 2, N.ptr(ep.node_size_x_clamped), N.ptr(ep.node_size_y_clamped),
+                   N.ptr(ep.offset_x), N.ptr(ep.offset_y), N.ptr(ep.ratio), float(ep.xl),
+                   float(ep.yl), float(ep.bin_size_x), float(ep.bin_size_y),
+                   int(ep.num_movable_nodes), int(ep.num_filler_nodes), ep.num_bins_x,
+                   ep.num_bins_y, self.grad_layout, ep.uniform_filler_arg(),
+                   N.ptr(self._density_grad), N.stream_ptr(pos.device))
+            return
         fx, fy = self.field_x, self.field_y
         if fx.dtype != pos.dtype:
             fx, fy = fx.to(pos.dtype), fy.to(pos.dtype)
-        w = self._density_grad_weight(pos)
         N.call("dp_electric_force", N.dtype_code(pos), N.ptr(w), N.ptr(fx), N.ptr(fy),
                N.ptr(pos), pos.numel() // 2, N.ptr(ep.node_size_x_clamped),
                N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x), N.ptr(ep.offset_y),

@@ -152,6 +152,25 @@ int dp_density_map_ex(int dtype, const void* pos, int64_t num_nodes,
                       void* workspace, void* density_map_out, void* movable_map_out,
                       void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): dp_density_map with algorithm DP_SCATTER_GLOBAL and no
+ * workspace argument.  The int64 map and the large-cell queue live in a
+ * file-scope device array of the library (8.25 MiB, one copy per device,
+ * held until the process ends); the call allocates nothing.  Limit: at most
+ * 1024 x 1024 bins (DP_ERR_UNSUPPORTED beyond).  Every call on a device
+ * shares that array, so calls that overlap on one device (two streams, two
+ * threads) race.  Same arguments and results as dp_density_map otherwise
+ * (bit-identical map and fixed-point sum). */
+int dp_density_map_noalloc(int dtype, const void* pos, int64_t num_nodes,
+                           const void* node_size_x_clamped, const void* node_size_y_clamped,
+                           const void* offset_x, const void* offset_y, const void* ratio,
+                           const void* initial_density_map, double xl, double yl, double xh,
+                           double yh, double bin_size_x, double bin_size_y,
+                           int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                           int num_bins_y, int convention, const double* uniform_filler,
+                           void* density_map_out, int64_t* fixed_map_out, void* stream);
+
 /* Replaces electric_potential_cuda.fixed_density_map
  * (electric_density_map_cuda.cpp:139-182, electric_density_map.cpp:155-213)
  * followed by the Python-side scaling of electric_overflow.py:258:
@@ -184,6 +203,29 @@ int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x
                       int64_t num_filler_nodes, int num_bins_x, int num_bins_y, int algorithm,
                       int64_t large_capacity, int grad_layout, const double* uniform_filler,
                       void* workspace, void* grad_out, void* stream);
+
+/* The exact gradient of the density energy E = sum(phi * rho) that
+ * ElectricPotentialFunction.forward returns (electric_potential.py:137-186;
+ * rho = map / (bin_size_x * bin_size_y), phi = IDCT2(DCT2(rho) * K)).  No
+ * counterpart in the reference, whose backward is the field gather above
+ * (electric_potential.py:219-245): this is its exact adjoint, offered beside it.
+ * The solve is symmetric in rho, so dE/dmap_b = 2 * phi_b / (bin_size_x *
+ * bin_size_y); through each cell's triangle overlaps over the scatter's bin
+ * range (density_function.h:12-17, electric_density_map.cpp:274-293):
+ *   grad_out_x[i] = (*grad_scale) * 2/(bin_size_x*bin_size_y) * ratio_i *
+ *                   sum_{k,h} dpx_k * py_h * phi[k,h]
+ * and y with px_k * dpy_h, where dpx_k = [x+sx < bin_k right] - [x >= bin_k
+ * left] is the overlap's derivative from the right.  0 for terminals.
+ * potential_map is phi ([num_bins_x * num_bins_y], `dtype`, as dp_poisson_solve
+ * writes it); grad_scale nullable (1); grad_layout and uniform_filler as for
+ * dp_electric_force.  Needs no workspace. */
+int dp_energy_gradient(int dtype, const void* grad_scale, const void* potential_map,
+                       const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
+                       const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
+                       const void* ratio, double xl, double yl, double bin_size_x,
+                       double bin_size_y, int64_t num_movable_nodes, int64_t num_filler_nodes,
+                       int num_bins_x, int num_bins_y, int grad_layout,
+                       const double* uniform_filler, void* grad_out, void* stream);
 
 /* ElectricOverflow metric (electric_overflow.py:260-278): out2[0] =
  * sum(max(map - target_area, 0)), out2[1] = max(map); FP64 accumulation,

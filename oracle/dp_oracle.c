@@ -168,6 +168,55 @@ int64_t dpo_fixed_scale(double xl, double yl, double xh, double yh) {
       gyo[i] = gy * grad;                                                                    \
     }                                                                                        \
   }                                                                                          \
+  /* d/dx of tri_(x, ...) from the right: [x + s < bin_xh] - [x >= bin_xl], the          \
+   * derivatives of the two sides of density_function.h:12-17. */                          \
+  static inline T dtri_##SFX(T x, T node_size, T xl, int k, T bin_size) {                  \
+    T bin_xl = xl + k * bin_size;                                                            \
+    T up = (x + node_size < bin_xl + bin_size) ? (T)1 : (T)0;                                \
+    T dn = (x >= bin_xl) ? (T)1 : (T)0;                                                      \
+    return up - dn;                                                                          \
+  }                                                                                          \
+  /* Exact gradient of the density energy E = sum_b phi_b*rho_b of                          \
+   * electric_potential.py:137-186 (rho = D/(bsx*bsy); phi = L rho with L symmetric, so      \
+   * dE/dD_b = 2*phi_b/(bsx*bsy)), through each cell's triangle overlaps                    \
+   * (density_function.h:12-17) over the scatter's bin range                                \
+   * (electric_density_map.cpp:274-293):                                                     \
+   *   g_x[i] = grad * (2/(bsx*bsy)) * ratio_i * sum_{k,h} dpx_k * py_h * phi[k,h]           \
+   * and g_y with px_k * dpy_h.  The reference's backward is the force instead              \
+   * (electric_force.cpp); no reference routine computes this one.  Same loop order         \
+   * as the force (k outer, h inner, accumulation in T, then * ratio). */                   \
+  void dpo_energy_gradient_##SFX(T grad, const T* phi, const T* xv, const T* yv,           \
+                                 const T* wxc, const T* wyc, const T* offx, const T* offy,   \
+                                 const T* ratio_v, int64_t n, int nbx, int nby, double xl_d, \
+                                 double yl_d, double bsx_d, double bsy_d, T* gxo, T* gyo) {  \
+    const T xl = (T)xl_d, yl = (T)yl_d, bsx = (T)bsx_d, bsy = (T)bsy_d;                     \
+    const T inv_x = (T)(1.0 / bsx), inv_y = (T)(1.0 / bsy);                                  \
+    const T coef = (T)(2.0 / (bsx_d * bsy_d));                                               \
+    _Pragma("omp parallel for schedule(static)")                                            \
+    for (int64_t i = 0; i < n; ++i) {                                                        \
+      T sx = wxc[i], sy = wyc[i];                                                            \
+      T nx = xv[i] + offx[i], ny = yv[i] + offy[i];                                          \
+      T r = ratio_v[i];                                                                      \
+      int bxl = imax((int)((nx - xl) * inv_x), 0);                                           \
+      int bxh = imin((int)((nx + sx - xl) * inv_x) + 1, nbx);                                \
+      int byl = imax((int)((ny - yl) * inv_y), 0);                                           \
+      int byh = imin((int)((ny + sy - yl) * inv_y) + 1, nby);                                \
+      T gx = 0, gy = 0;                                                                      \
+      for (int k = bxl; k < bxh; ++k) {                                                      \
+        T px = tri_##SFX(nx, sx, xl, k, bsx);                                                \
+        T dpx = dtri_##SFX(nx, sx, xl, k, bsx);                                              \
+        for (int h = byl; h < byh; ++h) {                                                    \
+          T py = tri_##SFX(ny, sy, yl, h, bsy);                                              \
+          T dpy = dtri_##SFX(ny, sy, yl, h, bsy);                                            \
+          T p = phi[(int64_t)k * nby + h];                                                   \
+          gx += (dpx * py) * p;                                                              \
+          gy += (px * dpy) * p;                                                              \
+        }                                                                                    \
+      }                                                                                      \
+      gxo[i] = ((gx * r) * coef) * grad;                                                     \
+      gyo[i] = ((gy * r) * coef) * grad;                                                     \
+    }                                                                                        \
+  }                                                                                          \
   /* Merged WA forward: weighted_average_wirelength_merged.cpp:31-116.                      \
    * partial_wl[i] is the pre-weight x+y wirelength of net i (0 when masked);               \
    * gi_x/gi_y receive d(WL_i)/d(pin) for every pin of an unmasked net. */                 \

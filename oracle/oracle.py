@@ -577,6 +577,35 @@ def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_co
     return obj, wgrad + dgrad, parts
 
 
+def energy_gradient(dp, pos, potential, grad=1.0):
+    """The exact gradient of the density energy sum(phi*rho) with respect to
+    the positions (dp_oracle.c dpo_energy_gradient): grad * 2/(bx*by) *
+    ratio * sum (d overlap) * phi for movables and fillers, 0 for fixed
+    cells.  Returns [2N] in dp.dtype.  Pinned by central finite differences
+    of the oracle's energy (tests/test_oracle.py)."""
+    n = pos.size // 2
+    T = dp.dtype
+    sfx, ct = _sfx(T)
+    out = np.zeros(2 * n, dtype=T)
+    phi = _c(potential, T).reshape(-1)
+    rng = [(0, dp.num_movable_nodes)]
+    if dp.num_filler_nodes:
+        rng.append((n - dp.num_filler_nodes, n))
+    for b, e in rng:
+        gx = np.zeros(e - b, dtype=T)
+        gy = np.zeros(e - b, dtype=T)
+        arrs = [_c(a, T) for a in (pos[b:e], pos[n + b:n + e], dp.wxc[b:e], dp.wyc[b:e],
+                                   dp.offx[b:e], dp.offy[b:e], dp.ratio[b:e])]
+        getattr(lib(), "dpo_energy_gradient_" + sfx)(
+            ct(grad), _p(phi), *[_p(a) for a in arrs], ctypes.c_int64(e - b),
+            ctypes.c_int(dp.num_bins_x), ctypes.c_int(dp.num_bins_y), ctypes.c_double(dp.xl),
+            ctypes.c_double(dp.yl), ctypes.c_double(dp.bin_size_x), ctypes.c_double(dp.bin_size_y),
+            _p(gx), _p(gy))
+        out[b:e] = gx
+        out[n + b:n + e] = gy
+    return out
+
+
 def overflow(density_map_movable, target_density, bin_size_x, bin_size_y, total_movable_area):
     """ElectricOverflow.forward metric (electric_overflow.py:260-278,
     EvalMetrics.py:116-128): sum(max(D - t*A_bin, 0)) / movable area, and

@@ -200,13 +200,19 @@ def entropy_injection(d, pos, shrink, intensity, z):
     pos[n + m0:n + m1] = keep[n + m0:n + m1]
 
 
-def place(d, iterations, params=None, seed=0):
+def place(d, iterations, params=None, seed=0, objective=None, log_every=0):
     """Run the default-parameter global placement loop on design `d` (a
     synth.make_design dict; positions in its dtype).  Returns (pos, metrics,
     info): metrics a list of dicts (hpwl, overflow, max_density,
     density_weight, gamma) taken before each step, as EvalMetrics records
     them; info has the best-overflow record, whether the divergence test
-    rolled back, and the number of plateau perturbations."""
+    rolled back, and the number of plateau perturbations.
+    `objective(d, density_weight, gamma, density_factor=, quad_coeff=)`
+    returns (obj, grad, parts with "energy", "wl_grad", "density_grad"); the
+    default is the oracle's own composition (oracle.objective_and_gradient),
+    ref_iteration.reference_objective drives the loop with the reference's
+    CPU ops instead."""
+    objective = objective or O.objective_and_gradient
     p = dict(DEFAULTS)
     p.update(params or {})
     d = dict(d)
@@ -242,8 +248,8 @@ def place(d, iterations, params=None, seed=0):
         dd["pos"] = x
         if st["init"] is None:
             # PlaceObj.obj_fn keeps the density of its first evaluation
-            st["init"] = float(T(O.objective_and_gradient(dd, st["lam"], st["gamma"])[2]["energy"]))
-        obj, g, _ = O.objective_and_gradient(dd, st["lam"], st["gamma"], density_factor=st["factor"],
+            st["init"] = float(T(objective(dd, st["lam"], st["gamma"])[2]["energy"]))
+        obj, g, _ = objective(dd, st["lam"], st["gamma"], density_factor=st["factor"],
                                              quad_coeff=quad_coeff())
         g = g.astype(dtype)
         pre = np.maximum(pws + (T(p["precondition_alpha"]) * T(st["lam"])) * areas, T(1))
@@ -283,12 +289,15 @@ def place(d, iterations, params=None, seed=0):
         if st["lam"] == 0.0:
             dd = dict(d)
             dd["pos"] = pos
-            _, _, parts = O.objective_and_gradient(dd, 1.0, st["gamma"])
+            _, _, parts = objective(dd, 1.0, st["gamma"])
             wl_norm = float(np.sum(np.abs(parts["wl_grad"].astype(np.float64))))
             d_norm = float(np.sum(np.abs(parts["density_grad"].astype(np.float64))))
             st["lam"] = float(T(p["density_weight"] * wl_norm / d_norm))
         cur = metrics(pos)
         hist.append(cur)
+        if log_every and it % log_every == 0:
+            print("iteration %4d  HPWL %.6E  overflow %.4f  lambda %.3E  gamma %.3E"
+                  % (it, cur["hpwl"], cur["overflow"], cur["density_weight"], cur["gamma"]), flush=True)
         opt.step()
         if info["best"] is None or info["best"]["overflow"] > cur["overflow"]:
             info["best"] = cur

@@ -160,3 +160,88 @@ class RefIteration:
         grad = wgrad + dgrad
         return obj, grad, dict(wl=wl, energy=energy, density_map=dmap, field_x=fx, field_y=fy, phi=phi,
                                density_grad=dgrad, wl_grad=wgrad)
+
+
+def reference_objective(d, dtype=torch.float32, num_threads=None):
+    """An objective for oracle.placer.place that evaluates PlaceObj.obj_fn /
+    obj_and_grad_fn (dreamplace/PlaceObj.py:293-320,394-411) with the
+    reference's own CPU ops: fn(d, density_weight, gamma, density_factor=1.0,
+    quad_coeff=None) -> (obj, grad[2N] numpy, parts).  With the density
+    factor f and the quadratic penalty c the density term is
+    (f*lambda) * E * (1 + c*E) and its gradient the force scaled by
+    g*(1 + c*E) + (g*E)*c, g = f*lambda (the product rule, in autograd's
+    order)."""
+    it = RefIteration(d, dtype=dtype, num_threads=num_threads, density_weight=1.0)
+    T = np.float32 if dtype == torch.float32 else np.float64
+
+    def fn(dd, density_weight, gamma, density_factor=1.0, quad_coeff=None):
+        pos = torch.from_numpy(np.ascontiguousarray(dd["pos"])).to(dtype)
+        it.inv_gamma = torch.tensor(1.0 / gamma, dtype=dtype)
+        g = T(density_weight)
+        if density_factor != 1.0 or quad_coeff is not None:
+            g = T(T(density_factor) * T(density_weight))
+        # the energy decides the quadratic penalty's scale: one evaluation at
+        # lambda = 1 gives E and the unit force, which is linear in lambda
+        it.lam = 1.0
+        _, _, parts = it.iterate(pos)
+        e = T(float(parts["energy"]))
+        scale = g
+        energy = e
+        if quad_coeff is not None:
+            c = T(quad_coeff)
+            scale = g * (T(1) + c * e) + (g * e) * c
+            energy = e * (T(1) + c * e)
+        wl = float(parts["wl"])
+        dgrad = parts["density_grad"].numpy().astype(T) * scale
+        wgrad = parts["wl_grad"].numpy().astype(T)
+        obj = wl + (density_factor * density_weight) * float(energy)
+        out = dict(energy=float(e), wl=wl, wl_grad=wgrad, density_grad=dgrad)
+        return obj, (wgrad + dgrad).astype(T), out
+
+    return fn
+
+
+def main(argv=None):
+    """Global placement of a synthetic design with the reference's CPU ops
+    driving the restated loop (oracle.placer.place); prints one JSON line
+    with the final metrics.  Test infrastructure: the reference CPU run that
+    the device runs are compared with."""
+    import argparse
+    import json
+    import time
+
+    from dreamplace_amd import synth
+
+    from . import placer as OP
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--iterations", type=int, default=1000)
+    ap.add_argument("--stop-overflow", type=float, default=0.07)
+    ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
+    ap.add_argument("--threads", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--log-every", type=int, default=50)
+    ap.add_argument("--history", default=None)
+    a = ap.parse_args(argv)
+    dt = np.float32 if a.dtype == "float32" else np.float64
+    d = synth.make_design(a.config, seed=a.seed, dtype=dt, scale=a.scale)
+    print(synth.summary(d), flush=True)
+    fn = reference_objective(d, torch.float32 if dt == np.float32 else torch.float64, a.threads)
+    t0 = time.time()
+    pos, hist, info = OP.place(d, a.iterations, params=dict(stop_overflow=a.stop_overflow),
+                               seed=a.seed, objective=fn, log_every=a.log_every)
+    t = time.time() - t0
+    last = hist[-1]
+    out = dict(config=a.config, scale=a.scale, dtype=a.dtype, threads=torch.get_num_threads(),
+               iterations=len(hist), final_hpwl=last["hpwl"], final_overflow=last["overflow"],
+               best_overflow=info["best"]["overflow"], rolled_back=info["rolled_back"],
+               seconds=t, s_per_iteration=t / max(len(hist), 1))
+    if a.history:
+        with open(a.history, "w") as f:
+            json.dump(hist, f)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

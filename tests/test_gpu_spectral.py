@@ -52,19 +52,21 @@ def test_poisson_solve(oracle, gpu, precision, tol):
     assert abs(e2.item() - ref["energy"]) <= tol * abs(ref["energy"])
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 @pytest.mark.parametrize("shape", [(2048, 2048), (2048, 4096)], ids=["2048x2048", "2048x4096"])
-def test_large_grid_bf16x3_against_fp64(gpu, shape):
-    """Grids with at least 256 tiles of 128 x 128 take the 128-tile BF16x3
-    GEMM: the DCT2 and the Poisson solve (fields, potential and the energy
-    from the GEMM epilogue) stay within the accuracy gate of the FP64 form on
-    the device (whose kernels the oracle pins at the sizes above)."""
+def test_large_grid_against_fp64(gpu, shape, precision):
+    """Large grids (the BF16x3 form takes its 128-tile GEMM from 256 tiles of
+    128 x 128 on): the DCT2 and the Poisson solve (fields, potential and the
+    energy from the GEMM epilogue) of the FP32 and BF16x3 forms stay within
+    the accuracy gate of the FP64 form on the device (whose kernels the
+    oracle pins at the sizes above)."""
     from dreamplace_amd.spectral import get_plan
-    print("case large grid bf16x3 %s" % (shape,), flush=True)
+    print("case large grid %s %s" % (precision, shape), flush=True)
     M, N = shape
     g = torch.Generator().manual_seed(7)
     x64 = (torch.rand(M, N, generator=g, dtype=torch.float64) * 2.0).to(gpu)
     p64 = get_plan(M, N, "fp64", gpu)
-    p16 = get_plan(M, N, "bf16x3", gpu)
+    p16 = get_plan(M, N, precision, gpu)
     x32 = x64.float()
     assert rel_l2(p16.dct2(x32).double().cpu().numpy(), p64.dct2(x64).cpu().numpy()) <= 2e-5
     if M != N:

@@ -487,6 +487,99 @@ def _wa_fixture(rng, n_nets=30, dtype=np.float64):
     return pos, flat, start, pin2net, w, mask
 
 
+def _edge_clear_coordinates(rng, d, dp, count, margin):
+    """`count` (cell, axis) pairs of movable or filler cells inside the die
+    whose stretched edges are at least `margin` from every bin edge."""
+    n = d.num_nodes
+    cells = np.concatenate([np.arange(d.num_movable_nodes),
+                            np.arange(n - d.num_filler_nodes, n)])
+    out = []
+    while len(out) < count:
+        i = int(rng.choice(cells))
+        ax = len(out) % 2
+        b = d.bin_size_x if ax == 0 else d.bin_size_y
+        lo = d.pos[ax * n + i] + (dp.offx if ax == 0 else dp.offy)[i]
+        hi = lo + (dp.wxc if ax == 0 else dp.wyc)[i]
+        l0, h0 = (d.xl, d.xh) if ax == 0 else (d.yl, d.yh)
+        if lo < l0 + b or hi > h0 - b:
+            continue
+        if min(abs(v / b - round(v / b)) * b for v in (lo - l0, hi - l0)) < margin:
+            continue
+        out.append((i, ax))
+    return out
+
+
+def test_energy_gradient_finite_differences(oracle):
+    """The exact energy gradient against central differences of the
+    oracle's own energy (FP64, config 0).  Between bin edges the map is
+    linear in a cell's position and the energy quadratic, so central
+    differences are exact up to the map's fixed-point quantum (2^-32 per
+    contribution): h = 1e-2*b must agree to 1e-6 of the largest derivative;
+    h = 1e-4*b (the quantum divided by a 100x smaller step) to 1e-3."""
+    d = design(0)
+    dp = density_params(oracle, d)
+    n, b = d.num_nodes, d.bin_size_x
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y)
+
+    def energy(pos):
+        return oracle.poisson_solve(oracle.density_map(dp, pos, init), b, d.bin_size_y)
+
+    sol = energy(d.pos)
+    g = oracle.energy_gradient(dp, d.pos, sol["phi"])
+    rng = np.random.default_rng(5)
+    for hf, tol in ((1e-2, 1e-6), (1e-4, 1e-3)):
+        h = hf * b
+        fd, an = [], []
+        for i, ax in _edge_clear_coordinates(rng, d, dp, 12, 3 * h):
+            p = d.pos.copy()
+            p[ax * n + i] += h
+            ep = energy(p)["energy"]
+            p[ax * n + i] -= 2 * h
+            em = energy(p)["energy"]
+            fd.append((ep - em) / (2 * h))
+            an.append(g[ax * n + i])
+        fd, an = np.array(fd), np.array(an)
+        assert np.abs(fd - an).max() <= tol * np.abs(fd).max(), hf
+    # fixed cells do not move the energy
+    assert not g[d.num_movable_nodes:n - d.num_filler_nodes].any()
+
+
+def test_energy_gradient_linear_potential_closed_form(oracle):
+    """For a linear potential phi[k,h] = a*(k+1/2) + c*(h+1/2) with its exact
+    field (E_x = -a, E_y = -c), both gathers have closed forms per cell inside
+    the die (stretched sizes sx, sy; edge bins kL, kR and hB, hT):
+      exact gradient  g_x = 2/(bx*by) * ratio * a * (kR - kL) * sy,
+      reference force f_x = ratio * a * sx * sy,
+    so f_x = (bx^2*by/2) * (sx/bx)/(kR - kL) * g_x: the force is the exact
+    gradient times bx^2*by/2 when a cell's edges lie its own width apart in
+    bins, as they do on average over positions (the relation of the
+    ePlace force to the energy's derivative)."""
+    d = design(0)
+    dp = density_params(oracle, d)
+    n, M, Nb = d.num_nodes, d.num_bins_x, d.num_bins_y
+    bx, by = d.bin_size_x, d.bin_size_y
+    a, c = 0.37, -1.3
+    phi = a * (np.arange(M)[:, None] + 0.5) + c * (np.arange(Nb)[None, :] + 0.5)
+    g = oracle.energy_gradient(dp, d.pos, phi)
+    f = oracle.electric_force(dp, d.pos, np.full((M, Nb), -a), np.full((M, Nb), -c))
+    cells = np.concatenate([np.arange(d.num_movable_nodes), np.arange(n - d.num_filler_nodes, n)])
+    x = d.pos[cells] + dp.offx[cells]
+    y = d.pos[n + cells] + dp.offy[cells]
+    sx, sy, r = dp.wxc[cells], dp.wyc[cells], dp.ratio[cells]
+    inside = (x > d.xl) & (x + sx < d.xh) & (y > d.yl) & (y + sy < d.yh)
+    assert inside.sum() > 0.9 * len(cells)
+    cells, x, y, sx, sy, r = (v[inside] for v in (cells, x, y, sx, sy, r))
+    kspan = np.floor((x + sx - d.xl) / bx) - np.floor((x - d.xl) / bx)
+    hspan = np.floor((y + sy - d.yl) / by) - np.floor((y - d.yl) / by)
+    coef = 2.0 / (bx * by)
+    assert np.allclose(g[cells], coef * r * a * kspan * sy, rtol=1e-12, atol=0)
+    assert np.allclose(g[n + cells], coef * r * c * hspan * sx, rtol=1e-12, atol=0)
+    assert np.allclose(f[cells], r * a * sx * sy, rtol=1e-12, atol=0)
+    assert np.allclose(f[n + cells], r * c * sx * sy, rtol=1e-12, atol=0)
+    const = bx * bx * by / 2
+    assert np.allclose(f[cells], const * (sx / bx) / kspan * g[cells], rtol=1e-12, atol=0)
+
+
 def test_wa_gradient_finite_difference(oracle):
     rng = np.random.default_rng(5)
     pos, flat, start, pin2net, w, mask = _wa_fixture(rng)
@@ -787,3 +880,21 @@ def test_lse_limits_and_gradient(oracle):
         fb = oracle.lse_merged_forward(b, d.flat_net2pin_map, d.flat_net2pin_start_map, np.ones(d.num_nets),
                                        ones, gamma)[1][e]
         assert (fa - fb) / (2 * hh) == pytest.approx(gi[i], abs=1e-7)
+
+
+def test_reference_ops_drive_the_same_loop(oracle, ref):
+    """The restated loop driven by the reference's own CPU ops
+    (ref_iteration.reference_objective) follows the loop driven by the
+    oracle's objective on config 0 (FP64): the ops agree to ~1e-13, so eight
+    iterations agree to 1e-8."""
+    import torch
+    from oracle import placer as OP
+    from oracle import ref_iteration as R
+    d = design(0)
+    fn = R.reference_objective(d, torch.float64, num_threads=4)
+    xr, hr, _ = OP.place(d, 8, objective=fn)
+    xo, ho, _ = OP.place(d, 8)
+    for a, b in zip(hr, ho):
+        for k in ("hpwl", "overflow", "density_weight", "gamma"):
+            assert a[k] == pytest.approx(b[k], rel=1e-8), k
+    assert rel_l2(xr, xo) <= 1e-8
