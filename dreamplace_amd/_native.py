@@ -20,6 +20,7 @@ DP_CONVERT_CPU, DP_CONVERT_GPU = 0, 1
 DP_SPECTRAL_FP64, DP_SPECTRAL_FP32, DP_SPECTRAL_BF16X3 = 0, 1, 2
 DP_SCATTER_GLOBAL, DP_SCATTER_TILED = 0, 1
 DP_LAYOUT_SPLIT, DP_LAYOUT_INTERLEAVED = 0, 1
+DP_NET_WEIGHTING_ADAMS, DP_NET_WEIGHTING_LILITH = 0, 1
 
 PRECISIONS = {"fp64": DP_SPECTRAL_FP64, "fp32": DP_SPECTRAL_FP32, "bf16x3": DP_SPECTRAL_BF16X3}
 
@@ -66,6 +67,7 @@ _SIGNATURES = {
     "dp_wa_merged_backward_recompute": (_I, [_I, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P,
                                              _P]),
     "dp_wa_net_weight_grad": (_I, [_I, _P, _P, _P, _L, _P, _P]),
+    "dp_net_weighting": (_I, [_I, _I, _L, _P, _P, _D, _P, _I, _D, _D, _P, _P, _P]),
     "dp_wa_forward_from_nodes": (_I, [_I, _P, _L, _P, _P, _P, _L, _P, _P, _L, _P, _P, _P, _P,
                                       _P, _P, _P, _P]),
     "dp_wa_backward_to_nodes": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P]),

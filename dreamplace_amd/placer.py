@@ -32,7 +32,9 @@ DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
                 density_factor=1.0, early_stop=True, filler_sort_interval=25, movable_order="netlist",
-                fuse_move_boundary=True, fused_metrics=True)
+                fuse_move_boundary=True, fused_metrics=True, timing_opt=False,
+                net_weighting_scheme="lilith", momentum_decay_factor=0.5, max_net_weight=math.inf,
+                timing_start=500, timing_interval=15)
 
 
 def noise(d, ratio, seed):
@@ -175,9 +177,19 @@ class GlobalPlacer:
     evaluated come from that evaluation (PlacementObjective track_metrics)
     instead of a pin-position pass, an HPWL pass and a second scatter (the
     same bits); where the objective cannot track them, the separate passes
-    run."""
+    run.
+    timing_opt: timing-driven net weighting (NonLinearPlace.py:463-510):
+    after the step of every iteration above timing_start (500) that is a
+    multiple of timing_interval (15), `timer(placer)` reports per-net slacks
+    and the WNS (LILITH) or the critical nets (ADAMS), and the net weights
+    are updated on the device (net_weighting.NetWeighting, params
+    net_weighting_scheme, momentum_decay_factor, max_net_weight); the
+    objective, the preconditioner's pin weights and the HPWL metric follow
+    the new weights, as the reference's in-place update of its net weights
+    makes them.  The default timer is net_weighting.WirelengthTimer, whose
+    net delays are the net-weight adjoint d obj / d w_n."""
 
-    def __init__(self, d, device, params=None, dtype=None, seed=0, **objective_kw):
+    def __init__(self, d, device, params=None, dtype=None, seed=0, timer=None, **objective_kw):
         p = dict(DEFAULTS)
         p.update(params or {})
         self.p = p
@@ -227,6 +239,18 @@ class GlobalPlacer:
         self.movable_area = float(np.sum((d["node_size_x"][:d["num_movable_nodes"]] *
                                           d["node_size_y"][:d["num_movable_nodes"]]).astype(np.float64)))
         self.pos = dd.pos
+        self.net_weighting = None
+        self.timer = None
+        if p["timing_opt"]:
+            from .net_weighting import NetWeighting, WirelengthTimer
+            self.timer = timer if timer is not None else WirelengthTimer()
+            if isinstance(self.timer, WirelengthTimer):
+                self.objective.keep_partial_wl = True
+            self.net_weighting = NetWeighting(
+                dd.flat_net2pin_start_map, scheme=p["net_weighting_scheme"],
+                momentum_decay_factor=p["momentum_decay_factor"],
+                max_net_weight=p["max_net_weight"], ignore_net_degree=d.get("ignore_net_degree", 100),
+                dtype=dd.dtype)
         self.history = []
         self.best = None
         self.best_positions = BestPositions(None, defer=False)
@@ -234,10 +258,26 @@ class GlobalPlacer:
         self.perturbations = 0
         self.optimizer = None
         self.filler_sorts = 0
+        self.net_weight_updates = 0
         interval = int(p["filler_sort_interval"])
         self.sorting = interval > 0 and dd.num_filler_nodes > 0 and dd.uniform_fillers()
         if self.sorting:
             self.sort_fillers()  # at set-up; `run` sorts again every `interval` iterations
+
+    def apply_net_weighting(self):
+        """One timing-driven net-weight update: the timer's report into the
+        device weights, then the pin weights of the preconditioner and the
+        objective's per-slot weights from them."""
+        dd = self.dd
+        rep = self.timer(self)
+        w = dd.net_weights
+        self.net_weighting.update(w, net_slack=rep.get("net_slack"), wns=rep.get("wns"),
+                                  net_critical=rep.get("net_critical"))
+        N.call("dp_pin_weight_sum", self.code, N.ptr(w), N.ptr(dd.flat_node2pin_map),
+               N.ptr(dd.flat_node2pin_start_map), N.ptr(dd.pin2net_map), dd.num_physical_nodes,
+               dd.num_nodes, N.ptr(self.pws), N.stream_ptr(dd.device))
+        self.objective.set_net_weights(w)
+        self.net_weight_updates += 1
 
     def positions(self):
         """The current positions with the fillers in the design's order."""
@@ -430,6 +470,9 @@ class GlobalPlacer:
                          cur["gamma"]), flush=True)
             self.optimizer.step()
             constrained = True
+            if (self.net_weighting is not None and it > p["timing_start"]
+                    and it % p["timing_interval"] == 0):
+                self.apply_net_weighting()
             if self.best is None or self.best["overflow"] > cur["overflow"]:
                 self.best = cur
                 self.best_positions.improved(x)

@@ -355,6 +355,24 @@ int dp_wa_net_weight_grad(int dtype, const void* grad, const void* partial_wl,
                           const uint8_t* net_mask, int64_t num_nets, void* grad_net_weights,
                           void* stream);
 
+/* Timing-driven net weighting: replaces the weighting step of
+ * timing_cpp.update_net_weights (timing.py:182-206 ->
+ * net_weighting_scheme.h:106-190) on the device, given what a static timer
+ * reports (the timer itself, OpenTimer / HeteroSTA in the reference, is not
+ * part of this library).  scheme DP_NET_WEIGHTING_ADAMS (the reference's 0):
+ * net_critical[e] != 0 marks nets on the reported critical paths;
+ * DP_NET_WEIGHTING_LILITH (1): net_slack[e] is the net's worst sink slack and
+ * wns the design's worst negative slack.  net_criticality and net_weights
+ * (`dtype`, per net) are updated in place; nets of degree
+ * (netpin_start[e+1] - netpin_start[e]) above ignore_net_degree keep their
+ * weight.  max_net_weight caps LILITH's weights (+inf: no cap); decay is the
+ * momentum decay factor (params.json: 0.5). */
+enum { DP_NET_WEIGHTING_ADAMS = 0, DP_NET_WEIGHTING_LILITH = 1 };
+int dp_net_weighting(int dtype, int scheme, int64_t num_nets, const void* net_slack,
+                     const uint8_t* net_critical, double wns, const int32_t* netpin_start,
+                     int ignore_net_degree, double decay, double max_net_weight,
+                     void* net_criticality, void* net_weights, void* stream);
+
 /* Fused forward of the wirelength op chain WA(PinPos(pos))
  * (dreamplace/PlaceObj.py:478-479): dp_pin_pos_forward followed by
  * dp_wa_merged_forward, with each pin position formed on the fly from

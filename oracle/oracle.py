@@ -606,6 +606,38 @@ def energy_gradient(dp, pos, potential, grad=1.0):
     return out
 
 
+def net_weighting(scheme, net_weights, net_criticality, netpin_start, ignore_net_degree,
+                  net_slack=None, wns=None, net_critical=None, decay=0.5, max_net_weight=np.inf):
+    """The weighting step of timing_cpp.update_net_weights, in the arrays'
+    dtype: "adams" (net_weighting_scheme.h:106-148) halves each net's
+    criticality and adds 0.5 for nets on a reported critical path, then
+    multiplies the weight by 1 + criticality; "lilith" (:150-188) blends the
+    criticality with nc = max(0, slack/wns) of negative-slack nets as
+    (1+c)^decay * (1+nc)^(1-decay) - 1 when wns < 0, then multiplies the
+    weight by 1 + criticality and caps it at max_net_weight.  Nets of degree
+    above ignore_net_degree keep their weight (and, for adams, their
+    criticality).  Returns (weights, criticality).  Pinned by hand-derived
+    updates (tests/test_oracle.py)."""
+    T = np.asarray(net_weights).dtype.type
+    w = np.array(net_weights, copy=True)
+    c = np.array(net_criticality, dtype=w.dtype, copy=True)
+    keep = np.diff(np.asarray(netpin_start, dtype=np.int64)) <= ignore_net_degree
+    if scheme == "adams":
+        flag = np.asarray(net_critical) != 0
+        c[keep] = c[keep] * T(0.5) + np.where(flag[keep], T(0.5), T(0))
+        w[keep] = w[keep] * (T(1) + c[keep])
+        return w, c
+    if scheme != "lilith":
+        raise ValueError("scheme must be adams or lilith")
+    if wns < 0:
+        sl = np.asarray(net_slack, dtype=w.dtype)
+        nc = np.where(sl < 0, np.maximum(T(0), sl / T(wns)), T(0)).astype(w.dtype)
+        c = (np.power(T(1) + c, T(decay)) * np.power(T(1) + nc, T(1) - T(decay)) - T(1)).astype(w.dtype)
+    nw = w[keep] * (T(1) + c[keep])
+    w[keep] = np.where(nw > T(max_net_weight), T(max_net_weight), nw)
+    return w, c
+
+
 def overflow(density_map_movable, target_density, bin_size_x, bin_size_y, total_movable_area):
     """ElectricOverflow.forward metric (electric_overflow.py:260-278,
     EvalMetrics.py:116-128): sum(max(D - t*A_bin, 0)) / movable area, and

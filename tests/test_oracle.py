@@ -898,3 +898,27 @@ def test_reference_ops_drive_the_same_loop(oracle, ref):
         for k in ("hpwl", "overflow", "density_weight", "gamma"):
             assert a[k] == pytest.approx(b[k], rel=1e-8), k
     assert rel_l2(xr, xo) <= 1e-8
+
+
+def test_net_weighting_known_answers(oracle):
+    """ADAMS and LILITH updates (net_weighting_scheme.h:106-188) derived by
+    hand: nets above ignore_net_degree keep their weights; LILITH with decay
+    0.5 gives c' = sqrt((1+c)(1+nc)) - 1, nc = max(0, slack/wns) for negative
+    slacks; weights are capped at max_net_weight; without a negative WNS the
+    criticality is kept and the weights still grow by 1 + c."""
+    start = np.array([0, 2, 5, 205, 207])  # degrees 2, 3, 200, 2
+    w, c = oracle.net_weighting("adams", np.array([1.0, 2.0, 3.0, 4.0]),
+                                np.array([0.0, 0.5, 1.0, 0.25]), start, 100,
+                                net_critical=np.array([1, 0, 1, 0], dtype=np.uint8))
+    assert w.tolist() == [1.5, 2.5, 3.0, 4.5] and c.tolist() == [0.5, 0.25, 1.0, 0.125]
+    w, c = oracle.net_weighting("lilith", np.array([1.0, 2.0, 3.0, 1.0]),
+                                np.array([0.0, 0.0, 0.44, 0.0]), start, 100,
+                                net_slack=np.array([-1.0, 0.5, -2.0, -4.0]), wns=-2.0,
+                                decay=0.5, max_net_weight=1.5)
+    ce = [math.sqrt(1.5) - 1, 0.0, math.sqrt(1.44 * 2.0) - 1, math.sqrt(3.0) - 1]
+    assert np.allclose(c, ce, rtol=1e-15, atol=1e-15)
+    # net 1 keeps c = 0 but its weight 2 is above the cap; net 2 is above the degree limit
+    assert np.allclose(w, [1 + ce[0], 1.5, 3.0, 1.5], rtol=1e-15)
+    w, c = oracle.net_weighting("lilith", np.array([1.0, 2.0]), np.array([0.5, 0.0]),
+                                np.array([0, 2, 4]), 100, net_slack=np.array([-1.0, 1.0]), wns=0.0)
+    assert c.tolist() == [0.5, 0.0] and w.tolist() == [1.5, 2.0]
