@@ -83,7 +83,8 @@ _SIGNATURES = {
                                  _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_pins_slot_weights": (_I, [_I, _L, _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
-    "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P]),
+    "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P,
+                                         _P, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_hpwl_from_nets": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
@@ -91,7 +92,7 @@ _SIGNATURES = {
     "dp_optimizer_workspace_bytes": (_S, []),
     "dp_nesterov_extrapolate": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dp_nesterov_extrapolate_bounded": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D,
-                                             _D, _D, _D, _L, _L, _P, _P]),
+                                             _D, _D, _D, _L, _L, _P, _P, _P, _P]),
     "dp_step_norms": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P]),
     "dp_l1_norm": (_I, [_I, _L, _P, _P, _P, _P]),
     "dp_precondition": (_I, [_I, _P, _L, _P, _P, _P, _D, _P]),

@@ -205,6 +205,15 @@ inline void zero_words(unsigned long long* p, int64_t n, hipStream_t s) {
 // Two-stage deterministic reduction helpers (reduce.hip).
 constexpr int kReduceBlock = 256;
 constexpr int kReduceMaxBlocks = 16384;
+// Workgroups of the reductions that pair the x and y entries of a [2N]
+// per-node vector (the step-size norms, fused or not): thread t of the
+// grid-stride loop takes nodes t, t + grid, ...; every such reduction uses
+// this grid, so their sums are bit-identical.
+inline int node_reduce_blocks(int64_t num_nodes) {
+  int64_t b = (num_nodes + kReduceBlock - 1) / kReduceBlock;
+  if (b < 1) b = 1;
+  return (int)(b > kReduceMaxBlocks ? kReduceMaxBlocks : b);
+}
 // Bytes of workspace used by one two-stage reduction slot.
 inline size_t reduce_slot_bytes() { return align256(sizeof(double) * kReduceMaxBlocks * 2); }
 // Final stage: out[k] (k < nout) = sum (or max when is_max[k]) of

@@ -53,43 +53,68 @@ struct Box {
 
 // k_nesterov_extrapolate followed by move_boundary on v_kp1, in one pass:
 // v_kp1 is formed as above, then clamped to the box of its node.  Entries
-// of fixed nodes are left as formed, as move_boundary leaves them.
+// of fixed nodes are left as formed, as move_boundary leaves them.  Thread
+// t takes nodes t, t + grid, ... (x and y); with `partials` it also sums
+// (v_kp1 - v_k)^2 in FP64 in the order k_step_norms does (the step-size
+// numerator of step_nobb, NesterovAcceleratedGradientOptimizer.py:131).
 template <typename T>
-__global__ void k_nesterov_extrapolate_box(int64_t n, const T* __restrict__ v_k,
-                                           const T* __restrict__ g_k, const T* __restrict__ u_k,
-                                           const T* __restrict__ alpha_p,
-                                           const T* __restrict__ a_p, T* u_kp1, T* v_kp1,
-                                           T* a_kp1_out, Box<T> box) {
+__device__ __forceinline__ T box_clamp(const Box<T>& box, T v, int64_t j, bool is_y) {
+  const int64_t fill_begin = box.num_nodes - box.num_filler;
+  if (j < box.num_movable || j >= fill_begin) {
+    const T lo = is_y ? box.yl : box.xl;
+    T size;
+    if (box.uniform_fillers && j >= fill_begin)
+      size = is_y ? box.filler_sy : box.filler_sx;
+    else
+      size = is_y ? box.sy[j] : box.sx[j];
+    const T hi = (is_y ? box.yh : box.xh) - size;
+    v = v < lo ? lo : v;
+    v = hi < v ? hi : v;
+  }
+  return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kReduceBlock) k_nesterov_extrapolate_box(
+    const T* __restrict__ v_k, const T* __restrict__ g_k, const T* __restrict__ u_k,
+    const T* __restrict__ alpha_p, const T* __restrict__ a_p, T* u_kp1, T* v_kp1, T* a_kp1_out,
+    Box<T> box, double* partials) {
+  __shared__ double smem[kReduceBlock / 64];
   const T a = *a_p;
   const T a_kp1 = (T(1) + sqrt(T(4) * (a * a) + T(1))) / T(2);
   const T coef = (a - T(1)) / a_kp1;
   const T alpha = *alpha_p;
-  const int64_t N = box.num_nodes, fill_begin = N - box.num_filler;
-  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i0 == 0 && a_kp1_out) *a_kp1_out = a_kp1;
-  for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const T u = v_k[i] - alpha * g_k[i];
-    const T uk = u_k[i];
-    u_kp1[i] = u;
-    T v = u + coef * (u - uk);
-    const bool is_y = i >= N;
-    const int64_t j = is_y ? i - N : i;
-    if (j < box.num_movable || j >= fill_begin) {
-      const T lo = is_y ? box.yl : box.xl;
-      T size;
-      if (box.uniform_fillers && j >= fill_begin)
-        size = is_y ? box.filler_sy : box.filler_sx;
-      else
-        size = is_y ? box.sy[j] : box.sx[j];
-      const T hi = (is_y ? box.yh : box.xh) - size;
-      v = v < lo ? lo : v;
-      v = hi < v ? hi : v;
+  const int64_t N = box.num_nodes;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 == 0 && a_kp1_out) *a_kp1_out = a_kp1;
+  double svv = 0;
+  for (int64_t j = t0; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis) {
+      const int64_t i = axis ? N + j : j;
+      const T vk = v_k[i];
+      const T u = vk - alpha * g_k[i];
+      const T uk = u_k[i];
+      u_kp1[i] = u;
+      const T v = box_clamp(box, u + coef * (u - uk), j, axis == 1);
+      v_kp1[i] = v;
+      if (partials) {
+        const double dv = (double)(v - vk);
+        svv += dv * dv;
+      }
     }
-    v_kp1[i] = v;
+  }
+  if (partials) {
+    const double r = block_sum<kReduceBlock>(svv, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
   }
 }
 
-// partials of sum(da^2), sum(db^2), sum(da*db), da = a1 - a0, db = b1 - b0
+// partials of sum(da^2), sum(db^2), sum(da*db), da = a1 - a0, db = b1 - b0:
+// the n entries taken as n/2 (x, y) pairs (i, n/2 + i) over the grid of
+// node_reduce_blocks, thread t taking pairs t, t + grid, ... (the order the
+// fused sums of the extrapolation and the node-gradient pass follow), and
+// an odd last entry by thread 0 at the end.
 template <typename T>
 __global__ void __launch_bounds__(kReduceBlock) k_step_norms(int64_t n, const T* __restrict__ a1,
                                                              const T* __restrict__ a0,
@@ -98,14 +123,20 @@ __global__ void __launch_bounds__(kReduceBlock) k_step_norms(int64_t n, const T*
                                                              double* partials) {
   __shared__ double smem[kReduceBlock / 64];
   double saa = 0, sbb = 0, sab = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t half = n / 2;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  auto add = [&](int64_t i) {
     const double da = (double)(a1[i] - a0[i]);
     const double db = (double)(b1[i] - b0[i]);
     saa += da * da;
     sbb += db * db;
     sab += da * db;
+  };
+  for (int64_t j = t0; j < half; j += (int64_t)gridDim.x * blockDim.x) {
+    add(j);
+    add(half + j);
   }
+  if ((n & 1) && t0 == 0) add(n - 1);
   double r = block_sum<kReduceBlock>(saa, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
   r = block_sum<kReduceBlock>(sbb, smem);
@@ -180,15 +211,17 @@ int dp_nesterov_extrapolate_bounded(int dtype, int64_t num_nodes, const void* v_
                                     const void* node_size_x, const void* node_size_y, double xl,
                                     double yl, double xh, double yh, int64_t num_movable_nodes,
                                     int64_t num_filler_nodes, const double* filler_size,
-                                    void* stream) {
+                                    void* workspace, double* dv_norm2_out, void* stream) {
   DP_ARG(v_k && g_k && u_k && alpha_k && a_k && u_kp1 && v_kp1 && node_size_x && node_size_y,
          "dp_nesterov_extrapolate_bounded: null pointer");
+  DP_ARG(!dv_norm2_out || workspace, "dp_nesterov_extrapolate_bounded: the norm needs a workspace");
   DP_ARG(num_nodes >= 0 && num_movable_nodes >= 0 && num_filler_nodes >= 0 &&
              num_movable_nodes + num_filler_nodes <= num_nodes,
          "dp_nesterov_extrapolate_bounded: bad node counts");
   DP_ARG(a_kp1 != a_k, "dp_nesterov_extrapolate_bounded: a_kp1 must not alias a_k");
-  const int64_t n = 2 * num_nodes;
   hipStream_t s = S(stream);
+  const int nb = node_reduce_blocks(num_nodes);
+  double* part = dv_norm2_out ? (double*)workspace : nullptr;
   return DP_DISPATCH(dtype, "dp_nesterov_extrapolate_bounded", [&] {
     Box<T> box;
     box.sx = (const T*)node_size_x;
@@ -203,10 +236,11 @@ int dp_nesterov_extrapolate_bounded(int dtype, int64_t num_nodes, const void* v_
     box.num_nodes = num_nodes;
     box.num_movable = num_movable_nodes;
     box.num_filler = num_filler_nodes;
-    hipLaunchKernelGGL(k_nesterov_extrapolate_box<T>, dim3(grid_for(n > 0 ? n : 1, 256)),
-                       dim3(256), 0, s, n, (const T*)v_k, (const T*)g_k, (const T*)u_k,
-                       (const T*)alpha_k, (const T*)a_k, (T*)u_kp1, (T*)v_kp1, (T*)a_kp1, box);
+    hipLaunchKernelGGL(k_nesterov_extrapolate_box<T>, dim3(nb), dim3(kReduceBlock), 0, s,
+                       (const T*)v_k, (const T*)g_k, (const T*)u_k, (const T*)alpha_k,
+                       (const T*)a_k, (T*)u_kp1, (T*)v_kp1, (T*)a_kp1, box, part);
     DP_LAUNCHED("k_nesterov_extrapolate_box");
+    if (part) return finish_reduction(part, nb, 1, nullptr, dv_norm2_out, -1, s);
     return DP_OK;
   });
 }
@@ -217,7 +251,7 @@ int dp_step_norms(int dtype, int64_t n, const void* a1, const void* a0, const vo
   DP_ARG(n >= 0, "dp_step_norms: bad size");
   hipStream_t s = S(stream);
   double* part = (double*)workspace;
-  const int nb = reduce_blocks(n);
+  const int nb = node_reduce_blocks(n / 2);
   return DP_DISPATCH(dtype, "dp_step_norms", [&] {
     hipLaunchKernelGGL(k_step_norms<T>, dim3(nb), dim3(kReduceBlock), 0, s, n, (const T*)a1,
                        (const T*)a0, (const T*)b1, (const T*)b0, part);

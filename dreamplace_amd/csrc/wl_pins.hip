@@ -450,42 +450,58 @@ __global__ void __launch_bounds__(256) k_wa_big(
 // (add nullable, split or interleaved).  Fillers (n >= nphys) get add[n] only.
 // kPre: then divided by max(pws[n] + (alpha * *dw) * area[n], 1), the
 // preconditioner of PlaceObj.py:69-102 applied to the finished gradient.
+// Thread t takes nodes t, t + grid, ...; with `partials` it also sums
+// (grad - g_prev)^2 over x and y in FP64, in k_step_norms's order (the
+// step-size denominator of step_nobb, NesterovAcceleratedGradientOptimizer.py:131).
 template <typename T, bool kPre>
-__global__ void k_node_grad(const typename Vec2<T>::type* __restrict__ gnode,
-                            const int* __restrict__ node2pin_start, const T* __restrict__ grad,
-                            const T* __restrict__ add, int add_interleaved, int64_t nphys,
-                            int64_t nnodes, T* gx_out, T* gy_out, const T* __restrict__ pws,
-                            const T* __restrict__ area, const T* __restrict__ dw, T alpha) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nnodes) return;
-  T sx = 0, sy = 0;
-  if (n < nphys) {
-    const T g = *grad;
-    for (int j = node2pin_start[n]; j < node2pin_start[n + 1]; ++j) {
-      const typename Vec2<T>::type v = gnode[j];
-      sx += v.x * g;
-      sy += v.y * g;
+__global__ void __launch_bounds__(256) k_node_grad(
+    const typename Vec2<T>::type* __restrict__ gnode, const int* __restrict__ node2pin_start,
+    const T* __restrict__ grad, const T* __restrict__ add, int add_interleaved, int64_t nphys,
+    int64_t nnodes, T* gx_out, T* gy_out, const T* __restrict__ pws, const T* __restrict__ area,
+    const T* __restrict__ dw, T alpha, const T* __restrict__ g_prev, double* partials) {
+  __shared__ double smem[4];
+  double sgg = 0;
+  const T g = *grad;
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nnodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    T sx = 0, sy = 0;
+    if (n < nphys) {
+      for (int j = node2pin_start[n]; j < node2pin_start[n + 1]; ++j) {
+        const typename Vec2<T>::type v = gnode[j];
+        sx += v.x * g;
+        sy += v.y * g;
+      }
+    }
+    if (add) {
+      if (add_interleaved) {
+        const typename Vec2<T>::type v = reinterpret_cast<const typename Vec2<T>::type*>(add)[n];
+        sx += v.x;
+        sy += v.y;
+      } else {
+        sx += add[n];
+        sy += add[nnodes + n];
+      }
+    }
+    if (kPre) {
+      const T c = alpha * *dw;
+      T p = pws[n] + c * area[n];
+      p = p < T(1) ? T(1) : p;
+      sx = sx / p;
+      sy = sy / p;
+    }
+    gx_out[n] = sx;
+    gy_out[n] = sy;
+    if (partials) {
+      const double dx = (double)(sx - g_prev[n]);
+      sgg += dx * dx;
+      const double dy = (double)(sy - g_prev[nnodes + n]);
+      sgg += dy * dy;
     }
   }
-  if (add) {
-    if (add_interleaved) {
-      const typename Vec2<T>::type v = reinterpret_cast<const typename Vec2<T>::type*>(add)[n];
-      sx += v.x;
-      sy += v.y;
-    } else {
-      sx += add[n];
-      sy += add[nnodes + n];
-    }
+  if (partials) {
+    const double r = block_sum<256>(sgg, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
   }
-  if (kPre) {
-    const T c = alpha * *dw;
-    T p = pws[n] + c * area[n];
-    p = p < T(1) ? T(1) : p;
-    sx = sx / p;
-    sy = sy / p;
-  }
-  gx_out[n] = sx;
-  gy_out[n] = sy;
 }
 
 static int grid_cap(int64_t items, int per_block, int cap) {
@@ -712,9 +728,13 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
                      const int32_t* flat_node2pin_start_map, const void* add, int add_layout,
                      int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
                      const void* pws, const void* area, const void* dw, double alpha,
+                     const void* grad_prev, void* workspace, double* dg_norm2_out,
                      void* stream) {
   DP_ARG(grad && grad_node_order && flat_node2pin_start_map && grad_pos_out,
          "%s: null pointer", name);
+  DP_ARG(!dg_norm2_out || (grad_prev && workspace), "%s: the norm needs grad_prev and a workspace",
+         name);
+  DP_ARG(!dg_norm2_out || grad_prev != grad_pos_out, "%s: grad_prev must not be the output", name);
   DP_ARG(num_physical_nodes <= num_nodes, "%s: bad node counts", name);
   DP_ARG(add_layout == DP_LAYOUT_SPLIT || add_layout == DP_LAYOUT_INTERLEAVED,
          "%s: bad add_layout %d", name, add_layout);
@@ -725,14 +745,17 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
     typedef typename Vec2<T>::type V;
     T* out = (T*)grad_pos_out;
     auto kern = pws ? k_node_grad<T, true> : k_node_grad<T, false>;
+    const int nb = node_reduce_blocks(num_nodes);
+    double* part = dg_norm2_out ? (double*)workspace : nullptr;
     DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
-              hipLaunchKernelGGL(kern, dim3((unsigned)ceil_div(num_nodes, 256)), dim3(256), 0, s,
-                                 (const V*)grad_node_order, (const int*)flat_node2pin_start_map,
-                                 (const T*)grad, (const T*)add,
-                                 add_layout == DP_LAYOUT_INTERLEAVED, num_physical_nodes,
-                                 num_nodes, out, out + num_nodes, (const T*)pws, (const T*)area,
-                                 (const T*)dw, (T)alpha));
+              hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, (const V*)grad_node_order,
+                                 (const int*)flat_node2pin_start_map, (const T*)grad,
+                                 (const T*)add, add_layout == DP_LAYOUT_INTERLEAVED,
+                                 num_physical_nodes, num_nodes, out, out + num_nodes,
+                                 (const T*)pws, (const T*)area, (const T*)dw, (T)alpha,
+                                 (const T*)grad_prev, part));
     DP_LAUNCHED("k_node_grad");
+    if (part) return finish_reduction(part, nb, 1, nullptr, dg_norm2_out, -1, s);
     return DP_OK;
   });
 }
@@ -743,7 +766,7 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
                            void* grad_pos_out, void* stream) {
   return node_grad("dp_node_grad_from_pins", dtype, grad, grad_node_order, flat_node2pin_start_map,
                    add, add_layout, num_physical_nodes, num_nodes, grad_pos_out, nullptr, nullptr,
-                   nullptr, 0.0, stream);
+                   nullptr, 0.0, nullptr, nullptr, nullptr, stream);
 }
 
 int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
@@ -751,12 +774,14 @@ int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_no
                                 int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                                 const void* pin_weight_sum, const void* node_areas,
                                 const void* density_weight, double alpha, void* grad_pos_out,
+                                const void* grad_prev, void* workspace, double* dg_norm2_out,
                                 void* stream) {
   DP_ARG(pin_weight_sum && node_areas && density_weight,
          "dp_node_grad_preconditioned: null pointer");
   return node_grad("dp_node_grad_preconditioned", dtype, grad, grad_node_order,
                    flat_node2pin_start_map, add, add_layout, num_physical_nodes, num_nodes,
-                   grad_pos_out, pin_weight_sum, node_areas, density_weight, alpha, stream);
+                   grad_pos_out, pin_weight_sum, node_areas, density_weight, alpha, grad_prev,
+                   workspace, dg_norm2_out, stream);
 }
 
 }  // extern "C"

@@ -426,6 +426,9 @@ class PlacementObjective:
         self.fork_at = "scatter"
         self.set_overlap(overlap)
         self.filler_ranges = None
+        self._dg2 = None
+        self._norm_ws = None
+        self.last_dg2 = None
         self.track_metrics = False
         self.metric_ptr = None
         if track_metrics:
@@ -611,7 +614,7 @@ class PlacementObjective:
                self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
                N.ptr(self._density_grad), N.stream_ptr(pos.device))
 
-    def evaluate(self, pos, precondition=None, grad_out=None):
+    def evaluate(self, pos, precondition=None, grad_out=None, grad_ref=None):
         """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers.
         precondition = (pin_weight_sum, node_areas, alpha) also applies
         PlaceObj's preconditioner to the gradient (with the current density
@@ -619,8 +622,13 @@ class PlacementObjective:
         grad_out: a contiguous [2N] tensor of the positions' type and device
         that receives the gradient (and is returned) instead of the
         objective's own buffer, so a caller that keeps gradients from step to
-        step needs no copy."""
+        step needs no copy.
+        grad_ref: a [2N] gradient (not the output buffer); where the
+        preconditioned node-gradient pass runs (the pins schedule) it also
+        forms sum((grad - grad_ref)^2) in FP64 and leaves it in `last_dg2`
+        (a device double[1]), else `last_dg2` is None."""
         grad = self.grad
+        self.last_dg2 = None
         if grad_out is not None:
             if (grad_out.dtype != self.grad.dtype or grad_out.device != self.grad.device
                     or grad_out.numel() != self.grad.numel() or not grad_out.is_contiguous()):
@@ -673,10 +681,19 @@ class PlacementObjective:
                 self._reduce_metrics(pos, s)
             if precondition is not None:
                 pws, areas, alpha = precondition
+                fuse = grad_ref is not None and grad_ref.data_ptr() != grad.data_ptr()
+                if fuse and self._dg2 is None:
+                    self._dg2 = torch.empty(1, dtype=torch.float64, device=pos.device)
+                    self._norm_ws = torch.empty(N.query("dp_optimizer_workspace_bytes"),
+                                                dtype=torch.uint8, device=pos.device)
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),
                        N.ptr(pp.grad_node_order), N.ptr(dd.flat_node2pin_start_map),
                        N.ptr(self._density_grad), self.grad_layout, dd.num_physical_nodes, n,
-                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(grad), s)
+                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(grad),
+                       N.ptr(grad_ref) if fuse else None, N.ptr(self._norm_ws) if fuse else None,
+                       N.ptr(self._dg2) if fuse else None, s)
+                if fuse:
+                    self.last_dg2 = self._dg2
                 precondition = None  # done
             else:
                 N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),

@@ -31,7 +31,7 @@ class _Norms:
 
 class NesterovAcceleratedGradientOptimizer(Optimizer):
     def __init__(self, params, lr=required, obj_and_grad_fn=required, constraint_fn=None,
-                 use_bb=True, bounds=None, grad_into=False):
+                 use_bb=True, bounds=None, grad_into=False, fused_norms=False):
         """bounds (new, optional): the move_boundary box as a dict with
         node_size_x, node_size_y (device), xl, yl, xh, yh, num_movable_nodes,
         num_filler_nodes and filler_size ((w, h) shared by all fillers, or
@@ -41,7 +41,12 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         grad_into (new, optional): obj_and_grad_fn also takes grad_out=, a
         buffer that receives the gradient.  step_nobb then evaluates into the
         storage of g_k_1 (which the line search no longer reads) and keeps
-        it as g_k, with no copy of the gradient."""
+        it as g_k, with no copy of the gradient.
+        fused_norms (new, optional; with bounds and grad_into): step_nobb
+        takes sum((v_kp1 - v_k)^2) from the extrapolation pass and asks
+        obj_and_grad_fn(x, grad_out=, grad_ref=g_k) for sum((g_kp1 - g_k)^2)
+        as a third result (a device FP64 scalar, or None to fall back to a
+        pass of its own); the same bits as dp_step_norms."""
         if lr is not required and lr < 0.0:
             raise ValueError("the learning rate must be non-negative, got {}".format(lr))
         defaults = dict(lr=lr, u_k=[], v_k=[], g_k=[], obj_k=[], a_k=[], alpha_k=[], v_k_1=[],
@@ -52,6 +57,7 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         self.use_bb = use_bb
         self.bounds = bounds
         self.grad_into = grad_into
+        self.fused_norms = bool(fused_norms) and bounds is not None and grad_into
         if len(self.param_groups) != 1 or len(self.param_groups[0]["params"]) != 1:
             raise ValueError("the optimizer takes exactly one parameter tensor (the [2N] positions)")
         self._norms = None
@@ -68,12 +74,13 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
             self._norms = _Norms(p.device)
             self._u_kp1 = torch.empty_like(p.data)
             self._a_kp1 = torch.empty(1, dtype=p.dtype, device=p.device)
+            self._dv2 = torch.empty(1, dtype=torch.float64, device=p.device)
 
     def _constrain(self, x):
         if self.constraint_fn is not None and self.bounds is None:
             self.constraint_fn(x)
 
-    def _extrapolate(self, v_k, g_k, u_k, alpha, a_k, v_kp1):
+    def _extrapolate(self, v_k, g_k, u_k, alpha, a_k, v_kp1, dv2=None):
         b = self.bounds
         if b is None:
             N.call("dp_nesterov_extrapolate", N.dtype_code(v_k), v_k.numel(), N.ptr(v_k),
@@ -86,7 +93,9 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
                N.ptr(v_kp1), N.ptr(self._a_kp1), N.ptr(b["node_size_x"]), N.ptr(b["node_size_y"]),
                float(b["xl"]), float(b["yl"]), float(b["xh"]), float(b["yh"]),
                int(b["num_movable_nodes"]), int(b["num_filler_nodes"]),
-               N.host_doubles(fs) if fs is not None else None, N.stream_ptr(v_k.device))
+               N.host_doubles(fs) if fs is not None else None,
+               N.ptr(self._norms.ws) if dv2 is not None else None, N.ptr(dv2),
+               N.stream_ptr(v_k.device))
 
     def step_nobb(self, closure=None):
         """Lines 65-166: Lipschitz step size with backtracking (at most 10
@@ -120,14 +129,21 @@ class NesterovAcceleratedGradientOptimizer(Optimizer):
         v_kp1 = group["v_kp1"][0]
         cnt = 0
         while True:
-            self._extrapolate(v_k.data, g_k, u_k, alpha_k, a_k, v_kp1)
+            dv2 = self._dv2 if self.fused_norms else None
+            self._extrapolate(v_k.data, g_k, u_k, alpha_k, a_k, v_kp1, dv2)
             self._constrain(v_kp1)
-            if self.grad_into:
+            dg2 = None
+            if self.fused_norms:
+                f_kp1, g_kp1, dg2 = f(v_kp1, grad_out=g_k_1, grad_ref=g_k)
+            elif self.grad_into:
                 f_kp1, g_kp1 = f(v_kp1, grad_out=g_k_1)
             else:
                 f_kp1, g_kp1 = f(v_kp1)
-            s = self._norms(v_kp1, v_k.data, g_kp1, g_k)
-            alpha_kp1 = (s[0] / s[1]).sqrt().to(g_k.dtype).reshape(1)
+            if dg2 is not None:
+                alpha_kp1 = (dv2 / dg2).sqrt().to(g_k.dtype).reshape(1)
+            else:
+                s = self._norms(v_kp1, v_k.data, g_kp1, g_k)
+                alpha_kp1 = (s[0] / s[1]).sqrt().to(g_k.dtype).reshape(1)
             cnt += 1
             group["obj_eval_count"] += 1
             done = bool((alpha_kp1 > 0.95 * alpha_k).item()) or cnt >= 10

@@ -32,7 +32,7 @@ DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate
                 ref_hpwl=350000.0, lower_pcof=0.95, upper_pcof=1.05, noise_ratio=0.025,
                 use_bb=False, precondition_alpha=1.0, quad_penalty=False, density_quad_coeff=2000.0,
                 density_factor=1.0, early_stop=True, filler_sort_interval=25, movable_order="netlist",
-                fuse_move_boundary=True, fused_metrics=True, timing_opt=False,
+                fuse_move_boundary=True, fused_metrics=True, fused_norms=True, timing_opt=False,
                 net_weighting_scheme="lilith", momentum_decay_factor=0.5, max_net_weight=math.inf,
                 timing_start=500, timing_interval=15)
 
@@ -173,6 +173,9 @@ class GlobalPlacer:
     fuse_move_boundary: the optimizer clamps its extrapolated positions to
     the die in the extrapolation pass instead of a move_boundary pass after
     it (the same bits).
+    fused_norms: the step sizes' FP64 norms sum((v_kp1 - v_k)^2) and
+    sum((g_kp1 - g_k)^2) are formed in the extrapolation and node-gradient
+    passes instead of a pass of their own (the same bits).
     fused_metrics: HPWL and overflow at positions the objective has just
     evaluated come from that evaluation (PlacementObjective track_metrics)
     instead of a pin-position pass, an HPWL pass and a second scatter (the
@@ -330,17 +333,22 @@ class GlobalPlacer:
             c = float(T(self.p["density_quad_coeff"] / 2) * T(inv))
         self.objective.set_density_penalty(self.density_factor, c)
 
-    def obj_and_grad(self, x, grad_out=None):
+    def obj_and_grad(self, x, grad_out=None, grad_ref=None):
         """PlaceObj.obj_and_grad_fn: objective, gradient, preconditioner
-        (the gradient into grad_out when given)."""
+        (the gradient into grad_out when given).  With grad_ref it returns
+        (objective, gradient, sum((gradient - grad_ref)^2) as a device FP64
+        scalar formed in the gradient's own pass, or None where it is not)."""
         pre = (self.pws, self.areas, self.p["precondition_alpha"])
-        f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out)
+        f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out, grad_ref=grad_ref)
         if self.init_density is None:
             # PlaceObj.obj_fn records the density of its first evaluation
             self.init_density = float(self.objective._energy(x).item())
             if self.quad_penalty:
                 self._penalty()
-                f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out)
+                f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out,
+                                               grad_ref=grad_ref)
+        if grad_ref is not None:
+            return f, g, self.objective.last_dg2
         return f, g
 
     def initialize_density_weight(self, x):
@@ -434,7 +442,7 @@ class GlobalPlacer:
         self.optimizer = NesterovAcceleratedGradientOptimizer(
             [x], lr=lr, obj_and_grad_fn=self.obj_and_grad, constraint_fn=self.constraint,
             use_bb=p["use_bb"], bounds=self.boundary_box() if p["fuse_move_boundary"] else None,
-            grad_into=True)
+            grad_into=True, fused_norms=p["fused_norms"])
         # a best kept from an earlier run lives in storage of that run's optimizer
         self.best_positions.materialize(x)
         self.best_positions.opt = self.optimizer

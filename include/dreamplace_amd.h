@@ -491,12 +491,17 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
  * regions (PlaceObj.py:69-102) in the same pass: each node's finished
  * gradient is divided by max(pin_weight_sum[n] + (alpha * *density_weight) *
  * node_areas[n], 1), as dp_precondition does; density_weight is a device
- * scalar of `dtype`. */
+ * scalar of `dtype`.  dg_norm2_out (device double, nullable; then grad_prev,
+ * a [2N] gradient not aliasing the output, and a dp_optimizer_workspace_bytes
+ * workspace are needed): sum((grad_pos_out - grad_prev)^2), the Lipschitz
+ * step's denominator (NesterovAcceleratedGradientOptimizer.py:131), formed in
+ * the same pass with the bits dp_step_norms gives. */
 int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
                                 const int32_t* flat_node2pin_start_map, const void* add,
                                 int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                                 const void* pin_weight_sum, const void* node_areas,
                                 const void* density_weight, double alpha, void* grad_pos_out,
+                                const void* grad_prev, void* workspace, double* dg_norm2_out,
                                 void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
@@ -554,17 +559,22 @@ int dp_nesterov_extrapolate(int dtype, int64_t n, const void* v_k, const void* g
  * and filler nodes are clamped to x in [xl, xh - node_size_x],
  * y in [yl, yh - node_size_y].  The same bits as the two calls.
  * filler_size (host double[2], nullable): the one size all fillers share,
- * read instead of their node_size entries. */
+ * read instead of their node_size entries.  dv_norm2_out (device double,
+ * nullable; then a dp_optimizer_workspace_bytes workspace is needed):
+ * sum((v_kp1 - v_k)^2), the Lipschitz step's numerator (:131), formed in the
+ * same pass with the bits dp_step_norms gives. */
 int dp_nesterov_extrapolate_bounded(int dtype, int64_t num_nodes, const void* v_k, const void* g_k,
                                     const void* u_k, const void* alpha_k, const void* a_k,
                                     void* u_kp1, void* v_kp1, void* a_kp1,
                                     const void* node_size_x, const void* node_size_y, double xl,
                                     double yl, double xh, double yh, int64_t num_movable_nodes,
                                     int64_t num_filler_nodes, const double* filler_size,
-                                    void* stream);
+                                    void* workspace, double* dv_norm2_out, void* stream);
 
 /* out3 (device double[3]) = {sum((a1-a0)^2), sum((b1-b0)^2), sum((a1-a0)(b1-b0))}:
- * differences in `dtype`, squares and sums in FP64, deterministic.  With
+ * differences in `dtype`, squares and sums in FP64, deterministic (the n
+ * entries are summed as n/2 (x, y) pairs, in the order the fused sums of
+ * dp_nesterov_extrapolate_bounded and dp_node_grad_preconditioned follow).  With
  * a = positions and b = gradients these give the Lipschitz step
  * sqrt(out3[0]/out3[1]) (:131) and the Barzilai-Borwein steps (:200-205). */
 int dp_step_norms(int dtype, int64_t n, const void* a1, const void* a0, const void* b1,

@@ -335,3 +335,26 @@ def test_bookshelf_design_through_the_loop(gpu, tmp_path):
     for i in range(db.num_movable_nodes):
         got = rows[db.node_names[i]]
         assert got[0] == pytest.approx(xe[i], rel=1e-5) and got[1] == pytest.approx(ye[i], rel=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_fused_step_norms_bit_identical(gpu, precision):
+    """The step sizes' FP64 norms formed inside the extrapolation and the
+    node-gradient passes (fused_norms, the default) give the loop the same
+    bits as the separate dp_step_norms pass: 40 iterations of config 0 with
+    identical metrics, line-search counts and positions."""
+    from dreamplace_amd.placer import GlobalPlacer
+    print("case fused step norms %s" % precision, flush=True)
+    dt = np.float64 if precision == "fp64" else np.float32
+    d = design(0, dtype=dt)
+    runs = []
+    for fused in (True, False):
+        gp = GlobalPlacer(d, gpu, params=dict(fused_norms=fused, early_stop=False),
+                          dtype=torch.float64 if dt == np.float64 else torch.float32,
+                          precision=precision, algorithm="tiled", schedule="pins")
+        pos, hist = gp.run(40)
+        runs.append((pos.cpu(), hist, list(gp.optimizer.backtracks)))
+    (p0, h0, b0), (p1, h1, b1) = runs
+    assert b0 == b1
+    assert h0 == h1
+    assert torch.equal(p0, p1)
