@@ -9,12 +9,18 @@ bins, synthetic, all inputs resident in HBM.  The timed form is FP32 with the
 spectral solve in BF16x3 (split bf16 on the matrix cores, FP32 accumulate);
 the same steps are also timed in FP64 for the speedup.
 
-Accuracy: the density map is bit-exact with the reference's deterministic
-FP32 map, so the precision forms are compared on that same map: FP64 solve
-vs the FP32/BF16x3 solve (potential, field) and the per-cell density
-gradient the FP64 gather forms from either field (l2err_solve_*, the gate).
-The FP32-form vs FP64-form differences of a whole step are reported too
-(l2err_step_*); they include the reference's own FP32 scatter rounding.
+Arithmetic: the timed FP32 form evaluates the wirelength relative to each
+net's extremes (dp_wa_forward_pins_centred) and every density overlap
+relative to the cell's first bin (DP_ARITH_BIN_RELATIVE): the same functions
+as the reference's, without its cancellation at die coordinates.  The form
+with the reference's own arithmetic (bit-exact with its FP32 map) is timed
+too (ms_per_step_reference_arith).
+
+Accuracy (the gate, l2err_gate): the whole step of the timed FP32 form
+against the FP64 form at the same positions -- potential, field, energy,
+per-cell density gradient and total gradient (l2err_step_*).  The solve
+alone on one map (l2err_solve_*) and the reference-arithmetic form's whole
+step (l2err_step_*_reference_arith) are reported beside it.
 
 Roofline: each op of the step (one C-ABI entry point, all of its kernels) is
 timed live with HIP events recorded by the library around its launches on
@@ -47,11 +53,11 @@ DP_OP_DENSITY_MAP, DP_OP_ELECTRIC_FORCE, DP_OP_WA_FORWARD, DP_OP_NODE_GRAD, DP_O
 
 # kernels of each op in the pins schedule with tiled density (rocprof names)
 OP_KERNELS = {
-    "wa_forward": ["k_pin_push", "k_wa_pins", "k_wa_big", "k_finish"],
-    "density_map": ["k_zero_words", "k_fill", "k_bin_count", "k_bin_scan", "k_bin_fill",
-                    "k_tile_scatter_f32", "k_drift_scatter", "k_rec_scatter_large", "k_convert_cpu"],
+    "wa_forward": ["k_pin_push", "k_wa_pins (finishes the total; no nets above 64 pins)"],
+    "density_map": ["k_zero_regions", "k_bin_count", "k_bin_scan", "k_bin_fill",
+                    "k_tile_scatter_f32(_acc)", "k_scatter_rest", "k_convert_cpu"],
     "poisson": ["k_sum_partial", "k_scale_centre", "k_gemm_bf16x3", "k_spectral_weights"],
-    "electric_force": ["k_fill", "k_tile_gather", "k_drift_gather", "k_rec_gather_large"],
+    "electric_force": ["k_tile_gather", "k_gather_rest"],
     "node_grad": ["k_node_grad"],
 }
 
@@ -159,10 +165,10 @@ def rel(torch, a, b):
     return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
 
-def accuracy(torch, obj32, obj64, dd64):
-    """Solve-only errors on the same (bit-exact) density map, and whole-step
-    FP32-form vs FP64-form differences.  Both objectives were last evaluated
-    at the same positions."""
+def accuracy(torch, obj32, obj64, dd64, suffix=""):
+    """Solve-only errors on the same density map, and whole-step FP32-form vs
+    FP64-form differences (the gate).  Both objectives were last evaluated at
+    the same positions.  `suffix` names a second FP32 form's fields."""
     ep = obj64.density
     m = obj32.density_map.double()
     fx64, fy64, phi64, e64 = obj64.plan.poisson(m, ep.bin_size_x, ep.bin_size_y)
@@ -175,19 +181,29 @@ def accuracy(torch, obj32, obj64, dd64):
     g64 = ep.electric_force(pos, fx64, fy64, one)
     g32 = ep.electric_force(pos, obj32.field_x.double(), obj32.field_y.double(), one)
     both = lambda a, b: torch.cat([a.reshape(-1), b.reshape(-1)])
+    # the FP64 form's potential map (its step takes the energy spectrally)
+    _, _, phi64s, _ = obj64.plan.poisson(obj64.density_map, ep.bin_size_x, ep.bin_size_y,
+                                         compute_energy=False)
+    e64s = float(obj64.energy.double().item())
+    step = {
+        "l2err_step_potential": rel(torch, phi32, phi64s),
+        "l2err_step_field": rel(torch, both(obj32.field_x, obj32.field_y),
+                                both(obj64.field_x, obj64.field_y)),
+        "l2err_step_energy": abs(float(obj32.energy.double().item()) - e64s) / abs(e64s),
+        "l2err_step_density_grad": rel(torch, obj32.density_grad, obj64.density_grad),
+        "l2err_step_grad": rel(torch, obj32.grad, obj64.grad),
+    }
+    if suffix:
+        return {k + suffix: v for k, v in step.items()}
     out = {
         "l2err_solve_potential": rel(torch, phi32, phi64),
         "l2err_solve_energy": abs(float(obj32.energy.double().item()) - float(e64.item()))
                               / abs(float(e64.item())),
         "l2err_solve_field": rel(torch, both(obj32.field_x, obj32.field_y), both(fx64, fy64)),
         "l2err_solve_density_grad": rel(torch, g32, g64),
-        "l2err_step_field": rel(torch, both(obj32.field_x, obj32.field_y),
-                                both(obj64.field_x, obj64.field_y)),
-        "l2err_step_density_grad": rel(torch, obj32.density_grad, obj64.density_grad),
-        "l2err_step_grad": rel(torch, obj32.grad, obj64.grad),
     }
-    out["l2err_gate"] = max(out["l2err_solve_potential"], out["l2err_solve_field"],
-                            out["l2err_solve_density_grad"], out["l2err_solve_energy"])
+    out.update(step)
+    out["l2err_gate"] = max(step.values())
     return out
 
 
@@ -329,15 +345,15 @@ def summarize_trace(trace_dir, warmup, steps):
 
 
 # kernels whose HBM bytes make each op's measured traffic (per-dispatch
-# medians; the int64 map fill belongs to the scatter, the float fill to the
-# gather, one k_finish to the wirelength reduction)
+# medians; the map clear belongs to the scatter; the wirelength total is
+# finished inside k_wa_pins when there are no nets above 64 pins, else by
+# k_wa_big and a k_finish)
 TRAFFIC_KERNELS = {
-    "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<", "dp::k_finish"],
-    "density_map": ["dp::k_zero_words", "dp::k_bin_count<", "dp::k_bin_scan",
-                    "dp::k_bin_fill<", "dp::k_tile_scatter", "dp::k_drift_scatter<",
-                    "dp::k_rec_scatter_large<", "dp::k_convert_cpu<"],
-    "electric_force": ["dp::k_fill<float>", "dp::k_tile_gather<", "dp::k_drift_gather<",
-                       "dp::k_rec_gather_large<"],
+    "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<"],
+    "density_map": ["dp::k_zero_regions", "dp::k_bin_count<", "dp::k_bin_scan",
+                    "dp::k_bin_fill<", "dp::k_tile_scatter", "dp::k_scatter_rest<",
+                    "dp::k_convert_cpu<"],
+    "electric_force": ["dp::k_tile_gather<", "dp::k_gather_rest<"],
     "node_grad": ["dp::k_node_grad<"],
 }
 
@@ -529,12 +545,13 @@ def precision_study(torch, args):
     log("[study] design built in %.1fs: %s" % (time.perf_counter() - t0, synth.summary(base)))
     reps = 20
 
-    def module(d, t, dtype, algorithm):
+    def module(d, t, dtype, algorithm, arithmetic="reference"):
         return ElectricPotential(
             t["node_size_x"].to(dtype), t["node_size_y"].to(dtype), t["bin_center_x"].to(dtype),
             t["bin_center_y"].to(dtype), d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x,
             d.bin_size_y, d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
-            t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"], algorithm=algorithm)
+            t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"], algorithm=algorithm,
+            arithmetic=arithmetic)
 
     for nb in ([1024, 2048] if args.scale == 1.0 else [base.num_bins_x, 2 * base.num_bins_x]):
         d = with_bins(base, nb)
@@ -544,6 +561,9 @@ def precision_study(torch, args):
         ep32 = module(d, t, torch.float32, "tiled")
         ep32.compute_initial_density_map(pos32)
         m32 = ep32.density_map(pos32)  # also bins the cells for the FP32 gather
+        ep32b = module(d, t, torch.float32, "tiled", "bin_relative")
+        ep32b.compute_initial_density_map(pos32)
+        ep32b.density_map(pos32)  # bins the cells for its gather
         ep64 = module(d, t, torch.float64, "global")
         one32 = torch.ones((), dtype=torch.float32, device=dev)
         one64 = torch.ones((), dtype=torch.float64, device=dev)
@@ -556,6 +576,7 @@ def precision_study(torch, args):
             m = m32.double() if prec == "fp64" else m32
             fx, fy, phi, e = plan.poisson(m, bx, by, True, True)
             g = ep32.electric_force(pos32, fx.float(), fy.float(), one32)
+            gb = ep32b.electric_force(pos32, fx.float(), fy.float(), one32)
             ms = []
             for _ in range(3):
                 plan.poisson(m, bx, by, False, True)
@@ -577,7 +598,8 @@ def precision_study(torch, args):
                        l2err_field=rel(torch, both(fx, fy), both(ref[0], ref[1])),
                        l2err_energy=abs(float(e.double().item()) - float(ref[3].item()))
                        / abs(float(ref[3].item())),
-                       l2err_density_grad_fp32_gather=rel(torch, g, g_ref))
+                       l2err_density_grad_fp32_gather=rel(torch, g, g_ref),
+                       l2err_density_grad_fp32_gather_bin_relative=rel(torch, gb, g_ref))
             if prec != "fp64":
                 rec["frac_of_fp32_peak"] = round(tf / MFMA_PEAK_TFLOPS["fp32"], 4)
             if prec == "bf16x3":
@@ -585,7 +607,7 @@ def precision_study(torch, args):
             if prec == "fp64":
                 rec["frac_of_fp64_peak"] = round(tf / MFMA_PEAK_TFLOPS["fp64"], 4)
             print(json.dumps(rec), flush=True)
-        del ep32, ep64, t
+        del ep32, ep32b, ep64, t
         torch.cuda.empty_cache()
 
 
@@ -611,6 +633,9 @@ def main():
                     help="tile: fillers relabelled into the density kernels' tile order before the "
                          "timed steps and taken by tile range (as the placer keeps them); "
                          "generation: as generated, binned every step")
+    ap.add_argument("--arith", default="accurate", choices=["accurate", "reference"],
+                    help="arithmetic of the timed FP32 form: accurate (centred wirelength, "
+                         "bin-relative density overlaps) or the reference's own")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -674,9 +699,11 @@ def main():
     lam = 1e-3
     dd32 = DeviceDesign(d, dev)
     relabel(dd32, args.movable_order)
+    arith = {"accurate": dict(wa_arith="centred", density_arith="bin_relative"),
+             "reference": dict(wa_arith="reference", density_arith="reference")}
     obj32 = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
                                algorithm=args.algorithm, schedule=args.schedule,
-                               energy_mode=args.energy_mode)
+                               energy_mode=args.energy_mode, **arith[args.arith])
     if args.filler_order == "tile":
         # fillers are interchangeable (one size, no pins): the engine keeps
         # them in the density kernels' tile order and takes them by tile range
@@ -750,6 +777,20 @@ def main():
         obj32.evaluate(pos32)
         result.update(fp64_ms_per_step=round(ms64, 4), speedup_vs_fp64=round(ms64 / ms32, 3))
         result.update(accuracy(torch, obj32, obj64, dd64))
+        if args.arith == "accurate":
+            # the same FP32 form in the reference's arithmetic (bit-exact with
+            # its FP32 map): its time and its whole-step errors
+            objr = PlacementObjective(dd32, gamma, density_weight=lam, precision=args.precision,
+                                      algorithm=args.algorithm, schedule=args.schedule,
+                                      energy_mode=args.energy_mode, **arith["reference"])
+            if args.filler_order == "tile":
+                objr.update_filler_ranges(pos32)
+            msr, _ = time_steps(torch, objr, pos32, max(args.steps // 2, 5), 2)
+            log("[bench] fp32/%s reference arithmetic: %.3f ms/step" % (args.precision, msr))
+            objr.evaluate(pos32)
+            result.update(ms_per_step_reference_arith=round(msr, 4))
+            result.update(accuracy(torch, objr, obj64, dd64, suffix="_reference_arith"))
+            del objr
         log("[bench] accuracy: " + ", ".join("%s %.2e" % (k, v) for k, v in result.items()
                                                if k.startswith("l2err")))
         del obj64, dd64
@@ -775,8 +816,8 @@ def main():
                         % (args.config, "" if args.scale == 1.0 else " scale %g" % args.scale,
                            d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, d.num_nets,
                            d.num_pins, d.num_bins_x, d.num_bins_y),
-            "form": "fp32 + %s spectral solve, %s schedule, %s density, %s energy" % (
-                args.precision, args.schedule, args.algorithm, args.energy_mode),
+            "form": "fp32 + %s spectral solve, %s schedule, %s density, %s energy, %s arithmetic" % (
+                args.precision, args.schedule, args.algorithm, args.energy_mode, args.arith),
             "filler_order": args.filler_order,
             "movable_order": args.movable_order,
             "ops": ops,

@@ -20,6 +20,7 @@ DP_CONVERT_CPU, DP_CONVERT_GPU = 0, 1
 DP_SPECTRAL_FP64, DP_SPECTRAL_FP32, DP_SPECTRAL_BF16X3 = 0, 1, 2
 DP_SCATTER_GLOBAL, DP_SCATTER_TILED = 0, 1
 DP_LAYOUT_SPLIT, DP_LAYOUT_INTERLEAVED = 0, 1
+DP_ARITH_BIN_RELATIVE = 0x100
 DP_NET_WEIGHTING_ADAMS, DP_NET_WEIGHTING_LILITH = 0, 1
 
 PRECISIONS = {"fp64": DP_SPECTRAL_FP64, "fp32": DP_SPECTRAL_FP32, "bf16x3": DP_SPECTRAL_BF16X3}
@@ -78,6 +79,8 @@ _SIGNATURES = {
                                 _P, _P]),
     "dp_wa_pins_workspace_bytes": (_S, [_L, _I]),
     "dp_wa_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
+                                _P, _P, _P, _P, _P, _P, _P]),
+    "dp_wa_forward_pins_centred": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
                                 _P, _P, _P, _P, _P, _P, _P]),
     "dp_lse_forward_pins": (_I, [_I, _P, _L, _L, _P, _P, _P, _P, _P, _L, _P, _L, _L, _P, _P, _P,
                                  _P, _P, _P, _P, _P, _P, _P]),

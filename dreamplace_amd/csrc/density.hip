@@ -43,10 +43,12 @@ __global__ void __launch_bounds__(256) k_tri_scatter(TriArgs<T> a, int64_t n_mov
       return;
     }
     // queue full: do the cell here (slow path, same arithmetic)
+    const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+    const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
     for (int k = c.bxl; k < c.bxh; ++k) {
-      T pxr = tri(c.x, c.sx, a.xl, k, a.bsx) * c.r;
+      T pxr = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, k - c.bxl, a.bsx) * c.r;
       for (int h = c.byl; h < c.byh; ++h) {
-        T area = pxr * tri(c.y, c.sy, a.yl, h, a.bsy);
+        T area = pxr * overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, h - c.byl, a.bsy);
         long long v = (long long)(area * a.scale);
         atomicAdd(&map[(int64_t)k * a.nby + h], (unsigned long long)v);
       }
@@ -54,10 +56,12 @@ __global__ void __launch_bounds__(256) k_tri_scatter(TriArgs<T> a, int64_t n_mov
     return;
   }
   T pxr[3], py[3];
+  const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+  const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    pxr[j] = tri(c.x, c.sx, a.xl, c.bxl + j, a.bsx) * c.r;
-    py[j] = tri(c.y, c.sy, a.yl, c.byl + j, a.bsy);
+    pxr[j] = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, j, a.bsx) * c.r;
+    py[j] = overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, j, a.bsy);
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -86,10 +90,12 @@ __global__ void __launch_bounds__(256) k_tri_scatter_large(TriArgs<T> a, unsigne
     CellBins<T> c = cell_bins(a, i);
     int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
     int total = nx * ny;
+    const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+    const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
       int k = c.bxl + t / ny, h = c.byl + t % ny;
-      T pxr = tri(c.x, c.sx, a.xl, k, a.bsx) * c.r;
-      T area = pxr * tri(c.y, c.sy, a.yl, h, a.bsy);
+      T pxr = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, k - c.bxl, a.bsx) * c.r;
+      T area = pxr * overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, h - c.byl, a.bsy);
       long long v = (long long)(area * a.scale);
       if (v != 0) atomicAdd(&map[(int64_t)k * a.nby + h], (unsigned long long)v);
     }
@@ -103,7 +109,7 @@ __global__ void __launch_bounds__(256) k_exact_scatter(const T* x, const T* y, c
                                                        const T* wy, const T* bcx, const T* bcy,
                                                        int64_t n, int nbx, int nby, T xl, T yl,
                                                        T xh, T yh, T bsx, T bsy, T scale,
-                                                       unsigned long long* map) {
+                                                       int rel, unsigned long long* map) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T bxl = x[i], byl = y[i];
@@ -113,9 +119,10 @@ __global__ void __launch_bounds__(256) k_exact_scatter(const T* x, const T* y, c
   int kyl = max((int)((byl - yl) / bsy), 0);
   int kyh = min((int)ceil((byh - yl) / bsy) + 1, nby);
   for (int k = kxl; k < kxh; ++k) {
-    T px = exact_overlap(bxl, bxh - bxl, bcx[k], bsx);
+    T px = rel ? exact_overlap_rel(bxl, wx[i], bcx[k], bsx) : exact_overlap(bxl, bxh - bxl, bcx[k], bsx);
     for (int h = kyl; h < kyh; ++h) {
-      T py = exact_overlap(byl, byh - byl, bcy[h], bsy);
+      T py = rel ? exact_overlap_rel(byl, wy[i], bcy[h], bsy)
+                 : exact_overlap(byl, byh - byl, bcy[h], bsy);
       T area = px * py;
       long long v = (long long)(area * scale);
       if (v != 0) atomicAdd(&map[(int64_t)k * nby + h], (unsigned long long)v);
@@ -261,6 +268,7 @@ TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, co
   a.uox = a.uni ? (T)uniform_filler[2] : T(0);
   a.uoy = a.uni ? (T)uniform_filler[3] : T(0);
   a.ur = a.uni ? (T)uniform_filler[4] : T(0);
+  a.acc = 0;
   return a;
 }
 
@@ -309,8 +317,10 @@ size_t dp_density_map_workspace_bytes(int64_t num_bins_x, int64_t num_bins_y, in
                                       int64_t large_capacity, int algorithm, int dtype) {
   const int64_t nb = num_bins_x * num_bins_y;
   size_t b = density_tiled_offset(nb, large_capacity);
-  if (algorithm == DP_SCATTER_TILED && tiled_supported((int)num_bins_x, (int)num_bins_y))
-    b += tiled_ws_bytes((int)num_bins_x, (int)num_bins_y, num_nodes, dtype);
+  const int alg = algorithm & ~DP_ARITH_BIN_RELATIVE;
+  if (alg == DP_SCATTER_TILED && tiled_supported((int)num_bins_x, (int)num_bins_y))
+    b += tiled_ws_bytes((int)num_bins_x, (int)num_bins_y, num_nodes, dtype,
+                        (algorithm & DP_ARITH_BIN_RELATIVE) != 0);
   return b;
 }
 
@@ -341,6 +351,8 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
          "dp_density_map: bad large_capacity");
   DP_ARG(convention == DP_CONVERT_CPU || convention == DP_CONVERT_GPU,
          "dp_density_map: bad convention %d", convention);
+  const int acc = (algorithm & DP_ARITH_BIN_RELATIVE) != 0;
+  algorithm &= ~DP_ARITH_BIN_RELATIVE;
   DP_ARG(algorithm == DP_SCATTER_GLOBAL || algorithm == DP_SCATTER_TILED,
          "dp_density_map: bad algorithm %d", algorithm);
   DP_ARG(density_map_out || fixed_map_out || movable_map_out, "dp_density_map: no output");
@@ -354,26 +366,28 @@ static int density_map_impl(const char* name, int dtype, const void* pos, int64_
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
                                     num_bins_x, num_bins_y, scale, uniform_filler,
                                     num_nodes - num_filler_nodes);
+    a.acc = acc;
     const T* init = (const T*)initial_density_map;
-    if (convention == DP_CONVERT_GPU && init) {
+    const bool tiled = algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y);
+    const bool quantize = convention == DP_CONVERT_GPU && init;
+    if (quantize) {
       hipLaunchKernelGGL(k_quantize<T>, dim3(grid_for(nb, 256)), dim3(256), 0, s, init, nb,
                          (T)scale, w.map);
-    } else {
-      zero_words(w.map, nb, s);
+    } else if (!tiled) {
+      zero_words(w.map, nb, s);  // the tiled scatter clears its maps itself
     }
     DP_LAUNCHED("density init");
     unsigned long long* fmap = nullptr;
     if (movable_map_out && num_filler_nodes > 0) {
       // the fillers' contributions in a map of their own: w.map then holds the
       // fixed and movable cells' alone, the map ElectricOverflow forms
+      // (cleared by the tiled scatter, the only path that takes it)
       fmap = w.filler_map;
-      zero_words(fmap, nb, s);
-      DP_LAUNCHED("filler map init");
     }
-    if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y)) {
+    if (tiled) {
       DP_CHECK_RC(scatter_tiled<T>(a, num_movable_nodes, num_nodes - num_filler_nodes,
                                    num_filler_nodes, w.tiled, w.map, fmap,
-                                   (const int*)filler_tile_start, s));
+                                   (const int*)filler_tile_start, !quantize, s));
     } else {
       DP_REQUIRE(!fmap, DP_ERR_UNSUPPORTED,
                  "%s: the movable map needs the tiled scatter", name);
@@ -429,7 +443,10 @@ int dp_density_map_ex(int dtype, const void* pos, int64_t num_nodes,
                           yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
                           num_bins_x, num_bins_y, convention, algorithm, large_capacity,
                           uniform_filler,
-                          algorithm == DP_SCATTER_TILED ? filler_tile_start : nullptr, workspace,
+                          (algorithm & ~DP_ARITH_BIN_RELATIVE) == DP_SCATTER_TILED
+                              ? filler_tile_start
+                              : nullptr,
+                          workspace,
                           density_map_out, nullptr, movable_map_out, stream);
 }
 
@@ -446,6 +463,8 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
   DP_ARG(num_movable_nodes >= 0 && num_terminals >= 0 &&
              num_movable_nodes + num_terminals <= num_nodes,
          "dp_fixed_density_map: bad node counts");
+  const int rel = (convention & DP_ARITH_BIN_RELATIVE) != 0;
+  convention &= ~DP_ARITH_BIN_RELATIVE;
   DP_ARG(convention == DP_CONVERT_CPU || convention == DP_CONVERT_GPU,
          "dp_fixed_density_map: bad convention");
   hipStream_t s = S(stream);
@@ -461,7 +480,7 @@ int dp_fixed_density_map(int dtype, const void* pos, int64_t num_nodes, const vo
                          dim3(256), 0, s, P + b, P + num_nodes + b, (const T*)node_size_x + b,
                          (const T*)node_size_y + b, (const T*)bin_center_x,
                          (const T*)bin_center_y, num_terminals, num_bins_x, num_bins_y, (T)xl,
-                         (T)yl, (T)xh, (T)yh, (T)bin_size_x, (T)bin_size_y, (T)scale, map);
+                         (T)yl, (T)xh, (T)yh, (T)bin_size_x, (T)bin_size_y, (T)scale, rel, map);
       DP_LAUNCHED("k_exact_scatter");
     }
     T bin_area = (T)(bin_size_x * bin_size_y);

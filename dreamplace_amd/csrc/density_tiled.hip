@@ -52,7 +52,7 @@ constexpr int kTile = 64;                 // bins per tile side
 // k_drift_scatter / k_drift_gather through global memory.
 constexpr int kHalo = 2;
 constexpr int kTileH = kTile + 2 + 2 * kHalo;
-constexpr int kMaxTiles = 2048;           // LDS histogram limit
+constexpr int kMaxTiles = 4096;           // LDS histogram limit (4096^2 bins)
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
 constexpr int kFillThreads = 1024;        // threads of a fill workgroup
 constexpr int kChunk = 8192;              // cells per scatter/gather work item
@@ -61,31 +61,44 @@ constexpr int kGatherSplit = 2;           // the gather does each work item in t
 // allows 4 workgroups per CU, so 8 waves each keep 8 waves per SIMD in flight
 constexpr int kTileThreads = 512;
 
-// Cells per LDS-staged fill sub-batch: 28 (FP32) or 48 (FP64) bytes of LDS
-// per cell, ~115 KB (FP32) / ~100 KB (FP64) in all with the tile arrays.
-template <typename T>
+// Cells per LDS-staged fill sub-batch: 26 (FP32) or 46 (FP64) bytes of LDS
+// per cell, 8 more (FP32) or 16 (FP64) with the bin-relative arithmetic's
+// residuals; with the 48 KB of tile arrays, 152 / 150 KB (FP32) and
+// 140 / 110 KB (FP64) in all.
+template <typename T, bool kAcc>
 struct FillSub {
-  static constexpr int value = sizeof(T) == 4 ? 4096 : 2048;
+  static constexpr int value = sizeof(T) == 4 ? (kAcc ? 3072 : 4096) : (kAcc ? 1024 : 2048);
 };
 
-// Records: one array per field, `total` entries each.
+// Records: one array per field, `total` entries each.  ex, ey (the
+// stretched origins' TwoSum residuals) exist with the bin-relative
+// arithmetic only, else they are null.
 template <typename T>
 struct Recs {
-  T *x, *y, *sx, *sy, *r;
+  T *x, *y, *sx, *sy, *r, *ex, *ey;
   int* idx;
-  __device__ __forceinline__ void put(int64_t j, T x_, T y_, T sx_, T sy_, T r_, int i) const {
+  __device__ __forceinline__ void put(int64_t j, T x_, T y_, T sx_, T sy_, T r_, int i, T ex_,
+                                      T ey_) const {
     x[j] = x_;
     y[j] = y_;
     sx[j] = sx_;
     sy[j] = sy_;
     r[j] = r_;
     idx[j] = i;
+    if (ex) {
+      ex[j] = ex_;
+      ey[j] = ey_;
+    }
   }
   // a uniform filler: size and ratio are the uniform constants
-  __device__ __forceinline__ void put_xy(int64_t j, T x_, T y_, int i) const {
+  __device__ __forceinline__ void put_xy(int64_t j, T x_, T y_, int i, T ex_, T ey_) const {
     x[j] = x_;
     y[j] = y_;
     idx[j] = i;
+    if (ex) {
+      ex[j] = ex_;
+      ey[j] = ey_;
+    }
   }
 };
 
@@ -130,11 +143,11 @@ static ClassSplit class_split(int64_t n0, int64_t total, int groups) {
 struct TiledLayout {
   int ntx, nty, ntiles;
   int64_t max_work;
-  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec[5], off_idx, off_drift,
-      total;
+  size_t off_count, off_start, off_wgbase, off_work, off_nwork, off_rec[5], off_err[2], off_idx,
+      off_drift, total;
 };
 
-static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) {
+static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize, bool acc) {
   TiledLayout L;
   L.ntx = (int)ceil_div(nbx, kTile);
   L.nty = (int)ceil_div(nby, kTile);
@@ -160,6 +173,12 @@ static TiledLayout tiled_layout(int nbx, int nby, int64_t ncells, size_t tsize) 
   o += align256(sizeof(int) * n);
   L.off_drift = o;  // filler-range cells outside their tile's window
   o += align256(sizeof(int) * n);
+  // the bin-relative arithmetic's residuals last, so that every other offset
+  // is the same with and without them
+  for (int f = 0; f < 2; ++f) {
+    L.off_err[f] = acc ? o : 0;
+    if (acc) o += align256(tsize * n);
+  }
   L.total = o;
   return L;
 }
@@ -172,6 +191,8 @@ static Recs<T> recs_of(char* ws, const TiledLayout& L) {
   r.sx = (T*)(ws + L.off_rec[2]);
   r.sy = (T*)(ws + L.off_rec[3]);
   r.r = (T*)(ws + L.off_rec[4]);
+  r.ex = L.off_err[0] ? (T*)(ws + L.off_err[0]) : nullptr;
+  r.ey = L.off_err[1] ? (T*)(ws + L.off_err[1]) : nullptr;
   r.idx = (int*)(ws + L.off_idx);
   return r;
 }
@@ -180,8 +201,8 @@ bool tiled_supported(int nbx, int nby) {
   return (int64_t)ceil_div(nbx, kTile) * ceil_div(nby, kTile) <= kMaxTiles;
 }
 
-size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype) {
-  return tiled_layout(nbx, nby, ncells, dtype == DP_FLOAT64 ? 8 : 4).total;
+size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype, bool acc) {
+  return tiled_layout(nbx, nby, ncells, dtype == DP_FLOAT64 ? 8 : 4, acc).total;
 }
 
 // index t of the (movable, filler) space -> node index
@@ -246,7 +267,7 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
       const int k = tile_key(c, nty);
       if (k < 0) {
         const int q = atomicAdd(ntail, 1);
-        rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i);
+        rec.put(total - 1 - q, c.x, c.y, c.sx, c.sy, c.r, (int)i, c.ex, c.ey);
       } else {
         // per-lane atomics: this pass is bound by its loads, and wave
         // aggregation (as in k_bin_fill) measured slower here
@@ -328,15 +349,17 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
 }
 
 // Exclusive scan of in[0, n) into out[0, n) by a workgroup of kFillThreads
-// (n <= 2 * kFillThreads).  wsum holds kFillThreads / 64 ints.  Ends with a
-// barrier.
+// (n <= 4 * kFillThreads: four consecutive entries per thread).  wsum holds
+// kFillThreads / 64 ints.  Ends with a barrier.
 __device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, int n, int* wsum) {
   constexpr int NW = kFillThreads / 64;
+  static_assert(kMaxTiles <= 4 * kFillThreads, "scan covers four entries per thread");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int k0 = 2 * threadIdx.x;
-  const int v0 = k0 < n ? in[k0] : 0;
-  const int v1 = k0 + 1 < n ? in[k0 + 1] : 0;
-  const int s = v0 + v1;
+  const int k0 = 4 * threadIdx.x;
+  int v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = k0 + j < n ? in[k0 + j] : 0;
+  const int s = v[0] + v[1] + v[2] + v[3];
   int incl = s;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -356,25 +379,32 @@ __device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, in
     if (lane < NW) wsum[lane] = wi - w;
   }
   __syncthreads();
-  const int base = wsum[wid] + incl - s;
-  if (k0 < n) out[k0] = base;
-  if (k0 + 1 < n) out[k0 + 1] = base + v0;
+  int base = wsum[wid] + incl - s;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (k0 + j < n) out[k0 + j] = base;
+    base += v[j];
+  }
   __syncthreads();
 }
 
-template <typename T>
+// kAcc: the bin-relative arithmetic (records carry the TwoSum residuals).
+template <typename T, bool kAcc>
 __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t n_mov,
                                                            int64_t fill_begin, int64_t total,
                                                            ClassSplit cs, int nty, int ntiles,
                                                            const int* tile_start,
                                                            const int* wg_base, Recs<T> rec) {
-  constexpr int S = FillSub<T>::value;
+  constexpr int S = FillSub<T, kAcc>::value;
   constexpr int PER = S / kFillThreads;
+  static_assert(PER * kFillThreads == S, "whole sub-batches per thread");
+  constexpr int SE = kAcc ? S : 1;
   __shared__ int s_base[kMaxTiles];  // next record slot of (this workgroup, tile)
   __shared__ int s_hist[kMaxTiles];  // sub-batch cells per tile
   __shared__ int s_off[kMaxTiles];   // exclusive scan of s_hist
   __shared__ int s_wsum[kFillThreads / 64];
   __shared__ T s_x[S], s_y[S], s_sx[S], s_sy[S], s_r[S];
+  __shared__ T s_ex[SE], s_ey[SE];
   __shared__ int s_idx[S];
   __shared__ unsigned short s_key[S];
   int64_t b, e;
@@ -426,6 +456,10 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
         s_sx[p] = c[m].sx;
         s_sy[p] = c[m].sy;
         s_r[p] = c[m].r;
+        if (kAcc) {
+          s_ex[p] = c[m].ex;
+          s_ey[p] = c[m].ey;
+        }
         s_idx[p] = node[m];
         s_key[p] = (unsigned short)key[m];
       }
@@ -435,10 +469,11 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
     for (int p = threadIdx.x; p < nsmall; p += kFillThreads) {
       const int k = s_key[p];
       const int64_t g = (int64_t)s_base[k] + (p - s_off[k]);
+      const T ex = kAcc ? s_ex[p] : T(0), ey = kAcc ? s_ey[p] : T(0);
       if (cls)
-        rec.put_xy(g, s_x[p], s_y[p], s_idx[p]);
+        rec.put_xy(g, s_x[p], s_y[p], s_idx[p], ex, ey);
       else
-        rec.put(g, s_x[p], s_y[p], s_sx[p], s_sy[p], s_r[p], s_idx[p]);
+        rec.put(g, s_x[p], s_y[p], s_sx[p], s_sy[p], s_r[p], s_idx[p], ex, ey);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < ntiles; k += kFillThreads) s_base[k] += s_hist[k];
@@ -464,13 +499,46 @@ constexpr int kBatch = 4;
 
 template <typename T>
 struct RecBatch {
-  T x[kBatch], y[kBatch], sx[kBatch], sy[kBatch], r[kBatch];
+  T x[kBatch], y[kBatch], sx[kBatch], sy[kBatch], r[kBatch], ex[kBatch], ey[kBatch];
   int idx[kBatch];
   bool ok[kBatch];
 };
 
-// uniform: a work item of uniform fillers (origin and index only stored).
+// The three overlaps per axis of a record cell with bins bxl + q, byl + q
+// (q = 0, 1, 2): the reference arithmetic (tri) or, with kAcc, the
+// bin-relative one.  The tile kernels take the arithmetic as a template
+// argument (a runtime choice costs them registers and occupancy); the
+// drift kernels pass a.acc.
+template <bool kAcc, typename T>
+__device__ __forceinline__ void rec_overlaps(const TriArgs<T>& a, T x, T y, T sx, T sy, T ex,
+                                             T ey, int bxl, int byl, T px[3], T py[3]) {
+  if constexpr (kAcc) {
+    const T dx = rel_origin(x, ex, a.xl, bxl, a.bsx);
+    const T dy = rel_origin(y, ey, a.yl, byl, a.bsy);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      px[q] = tri_rel(dx, sx, q, a.bsx);
+      py[q] = tri_rel(dy, sy, q, a.bsy);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
+      py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
+    }
+  }
+}
 template <typename T>
+__device__ __forceinline__ void rec_overlaps_rt(const TriArgs<T>& a, T x, T y, T sx, T sy, T ex,
+                                                T ey, int bxl, int byl, T px[3], T py[3]) {
+  if (a.acc)
+    rec_overlaps<true>(a, x, y, sx, sy, ex, ey, bxl, byl, px, py);
+  else
+    rec_overlaps<false>(a, x, y, sx, sy, ex, ey, bxl, byl, px, py);
+}
+
+// uniform: a work item of uniform fillers (origin and index only stored).
+template <bool kAcc, typename T>
 __device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& rec, int j0,
                                            int stride, int end, bool with_idx, bool uniform,
                                            RecBatch<T>& b) {
@@ -481,6 +549,8 @@ __device__ __forceinline__ void load_batch(const TriArgs<T>& a, const Recs<T>& r
     if (b.ok[u]) {
       b.x[u] = rec.x[j];
       b.y[u] = rec.y[j];
+      b.ex[u] = kAcc ? rec.ex[j] : T(0);
+      b.ey[u] = kAcc ? rec.ey[j] : T(0);
       if (uniform) {
         b.sx[u] = a.uwx;
         b.sy[u] = a.uwy;
@@ -513,7 +583,7 @@ __device__ __forceinline__ void wave_append(int* list, int* count, bool put, int
 // A work item of class 2: a range [j0, end) of uniform fillers (filler-
 // relative), read from the positions directly.  The stretched origin is the
 // same T sum the fill pass stores in a record.
-template <typename T>
+template <bool kAcc, typename T>
 __device__ __forceinline__ void load_range(const TriArgs<T>& a, int j0, int stride, int end,
                                            RecBatch<T>& b) {
 #pragma unroll
@@ -522,8 +592,11 @@ __device__ __forceinline__ void load_range(const TriArgs<T>& a, int j0, int stri
     b.ok[u] = j < end;
     if (b.ok[u]) {
       const int64_t i = a.fill_begin + j;
-      b.x[u] = a.x[i] + a.uox;
-      b.y[u] = a.y[i] + a.uoy;
+      const T x = a.x[i], y = a.y[i];
+      b.x[u] = x + a.uox;
+      b.y[u] = y + a.uoy;
+      b.ex[u] = kAcc ? two_sum_err(x, a.uox, b.x[u]) : T(0);
+      b.ey[u] = kAcc ? two_sum_err(y, a.uoy, b.y[u]) : T(0);
       b.sx[u] = a.uwx;
       b.sy[u] = a.uwy;
       b.r[u] = a.ur;
@@ -532,7 +605,7 @@ __device__ __forceinline__ void load_range(const TriArgs<T>& a, int j0, int stri
   }
 }
 
-template <typename T>
+template <bool kAcc, typename T>
 __device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const int4* work,
                                              const int* nwork, const Recs<T>& rec,
                                              unsigned long long* map,
@@ -550,9 +623,9 @@ __device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
       if (it.w == 2)
-        load_range(a, j0, blockDim.x, it.z, b);
+        load_range<kAcc>(a, j0, blockDim.x, it.z, b);
       else
-        load_batch(a, rec, j0, blockDim.x, it.z, false, it.w != 0, b);
+        load_batch<kAcc>(a, rec, j0, blockDim.x, it.z, false, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const T x = b.x[u], y = b.y[u], sx = b.sx[u], sy = b.sy[u], r = b.r[u];
@@ -567,11 +640,9 @@ __device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const
         wave_append(drift, ndrift, drifting, drifting ? b.idx[u] : 0);
         if (!b.ok[u] || drifting) continue;
         T pxr[3], py[3];
+        rec_overlaps<kAcc>(a, x, y, sx, sy, b.ex[u], b.ey[u], bxl, byl, pxr, py);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          pxr[q] = tri(x, sx, a.xl, bxl + q, a.bsx) * r;
-          py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
-        }
+        for (int q = 0; q < 3; ++q) pxr[q] = pxr[q] * r;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           if (q < nx) {
@@ -607,28 +678,42 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
 k_tile_scatter_f32(TriArgs<float> a, int nty, const int4* work, const int* nwork,
                    Recs<float> rec, unsigned long long* map, unsigned long long* filler_map,
                    int* drift, int* ndrift) {
-  tile_scatter(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
+  tile_scatter<false>(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
 }
 
+// The bin-relative arithmetic's residuals take registers that 8 waves per
+// SIMD would spill: the compiler's choice there.
+__global__ void __launch_bounds__(kTileThreads)
+k_tile_scatter_f32_acc(TriArgs<float> a, int nty, const int4* work, const int* nwork,
+                       Recs<float> rec, unsigned long long* map, unsigned long long* filler_map,
+                       int* drift, int* ndrift) {
+  tile_scatter<true>(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
+}
+
+template <bool kAcc>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile_scatter_f64(TriArgs<double> a, int nty, const int4* work, const int* nwork,
                    Recs<double> rec, unsigned long long* map, unsigned long long* filler_map,
                    int* drift, int* ndrift) {
-  tile_scatter(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
+  tile_scatter<kAcc>(a, nty, work, nwork, rec, map, filler_map, drift, ndrift);
 }
 
-template <typename T>
+template <typename T, bool kAcc>
 struct TileScatterKernel;
 template <>
-struct TileScatterKernel<float> {
+struct TileScatterKernel<float, false> {
   static constexpr auto fn = k_tile_scatter_f32;
 };
 template <>
-struct TileScatterKernel<double> {
-  static constexpr auto fn = k_tile_scatter_f64;
+struct TileScatterKernel<float, true> {
+  static constexpr auto fn = k_tile_scatter_f32_acc;
+};
+template <bool kAcc>
+struct TileScatterKernel<double, kAcc> {
+  static constexpr auto fn = k_tile_scatter_f64<kAcc>;
 };
 
-template <typename T>
+template <typename T, bool kAcc>
 __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int nty, const int4* work,
                                                      const int* nwork, Recs<T> rec,
                                                      const T* __restrict__ fx,
@@ -662,9 +747,9 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
     for (int j0 = it.y + threadIdx.x; j0 < it.z; j0 += kBatch * blockDim.x) {
       RecBatch<T> b;
       if (it.w == 2)
-        load_range(a, j0, blockDim.x, it.z, b);
+        load_range<kAcc>(a, j0, blockDim.x, it.z, b);
       else
-        load_batch(a, rec, j0, blockDim.x, it.z, true, it.w != 0, b);
+        load_batch<kAcc>(a, rec, j0, blockDim.x, it.z, true, it.w != 0, b);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (!b.ok[u]) continue;
@@ -672,11 +757,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
         int bxl, nx, byl, ny;
         rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
         T px[3], py[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
-          py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
-        }
+        rec_overlaps<kAcc>(a, x, y, sx, sy, b.ex[u], b.ey[u], bxl, byl, px, py);
         const int lx = bxl - wx0, ly = byl - wy0;
         // a range cell outside its tile's window with a non-empty span is on
         // the drift list (k_drift_gather); an empty span gives 0 here
@@ -712,21 +793,21 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
 // done through global memory with the tile kernels' per-cell arithmetic, so
 // the results are the bits the window path would give.
 template <typename T>
-__global__ void __launch_bounds__(256) k_drift_scatter(TriArgs<T> a, const int* drift,
-                                                       const int* ndrift,
-                                                       unsigned long long* dst) {
+__device__ __forceinline__ void drift_scatter(const TriArgs<T>& a, const int* drift,
+                                              const int* ndrift, unsigned long long* dst) {
   const int cnt = *ndrift;
   for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < cnt; q0 += gridDim.x * blockDim.x) {
     const int64_t i = drift[q0];
-    const T x = a.x[i] + a.uox, y = a.y[i] + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    const T x0 = a.x[i], y0 = a.y[i];
+    const T x = x0 + a.uox, y = y0 + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    const T ex = a.acc ? two_sum_err(x0, a.uox, x) : T(0);
+    const T ey = a.acc ? two_sum_err(y0, a.uoy, y) : T(0);
     int bxl, nx, byl, ny;
     rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
     T pxr[3], py[3];
+    rec_overlaps_rt(a, x, y, sx, sy, ex, ey, bxl, byl, pxr, py);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      pxr[q] = tri(x, sx, a.xl, bxl + q, a.bsx) * r;
-      py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
-    }
+    for (int q = 0; q < 3; ++q) pxr[q] = pxr[q] * r;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       if (q < nx) {
@@ -745,23 +826,22 @@ __global__ void __launch_bounds__(256) k_drift_scatter(TriArgs<T> a, const int* 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_drift_gather(TriArgs<T> a, const int* drift,
-                                                      const int* ndrift, const T* __restrict__ fx,
-                                                      const T* __restrict__ fy, const T* grad,
-                                                      int64_t num_nodes, int interleaved, T* out) {
+__device__ __forceinline__ void drift_gather(const TriArgs<T>& a, const int* drift,
+                                             const int* ndrift, const T* __restrict__ fx,
+                                             const T* __restrict__ fy, const T* grad,
+                                             int64_t num_nodes, int interleaved, T* out) {
   const int cnt = *ndrift;
   const T g = grad ? *grad : T(1);
   for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < cnt; q0 += gridDim.x * blockDim.x) {
     const int64_t i = drift[q0];
-    const T x = a.x[i] + a.uox, y = a.y[i] + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    const T x0 = a.x[i], y0 = a.y[i];
+    const T x = x0 + a.uox, y = y0 + a.uoy, sx = a.uwx, sy = a.uwy, r = a.ur;
+    const T ex = a.acc ? two_sum_err(x0, a.uox, x) : T(0);
+    const T ey = a.acc ? two_sum_err(y0, a.uoy, y) : T(0);
     int bxl, nx, byl, ny;
     rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
     T px[3], py[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      px[q] = tri(x, sx, a.xl, bxl + q, a.bsx);
-      py[q] = tri(y, sy, a.yl, byl + q, a.bsy);
-    }
+    rec_overlaps_rt(a, x, y, sx, sy, ex, ey, bxl, byl, px, py);
     T gx = 0, gy = 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -782,11 +862,11 @@ __global__ void __launch_bounds__(256) k_drift_gather(TriArgs<T> a, const int* d
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T> rec,
-                                                           int64_t total, const int* ntail,
-                                                           unsigned long long* map,
-                                                           unsigned long long* filler_map,
-                                                           int64_t fill_begin) {
+__device__ __forceinline__ void rec_scatter_large(const TriArgs<T>& a, const Recs<T>& rec,
+                                                  int64_t total, const int* ntail,
+                                                  unsigned long long* map,
+                                                  unsigned long long* filler_map,
+                                                  int64_t fill_begin) {
   const int cnt = *ntail;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
     const int64_t j = total - 1 - q;
@@ -795,10 +875,12 @@ __global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T>
     rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
     if (nx <= 0 || ny <= 0) continue;
     unsigned long long* dst = (filler_map && rec.idx[j] >= fill_begin) ? filler_map : map;
+    const T dx = rel_origin(x, rec.ex ? rec.ex[j] : T(0), a.xl, bxl, a.bsx);
+    const T dy = rel_origin(y, rec.ey ? rec.ey[j] : T(0), a.yl, byl, a.bsy);
     for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
       const int k = bxl + t / ny, h = byl + t % ny;
-      T pxr = tri(x, sx, a.xl, k, a.bsx) * r;
-      T area = pxr * tri(y, sy, a.yl, h, a.bsy);
+      T pxr = overlap(a.acc, x, dx, sx, a.xl, bxl, k - bxl, a.bsx) * r;
+      T area = pxr * overlap(a.acc, y, dy, sy, a.yl, byl, h - byl, a.bsy);
       long long v = (long long)(area * a.scale);
       if (v != 0) atomicAdd(&dst[(int64_t)k * a.nby + h], (unsigned long long)v);
     }
@@ -808,12 +890,11 @@ __global__ void __launch_bounds__(256) k_rec_scatter_large(TriArgs<T> a, Recs<T>
 // Tail records: one wavefront per cell, fixed-order reduction; a cell with an
 // empty span gets a zero gradient.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> rec,
-                                                          int64_t total, const int* ntail,
-                                                          const T* __restrict__ fx,
-                                                          const T* __restrict__ fy, const T* grad,
-                                                          int64_t num_nodes, int interleaved,
-                                                          T* out) {
+__device__ __forceinline__ void rec_gather_large(const TriArgs<T>& a, const Recs<T>& rec,
+                                                 int64_t total, const int* ntail,
+                                                 const T* __restrict__ fx,
+                                                 const T* __restrict__ fy, const T* grad,
+                                                 int64_t num_nodes, int interleaved, T* out) {
   const int cnt = *ntail;
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -826,10 +907,13 @@ __global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> 
     rec_bins(a, x, y, sx, sy, bxl, nx, byl, ny);
     const bool empty = nx <= 0 || ny <= 0;
     const int nbins = empty ? 0 : nx * ny;
+    const T dx = rel_origin(x, rec.ex ? rec.ex[j] : T(0), a.xl, bxl, a.bsx);
+    const T dy = rel_origin(y, rec.ey ? rec.ey[j] : T(0), a.yl, byl, a.bsy);
     T gx = 0, gy = 0;
     for (int t = lane; t < nbins; t += 64) {
       const int k = bxl + t / ny, h = byl + t % ny;
-      T area = tri(x, sx, a.xl, k, a.bsx) * tri(y, sy, a.yl, h, a.bsy);
+      T area = overlap(a.acc, x, dx, sx, a.xl, bxl, k - bxl, a.bsx) *
+               overlap(a.acc, y, dy, sy, a.yl, byl, h - byl, a.bsy);
       const int64_t idx = (int64_t)k * a.nby + h;
       gx += area * fx[idx];
       gy += area * fy[idx];
@@ -845,12 +929,44 @@ __global__ void __launch_bounds__(256) k_rec_gather_large(TriArgs<T> a, Recs<T> 
   }
 }
 
+// What the tile scatter leaves, in one launch: the drift list (a thread per
+// cell; empty unless filler ranges are used), then the tail records (a
+// workgroup per cell).  Every contribution is an order-free integer add.
+template <typename T>
+__global__ void __launch_bounds__(256) k_scatter_rest(TriArgs<T> a, const int* drift,
+                                                      const int* ndrift,
+                                                      unsigned long long* drift_dst, Recs<T> rec,
+                                                      int64_t total, const int* ntail,
+                                                      unsigned long long* map,
+                                                      unsigned long long* filler_map,
+                                                      int64_t fill_begin) {
+  drift_scatter(a, drift, ndrift, drift_dst);
+  rec_scatter_large(a, rec, total, ntail, map, filler_map, fill_begin);
+}
+
+// What the tile gather leaves, in one launch: the zero gradient of the
+// terminals [t0, t1), the drift list and the tail records (a wavefront per
+// cell).  Each node's gradient is written by exactly one of them.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_rest(TriArgs<T> a, int64_t t0, int64_t t1,
+                                                     const int* drift, const int* ndrift,
+                                                     Recs<T> rec, int64_t total,
+                                                     const int* ntail, const T* __restrict__ fx,
+                                                     const T* __restrict__ fy, const T* grad,
+                                                     int64_t num_nodes, int interleaved, T* out) {
+  for (int64_t i = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t1;
+       i += (int64_t)gridDim.x * blockDim.x)
+    put_grad(out, num_nodes, i, T(0), T(0), interleaved);
+  drift_gather(a, drift, ndrift, fx, fy, grad, num_nodes, interleaved, out);
+  rec_gather_large(a, rec, total, ntail, fx, fy, grad, num_nodes, interleaved, out);
+}
+
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
                   char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
-                  const int* filler_ranges, hipStream_t s) {
+                  const int* filler_ranges, bool zero_map, hipStream_t s) {
   const int64_t total = n_mov + n_fill;
-  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
+  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T), a.acc != 0);
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled scatter: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
   DP_REQUIRE(!filler_map || n_fill == 0 || a.uni, DP_ERR_UNSUPPORTED,
@@ -869,11 +985,21 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
   int* nwork = (int*)(tiled_ws + L.off_nwork);
   int* ntail = nwork + 1;
   Recs<T> rec = recs_of<T>(tiled_ws, L);
-  hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(2 * L.ntiles, 256)), dim3(256), 0, s, tile_count,
-                     2 * (int64_t)L.ntiles, 0);
   int* ndrift = nwork + 2;
   int* drift = (int*)(tiled_ws + L.off_drift);
-  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, nwork, (int64_t)3, 0);
+  // one launch clears the maps, the per-tile counts and the work / tail /
+  // drift counters
+  const int64_t nb = (int64_t)a.nbx * a.nby;
+  ZeroRegions z = {};
+  z.p[0] = (uint32_t*)tile_count;
+  z.n[0] = 2 * (int64_t)L.ntiles;
+  z.p[1] = (uint32_t*)nwork;
+  z.n[1] = 3;
+  z.p[2] = zero_map ? (uint32_t*)map : nullptr;
+  z.n[2] = 2 * nb;
+  z.p[3] = (uint32_t*)filler_map;
+  z.n[3] = 2 * nb;
+  zero_regions(z, s);
   DP_LAUNCHED("tiled init");
   if (total > 0) {
     // uniform fillers are class 1 (their own workgroups and record runs);
@@ -886,24 +1012,22 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, (const int*)tile_count, L.ntiles,
                        tile_start, work, nwork, ranges ? filler_ranges : nullptr, (int)n_fill);
     DP_LAUNCHED("k_bin_scan");
+    auto fill = a.acc ? k_bin_fill<T, true> : k_bin_fill<T, false>;
     DP_PROBED(DP_KERNEL_DENSITY_BIN, s,
-              hipLaunchKernelGGL(k_bin_fill<T>, dim3(kBinGroups), dim3(kFillThreads), 0, s, a,
-                                 n_mov, fill_begin, total, cs, L.nty, L.ntiles,
-                                 (const int*)tile_start, (const int*)wg_base, rec));
+              hipLaunchKernelGGL(fill, dim3(kBinGroups), dim3(kFillThreads), 0, s, a, n_mov,
+                                 fill_begin, total, cs, L.nty, L.ntiles, (const int*)tile_start,
+                                 (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
+    auto scat = a.acc ? TileScatterKernel<T, true>::fn : TileScatterKernel<T, false>::fn;
     DP_PROBED(DP_KERNEL_DENSITY_SCATTER, s,
-              hipLaunchKernelGGL(TileScatterKernel<T>::fn, dim3(2048), dim3(kTileThreads), 0, s,
+              hipLaunchKernelGGL(scat, dim3(2048), dim3(kTileThreads), 0, s,
                                  a, L.nty, (const int4*)work, (const int*)nwork, rec, map,
                                  filler_map, drift, ndrift));
     DP_LAUNCHED("k_tile_scatter");
-    if (ranges) {
-      hipLaunchKernelGGL(k_drift_scatter<T>, dim3(256), dim3(256), 0, s, a, (const int*)drift,
-                         (const int*)ndrift, filler_map ? filler_map : map);
-      DP_LAUNCHED("k_drift_scatter");
-    }
-    hipLaunchKernelGGL(k_rec_scatter_large<T>, dim3(1024), dim3(256), 0, s, a, rec, total,
+    hipLaunchKernelGGL(k_scatter_rest<T>, dim3(1024), dim3(256), 0, s, a, (const int*)drift,
+                       (const int*)ndrift, filler_map ? filler_map : map, rec, total,
                        (const int*)ntail, map, filler_map, fill_begin);
-    DP_LAUNCHED("k_rec_scatter_large");
+    DP_LAUNCHED("k_scatter_rest");
   }
   return DP_OK;
 }
@@ -913,7 +1037,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
                  const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
                  hipStream_t s) {
   const int64_t total = n_mov + n_fill;
-  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T));
+  TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T), a.acc != 0);
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
              "tiled gather: %d tiles exceed the LDS histogram (%d)", L.ntiles, kMaxTiles);
   const int4* work = (const int4*)(tiled_ws + L.off_work);
@@ -921,40 +1045,27 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
   const int* ntail = nwork + 1;
   Recs<T> rec = recs_of<T>(tiled_ws, L);
   // terminals [n_mov, num_nodes - n_fill) get 0; every other node has a record
-  const int64_t nterm = num_nodes - n_fill - n_mov;
-  if (nterm > 0) {
-    if (interleaved) {
-      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(2 * nterm, 256)), dim3(256), 0, s,
-                         out + 2 * n_mov, 2 * nterm, T(0));
-    } else {
-      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(nterm, 256)), dim3(256), 0, s, out + n_mov,
-                         nterm, T(0));
-      hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(nterm, 256)), dim3(256), 0, s,
-                         out + num_nodes + n_mov, nterm, T(0));
-    }
-    DP_LAUNCHED("terminal gradient clear");
-  }
+  const int64_t t0 = n_mov, t1 = num_nodes - n_fill;
   if (total > 0) {
+    auto gath = a.acc ? k_tile_gather<T, true> : k_tile_gather<T, false>;
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
-              hipLaunchKernelGGL(k_tile_gather<T>, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty, work,
+              hipLaunchKernelGGL(gath, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty, work,
                                  nwork, rec, fx, fy, grad, num_nodes, interleaved, out));
     DP_LAUNCHED("k_tile_gather");
-    // fillers taken by range whose span left their tile's window (the list is
-    // empty unless the last density map used filler ranges)
-    hipLaunchKernelGGL(k_drift_gather<T>, dim3(256), dim3(256), 0, s, a,
-                       (const int*)(tiled_ws + L.off_drift), nwork + 2, fx, fy, grad, num_nodes,
-                       interleaved, out);
-    DP_LAUNCHED("k_drift_gather");
-    hipLaunchKernelGGL(k_rec_gather_large<T>, dim3(256), dim3(256), 0, s, a, rec, total, ntail,
-                       fx, fy, grad, num_nodes, interleaved, out);
-    DP_LAUNCHED("k_rec_gather_large");
   }
+  // the terminals, the fillers taken by range whose span left their tile's
+  // window (the list is empty unless the last density map used filler
+  // ranges) and the tail records
+  hipLaunchKernelGGL(k_gather_rest<T>, dim3(256), dim3(256), 0, s, a, t0, t1,
+                     (const int*)(tiled_ws + L.off_drift), nwork + 2, rec, total, ntail, fx, fy,
+                     grad, num_nodes, interleaved, out);
+  DP_LAUNCHED("k_gather_rest");
   return DP_OK;
 }
 
 #define DP_TILED_INST(T)                                                                     \
   template int scatter_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, char*,        \
-                                unsigned long long*, unsigned long long*, const int*,        \
+                                unsigned long long*, unsigned long long*, const int*, bool,  \
                                 hipStream_t);                                                \
   template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, const T*,      \
                                const T*, const T*, char*, int, T*, hipStream_t);

@@ -194,6 +194,32 @@ static __global__ void k_zero_words(unsigned long long* p, int64_t n) {
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) p[n - 1] = 0ull;
 }
 
+// Zero up to four regions in one launch: region r is n[r] 4-byte words at
+// p[r] (16-byte aligned; null skips it), cleared with 16-byte stores and
+// 4-byte stores for the rest.
+struct ZeroRegions {
+  uint32_t* p[4];
+  int64_t n[4];
+};
+static __global__ void k_zero_regions(ZeroRegions z) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (!z.p[r]) continue;
+    const int64_t q = z.n[r] >> 2;
+    uint4* v = (uint4*)z.p[r];
+    for (int64_t i = t; i < q; i += stride) v[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t i = 4 * q + t; i < z.n[r]; i += stride) z.p[r][i] = 0u;
+  }
+}
+inline void zero_regions(const ZeroRegions& z, hipStream_t s) {
+  int64_t m = 1;
+  for (int r = 0; r < 4; ++r)
+    if (z.p[r] && (z.n[r] >> 2) > m) m = z.n[r] >> 2;
+  hipLaunchKernelGGL(k_zero_regions, dim3(grid_for(m, 256)), dim3(256), 0, s, z);
+}
+
 // Launch the zeroing of n words at p on s (p 16-byte aligned, else word stores).
 inline void zero_words(unsigned long long* p, int64_t n, hipStream_t s) {
   if (((uintptr_t)p & 15) == 0)

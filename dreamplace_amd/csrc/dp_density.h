@@ -18,6 +18,8 @@ struct TriArgs {
   int uni;
   int64_t fill_begin;
   T uwx, uwy, uox, uoy, ur;
+  // bin-relative overlap arithmetic (DP_ARITH_BIN_RELATIVE; see overlap())
+  int acc;
 };
 
 // Overlap of [x, x+s) with bin k along one axis:
@@ -32,6 +34,40 @@ __device__ __forceinline__ T tri(T x, T s, T xl, int k, T b) {
   return hi - lo;
 }
 
+// x + o == s + err exactly, for s = x + o rounded to T (Knuth's TwoSum; the
+// file is compiled without contraction or reassociation).
+template <typename T>
+__device__ __forceinline__ T two_sum_err(T x, T o, T s) {
+  const T bb = s - x;
+  return (x - (s - bb)) + (o - bb);
+}
+
+// Bin-relative form of tri for bin first + q: the cell's stretched origin
+// xs + err (err from two_sum_err) measured from the lower edge of bin
+// `first`, d = (xs - (xl + first*b)) + err, then
+// min(d + s, q*b + b) - max(d, q*b).  Every operand is of the order of a few
+// bins, where tri's x + s rounds at the order of the die coordinates
+// (2.4e-4 at 5000 units in FP32, ~2.5e-5 of a bin of 10 units).
+template <typename T>
+__device__ __forceinline__ T rel_origin(T xs, T err, T xl, int first, T b) {
+  return (xs - (xl + (T)first * b)) + err;
+}
+template <typename T>
+__device__ __forceinline__ T tri_rel(T d, T s, int q, T b) {
+  const T lo_q = (T)q * b;
+  const T e = d + s, be = lo_q + b;
+  T hi = (be < e) ? be : e;
+  T lo = (d < lo_q) ? lo_q : d;
+  return hi - lo;
+}
+// Overlap of a cell's span with bin first + q along one axis, in the
+// arithmetic `acc` selects: tri (the reference's) or tri_rel with d from
+// rel_origin.
+template <typename T>
+__device__ __forceinline__ T overlap(int acc, T xs, T d, T s, T xl, int first, int q, T b) {
+  return acc ? tri_rel(d, s, q, b) : tri(xs, s, xl, first + q, b);
+}
+
 // Exact overlap with a bin given its centre, no boundary stretch
 // (density_function.h:19-36 with flag = true), clamped at 0.
 template <typename T>
@@ -43,10 +79,23 @@ __device__ __forceinline__ T exact_overlap(T x, T s, T bc, T b) {
   T v = hi - lo;
   return (T(0) < v) ? v : T(0);
 }
+// exact_overlap in the bin-relative arithmetic: d = (x - bc) + b/2, then
+// min(d + s, b) - max(d, 0), clamped at 0.
+template <typename T>
+__device__ __forceinline__ T exact_overlap_rel(T x, T s, T bc, T b) {
+  T d = (x - bc) + b / 2;
+  T e = d + s;
+  T hi = (b < e) ? b : e;
+  T lo = (d < T(0)) ? T(0) : d;
+  T v = hi - lo;
+  return (T(0) < v) ? v : T(0);
+}
 
+// A cell's stretched origin (x, y), its TwoSum residuals (ex, ey; 0 unless
+// TriArgs::acc), clamped sizes, ratio and bin range.
 template <typename T>
 struct CellBins {
-  T x, y, sx, sy, r;
+  T x, y, sx, sy, r, ex, ey;
   int bxl, bxh, byl, byh;
 };
 
@@ -86,6 +135,8 @@ __device__ __forceinline__ CellBins<T> bins_of(const TriArgs<T>& a, const RawCel
   c.sy = v.wy;
   c.x = v.x + v.ox;
   c.y = v.y + v.oy;
+  c.ex = a.acc ? two_sum_err(v.x, v.ox, c.x) : T(0);
+  c.ey = a.acc ? two_sum_err(v.y, v.oy, c.y) : T(0);
   c.r = v.r;
   c.bxl = max((int)((c.x - a.xl) * a.inv_x), 0);
   c.bxh = min((int)((c.x + c.sx - a.xl) * a.inv_x) + 1, a.nbx);
@@ -123,16 +174,19 @@ TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, co
 
 // Tiled (LDS-binned) scatter and gather, density_tiled.hip.
 bool tiled_supported(int nbx, int nby);
-size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype);
+// acc: the records also hold the stretched origins' TwoSum residuals
+size_t tiled_ws_bytes(int nbx, int nby, int64_t ncells, int dtype, bool acc);
 // filler_map (nullable): uniform fillers' contributions go there instead of
 // `map`, which then holds the other cells' only (requires uniform fillers).
 // filler_ranges (nullable, device int[ntiles + 1]): fillers kept in tile
 // order, scattered (and later gathered) from their positions by tile range
 // instead of being binned.
+// zero_map: clear `map` first (with the tiling counters, in one launch);
+// filler_map, when given, is always cleared there.
 template <typename T>
 int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_t n_fill,
                   char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
-                  const int* filler_ranges, hipStream_t s);
+                  const int* filler_ranges, bool zero_map, hipStream_t s);
 template <typename T>
 int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
                  const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,

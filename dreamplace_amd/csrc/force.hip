@@ -38,10 +38,12 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
       return;
     }
     T gx = 0, gy = 0;
+    const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+    const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
     for (int k = c.bxl; k < c.bxh; ++k) {
-      T px = tri(c.x, c.sx, a.xl, k, a.bsx);
+      T px = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, k - c.bxl, a.bsx);
       for (int h = c.byl; h < c.byh; ++h) {
-        T area = px * tri(c.y, c.sy, a.yl, h, a.bsy);
+        T area = px * overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, h - c.byl, a.bsy);
         int64_t idx = (int64_t)k * a.nby + h;
         gx += area * fx[idx];
         gy += area * fy[idx];
@@ -51,10 +53,12 @@ __global__ void __launch_bounds__(256) k_force_small(TriArgs<T> a, const T* __re
     return;
   }
   T px[3], py[3];
+  const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+  const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    px[j] = tri(c.x, c.sx, a.xl, c.bxl + j, a.bsx);
-    py[j] = tri(c.y, c.sy, a.yl, c.byl + j, a.bsy);
+    px[j] = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, j, a.bsx);
+    py[j] = overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, j, a.bsy);
   }
   T gx = 0, gy = 0;
 #pragma unroll
@@ -94,9 +98,12 @@ __global__ void __launch_bounds__(256) k_force_large(TriArgs<T> a, const T* __re
     int nx = c.bxh - c.bxl, ny = c.byh - c.byl;
     int total = nx * ny;
     T gx = 0, gy = 0;
+    const T dx = rel_origin(c.x, c.ex, a.xl, c.bxl, a.bsx);
+    const T dy = rel_origin(c.y, c.ey, a.yl, c.byl, a.bsy);
     for (int t = lane; t < total; t += 64) {
       int k = c.bxl + t / ny, h = c.byl + t % ny;
-      T area = tri(c.x, c.sx, a.xl, k, a.bsx) * tri(c.y, c.sy, a.yl, h, a.bsy);
+      T area = overlap(a.acc, c.x, dx, c.sx, a.xl, c.bxl, k - c.bxl, a.bsx) *
+               overlap(a.acc, c.y, dy, c.sy, a.yl, c.byl, h - c.byl, a.bsy);
       int64_t idx = (int64_t)k * a.nby + h;
       gx += area * fx[idx];
       gy += area * fy[idx];
@@ -209,6 +216,8 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
          "dp_electric_force: bad large_capacity");
   hipStream_t s = S(stream);
   const int64_t nb = (int64_t)num_bins_x * num_bins_y;
+  const int acc = (algorithm & DP_ARITH_BIN_RELATIVE) != 0;
+  algorithm &= ~DP_ARITH_BIN_RELATIVE;
   DP_ARG(algorithm == DP_SCATTER_GLOBAL || algorithm == DP_SCATTER_TILED,
          "dp_electric_force: bad algorithm %d", algorithm);
   DP_ARG(grad_layout == DP_LAYOUT_SPLIT || grad_layout == DP_LAYOUT_INTERLEAVED,
@@ -226,6 +235,7 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                     offset_x, offset_y, ratio, xl, yl, bin_size_x, bin_size_y,
                                     num_bins_x, num_bins_y, 1, uniform_filler,
                                     num_nodes - num_filler_nodes);
+    a.acc = acc;
     if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y))
       return gather_tiled<T>(a, num_nodes, num_movable_nodes, num_filler_nodes,
                              (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,

@@ -671,10 +671,15 @@ __global__ void k_scale_centre(const T* in, int64_t n, double s, const double* s
 
 // Spectral multipliers of electric_potential.py:504-519, evaluated in FP64
 // per element: wu = 2 pi u/M, wv = 2 pi v/N * (bx/by), K = 1/(wu^2+wv^2),
-// K[0,0] = 0.  Writes a*wu*K/2, a*wv*K/2 and (optionally) a*K.
+// K[0,0] = 0.  Writes a*wu*K/2, a*wv*K/2 and (optionally) a*K.  With
+// `partials` it also forms the per-workgroup partials of the spectral energy
+// (MN/4) sum c_u c_v K a^2 (FP64) in the same pass.
 template <typename T>
-__global__ void k_spectral_weights(const T* a, int M, int N, double aspect, T* ax, T* ay, T* ak) {
+__global__ void __launch_bounds__(kReduceBlock) k_spectral_weights(
+    const T* a, int M, int N, double aspect, T* ax, T* ay, T* ak, double* partials) {
+  __shared__ double smem[kReduceBlock / 64];
   const int64_t n = (int64_t)M * N;
+  double acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     int u = (int)(i / N), v = (int)(i % N);
@@ -686,25 +691,12 @@ __global__ void k_spectral_weights(const T* a, int M, int N, double aspect, T* a
     ax[i] = av * (T)(wu * inv * 0.5);
     ay[i] = av * (T)(wv * inv * 0.5);
     if (ak) ak[i] = av * (T)inv;
+    if (partials && !(u == 0 && v == 0)) {
+      const double ad = (double)av;
+      acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * ad * ad / w2;
+    }
   }
-}
-
-// Spectral energy partials: (MN/4) sum c_u c_v K a^2, FP64.
-template <typename T>
-__global__ void __launch_bounds__(kReduceBlock) k_spectral_energy(const T* a, int M, int N,
-                                                                  double aspect, double* partials) {
-  __shared__ double smem[kReduceBlock / 64];
-  const int64_t n = (int64_t)M * N;
-  double acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int u = (int)(i / N), v = (int)(i % N);
-    if (u == 0 && v == 0) continue;
-    double wu = (2.0 * M_PI / M) * u;
-    double wv = (2.0 * M_PI / N) * v * aspect;
-    double av = (double)a[i];
-    acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * av * av / (wu * wu + wv * wv);
-  }
+  if (!partials) return;  // uniform: the whole grid returns
   double bs = block_sum<kReduceBlock>(acc, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
 }
@@ -828,17 +820,16 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
     return DP_OK;
   }));
   DP_CHECK_RC(transform(precision, plan, M, N, rho, auv, tmp, L_DCT, R_DCT, s));
-  int eblk = 0;
+  int wblk = 0;
   DP_CHECK_RC(DP_DISPATCH(dt, "dp_poisson_solve", [&] {
-    hipLaunchKernelGGL(k_spectral_weights<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const T*)auv, M, N, aspect, (T*)ax, (T*)ay, want_phi ? (T*)ak : nullptr);
+    // without the potential map the energy is spectral, formed in this pass
+    const bool spectral_energy = energy && !want_phi;
+    wblk = (int)(ceil_div(n, kReduceBlock) < kReduceMaxBlocks ? ceil_div(n, kReduceBlock)
+                                                                        : kReduceMaxBlocks);
+    hipLaunchKernelGGL(k_spectral_weights<T>, dim3(wblk), dim3(kReduceBlock), 0, s,
+                       (const T*)auv, M, N, aspect, (T*)ax, (T*)ay, want_phi ? (T*)ak : nullptr,
+                       spectral_energy ? part : nullptr);
     DP_LAUNCHED("k_spectral_weights");
-    if (energy && !want_phi) {
-      eblk = (int)(ceil_div(n, kReduceBlock) < 1024 ? ceil_div(n, kReduceBlock) : 1024);
-      hipLaunchKernelGGL(k_spectral_energy<T>, dim3(eblk), dim3(kReduceBlock), 0, s,
-                         (const T*)auv, M, N, aspect, part);
-      DP_LAUNCHED("k_spectral_energy");
-    }
     return DP_OK;
   }));
   // stage 1 of the inverse transforms (one batched launch):
@@ -867,7 +858,7 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
   int nblk = 0;
   DP_CHECK_RC(launch_gemm(precision, bt, count, s, &nblk));
   if (energy) {
-    DP_CHECK_RC(finish_reduction(part, want_phi ? nblk : eblk, 1, nullptr, energy, dt, s));
+    DP_CHECK_RC(finish_reduction(part, want_phi ? nblk : wblk, 1, nullptr, energy, dt, s));
   }
   return DP_OK;
 }

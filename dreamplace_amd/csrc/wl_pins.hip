@@ -134,6 +134,27 @@ __global__ void __launch_bounds__(256) k_pin_push(const T* __restrict__ x, const
   pin_xy[dst[i]] = v;
 }
 
+// Wirelength arithmetic of the pin-parallel kernels.
+//   kWaRef: d/dx_p = (a+ + b+ x_p) e+_p - (a- + b- x_p) e-_p with
+//     b+ = (1/g)/S+, a+ = (1 - b+ XS+)/S+ (XS+ = sum x e+), and the value
+//     XS+/S+ - XS-/S- (weighted_average_wirelength_merged.cpp:85-111).
+//   kWaCentred: the same function with x measured from the net's extremes,
+//     u+ = x - x_max and u- = x - x_min (exact in floating point for a net's
+//     pins, which lie within a factor of two of each other or straddle 0 by
+//     less than a bin), C+ = sum u+ e+ / S+, C- = sum u- e- / S-:
+//       value  = (x_max - x_min) + C+ - C-
+//       d/dx_p = (e+_p/S+)(1 + (u+_p - C+)/g) - (e-_p/S-)(1 - (u-_p - C-)/g).
+//     In exact arithmetic both are the same; the reference form cancels
+//     b+ x_p (~ x/g, thousands at die coordinates) against a+, which costs
+//     FP32 about 1e-5 relative in the gradient, while every term here is of
+//     the order of the net's span.
+constexpr int kWaRef = 0, kWaLse = 1, kWaCentred = 2;
+// kWaCentred groups whose first (largest) net has more pins than this reduce
+// their per-net extremes and sums across lanes in six shuffle steps instead
+// of the head lane's serial loops over the net's pins (the order of the sums
+// then differs from the reference's pin order, which only kWaRef keeps).
+constexpr int kTreeDeg = 8;
+
 // A pin's four exponentials as one aligned LDS word.
 template <typename T>
 struct alignas(4 * sizeof(T)) Exp4 {
@@ -188,9 +209,12 @@ __device__ __forceinline__ void load_group(GroupLoad<T>& L, int g, int ngroups, 
 // workgroup sum.  Each wavefront issues the loads of its next group before it
 // evaluates the current one, so that their latency overlaps the evaluation
 // (244 -> 206 us per call at 9.6M pins, the same bits).
-// kLse: the log-sum-exp model (logsumexp_wirelength_merged.cpp:14-85) on the
-// same skeleton; g_p then points at gamma instead of 1/gamma.
-template <typename T, bool kLse>
+// kMode: kWaRef, the reference's merged WA arithmetic; kWaLse, the
+// log-sum-exp model (logsumexp_wirelength_merged.cpp:14-85) on the same
+// skeleton, g_p then pointing at gamma instead of 1/gamma; kWaCentred, the
+// same WA model with every sum and derivative taken relative to the net's
+// extremes (see wa_centred below).
+template <typename T, int kMode>
 __global__ void __launch_bounds__(256) k_wa_pins(
     const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int* __restrict__ group_start, int ngroups, const T* __restrict__ w,
@@ -207,6 +231,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
   V* s_xy = sh_xy[wv];
   Exp4<T>* s_e = sh_e[wv];
   T(*s_c)[64] = sh_c[wv];
+  constexpr bool kLse = kMode == kWaLse;
   const T gam = kLse ? *g_p : T(0);
   const T ig = kLse ? T(1) / gam : *g_p;
   // weights (and the live flag) per pin slot when the caller prepared them
@@ -250,8 +275,38 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     }
     wave_sync();
     const bool is_head = on && lane == head;
+    // nets are packed largest first: lane 0 holds the group's largest degree
+    const bool tree = kMode == kWaCentred && __builtin_amdgcn_readfirstlane(deg) > kTreeDeg;
+    const int seg_end = head + deg;  // one past this lane's net's last lane
     T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
-    if (is_head && (live || net_hpwl)) {
+    if (tree) {
+      // segmented max / min: after the step of offset o, lane l holds the
+      // extreme over lanes [l, min(l + 2o, seg_end)); the head ends with its net's
+      T a = on ? x : -big_value<T>(), b = on ? x : big_value<T>();
+      T c = on ? y : -big_value<T>(), dd = on ? y : big_value<T>();
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const T a2 = __shfl_down(a, o, 64), b2 = __shfl_down(b, o, 64);
+        const T c2 = __shfl_down(c, o, 64), d2 = __shfl_down(dd, o, 64);
+        if (lane + o < seg_end) {
+          a = a2 > a ? a2 : a;
+          b = b2 < b ? b2 : b;
+          c = c2 > c ? c2 : c;
+          dd = d2 < dd ? d2 : dd;
+        }
+      }
+      if (is_head && (live || net_hpwl)) {
+        xmax = a;
+        xmin = b;
+        ymax = c;
+        ymin = dd;
+        s_c[0][lane] = xmax;
+        s_c[1][lane] = xmin;
+        s_c[2][lane] = ymax;
+        s_c[3][lane] = ymin;
+        if (net_hpwl) net_hpwl[e] = xmax - xmin + ymax - ymin;
+      }
+    } else if (is_head && (live || net_hpwl)) {
       for (int k = head; k < head + deg; ++k) {
         const V v = s_xy[k];
         const T xx = v.x, yy = v.y;
@@ -269,11 +324,23 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     }
     wave_sync();
     T ex = 0, enx = 0, ey = 0, eny = 0;
+    T ux = 0, unx = 0, uy = 0, uny = 0;  // kWaCentred: x - x_max, x - x_min, ...
     if (on && live) {
-      ex = exp((x - s_c[0][head]) * ig);
-      enx = exp((s_c[1][head] - x) * ig);
-      ey = exp((y - s_c[2][head]) * ig);
-      eny = exp((s_c[3][head] - y) * ig);
+      if (kMode == kWaCentred) {
+        ux = x - s_c[0][head];
+        unx = x - s_c[1][head];
+        uy = y - s_c[2][head];
+        uny = y - s_c[3][head];
+        ex = exp(ux * ig);
+        enx = exp(-unx * ig);
+        ey = exp(uy * ig);
+        eny = exp(-uny * ig);
+      } else {
+        ex = exp((x - s_c[0][head]) * ig);
+        enx = exp((s_c[1][head] - x) * ig);
+        ey = exp((y - s_c[2][head]) * ig);
+        eny = exp((s_c[3][head] - y) * ig);
+      }
       Exp4<T> q;
       q.a0 = ex;
       q.a1 = enx;
@@ -299,7 +366,66 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       s_c[2][lane] = (T)(1.0 / fs);
       s_c[3][lane] = (T)(1.0 / fns);
     }
-    if (!kLse && is_head && live) {
+    T sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // kWaCentred, tree: this lane's partial sums
+    if (kMode == kWaCentred && tree) {
+      sum[0] = ux * ex;
+      sum[1] = unx * enx;
+      sum[2] = ex;
+      sum[3] = enx;
+      sum[4] = uy * ey;
+      sum[5] = uny * eny;
+      sum[6] = ey;
+      sum[7] = eny;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const bool in = lane + o < seg_end;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const T t = __shfl_down(sum[k], o, 64);
+          if (in) sum[k] += t;
+        }
+      }
+    }
+    if (kMode == kWaCentred && is_head && live) {
+      T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
+      if (tree) {
+        xs = sum[0];
+        xns = sum[1];
+        es = sum[2];
+        ens = sum[3];
+        ys = sum[4];
+        yns = sum[5];
+        fs = sum[6];
+        fns = sum[7];
+      } else {
+        for (int k = head; k < head + deg; ++k) {
+          const V v = s_xy[k];
+          const Exp4<T> q = s_e[k];
+          const T a0 = q.a0, a1 = q.a1, a2 = q.a2, a3 = q.a3;
+          xs += (v.x - xmax) * a0;
+          xns += (v.x - xmin) * a1;
+          es += a0;
+          ens += a1;
+          ys += (v.y - ymax) * a2;
+          yns += (v.y - ymin) * a3;
+          fs += a2;
+          fns += a3;
+        }
+      }
+      const T cx = xs / es, cnx = xns / ens, cy = ys / fs, cny = yns / fns;
+      const T wl = ((xmax - xmin) + cx - cnx) + ((ymax - ymin) + cy - cny);
+      if (partial) partial[e] = wl;
+      contrib += (double)(weighted ? wl * wn : wl);
+      s_c[0][lane] = T(1) / es;
+      s_c[1][lane] = cx;
+      s_c[2][lane] = T(1) / ens;
+      s_c[3][lane] = cnx;
+      s_c[4][lane] = T(1) / fs;
+      s_c[5][lane] = cy;
+      s_c[6][lane] = T(1) / fns;
+      s_c[7][lane] = cny;
+    }
+    if (kMode == kWaRef && is_head && live) {
       T xs = 0, xns = 0, es = 0, ens = 0, ys = 0, yns = 0, fs = 0, fns = 0;
       for (int k = head; k < head + deg; ++k) {
         const V v = s_xy[k];
@@ -335,6 +461,11 @@ __global__ void __launch_bounds__(256) k_wa_pins(
         if (kLse) {
           gx = ex * s_c[0][head] - enx * s_c[1][head];
           gy = ey * s_c[2][head] - eny * s_c[3][head];
+        } else if (kMode == kWaCentred) {
+          gx = (ex * s_c[0][head]) * (T(1) + (ux - s_c[1][head]) * ig) -
+               (enx * s_c[2][head]) * (T(1) - (unx - s_c[3][head]) * ig);
+          gy = (ey * s_c[4][head]) * (T(1) + (uy - s_c[5][head]) * ig) -
+               (eny * s_c[6][head]) * (T(1) - (uny - s_c[7][head]) * ig);
         } else {
           gx = (s_c[0][head] + s_c[1][head] * x) * ex - (s_c[2][head] + s_c[3][head] * x) * enx;
           gy = (s_c[4][head] + s_c[5][head] * y) * ey - (s_c[6][head] + s_c[7][head] * y) * eny;
@@ -354,7 +485,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
 
 // Nets of more than 64 pins: one wavefront per net; big[q] = {net, first
 // group-order slot, degree, 0}.
-template <typename T, bool kLse>
+template <typename T, int kMode>
 __global__ void __launch_bounds__(256) k_wa_big(
     const typename Vec2<T>::type* __restrict__ pin_xy, const PinMeta* __restrict__ meta,
     const int4* __restrict__ big, int nbig, const T* __restrict__ w,
@@ -364,6 +495,8 @@ __global__ void __launch_bounds__(256) k_wa_big(
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  constexpr bool kLse = kMode == kWaLse;
+  constexpr bool kCen = kMode == kWaCentred;
   const T gam = kLse ? *g_p : T(0);
   const T ig = kLse ? T(1) / gam : *g_p;
   double contrib = 0;
@@ -390,12 +523,12 @@ __global__ void __launch_bounds__(256) k_wa_big(
       const T xx = v.x, yy = v.y;
       const T a0 = exp((xx - xmax) * ig), a1 = exp((xmin - xx) * ig);
       const T a2 = exp((yy - ymax) * ig), a3 = exp((ymin - yy) * ig);
-      xs += xx * a0;
-      xns += xx * a1;
+      xs += (kCen ? xx - xmax : xx) * a0;
+      xns += (kCen ? xx - xmin : xx) * a1;
       es += a0;
       ens += a1;
-      ys += yy * a2;
-      yns += yy * a3;
+      ys += (kCen ? yy - ymax : yy) * a2;
+      yns += (kCen ? yy - ymin : yy) * a3;
       fs += a2;
       fns += a3;
     }
@@ -407,8 +540,9 @@ __global__ void __launch_bounds__(256) k_wa_big(
     yns = wave_sum(yns);
     fs = wave_sum(fs);
     fns = wave_sum(fns);
-    const T wl = kLse ? (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin
-                      : xs / es - xns / ens + ys / fs - yns / fns;
+    const T wl = kLse   ? (log(es) + log(ens)) * gam + xmax - xmin + (log(fs) + log(fns)) * gam + ymax - ymin
+                 : kCen ? ((xmax - xmin) + xs / es - xns / ens) + ((ymax - ymin) + ys / fs - yns / fns)
+                        : xs / es - xns / ens + ys / fs - yns / fns;
     if (lane == 0) {
       if (partial && live) partial[e] = wl;
       if (live) contrib += (double)(w ? wl * w[e] : wl);
@@ -429,6 +563,12 @@ __global__ void __launch_bounds__(256) k_wa_big(
         if (kLse) {
           gx = exp((xx - xmax) * ig) * ax - exp((xmin - xx) * ig) * anx;
           gy = exp((yy - ymax) * ig) * ay - exp((ymin - yy) * ig) * any;
+        } else if (kCen) {
+          // the centred form of the packed kernel: 1/S and C = sum u e / S
+          gx = (exp((xx - xmax) * ig) * (T(1) / es)) * (T(1) + ((xx - xmax) - xs / es) * ig) -
+               (exp((xmin - xx) * ig) * (T(1) / ens)) * (T(1) - ((xx - xmin) - xns / ens) * ig);
+          gy = (exp((yy - ymax) * ig) * (T(1) / fs)) * (T(1) + ((yy - ymax) - ys / fs) * ig) -
+               (exp((ymin - yy) * ig) * (T(1) / fns)) * (T(1) - ((yy - ymin) - yns / fns) * ig);
         } else {
           gx = (ax + bx * xx) * exp((xx - xmax) * ig) - (anx + bnx * xx) * exp((xmin - xx) * ig);
           gy = (ay + by * yy) * exp((yy - ymax) * ig) - (any + bny * yy) * exp((ymin - yy) * ig);
@@ -625,7 +765,7 @@ size_t dp_wa_pins_workspace_bytes(int64_t num_pins, int dtype) {
   return reduce_slot_bytes() + align256(eb * (size_t)(num_pins > 0 ? num_pins : 1));
 }
 
-static int forward_pins(bool lse, const char* name, int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+static int forward_pins(int mode, const char* name, int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
                        const int32_t* slot_node, const void* node_offsets,
                        const int32_t* node_dst, const int32_t* pin_meta,
                        const int32_t* group_start, int64_t num_groups, const int32_t* big_nets,
@@ -662,8 +802,12 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
                                    (const int*)node_dst, (V*)pin_xy));
       DP_LAUNCHED("k_pin_push");
     }
-    auto packed = lse ? k_wa_pins<T, true> : k_wa_pins<T, false>;
-    auto bignet = lse ? k_wa_big<T, true> : k_wa_big<T, false>;
+    auto packed = mode == kWaLse       ? k_wa_pins<T, kWaLse>
+                  : mode == kWaCentred ? k_wa_pins<T, kWaCentred>
+                                       : k_wa_pins<T, kWaRef>;
+    auto bignet = mode == kWaLse       ? k_wa_big<T, kWaLse>
+                  : mode == kWaCentred ? k_wa_big<T, kWaCentred>
+                                       : k_wa_big<T, kWaRef>;
     DP_PROBED(DP_KERNEL_WA_FORWARD, s,
               hipLaunchKernelGGL(packed, dim3(g1), dim3(256), 0, s, (const V*)pin_xy,
                                  (const PinMeta*)pin_meta, (const int*)group_start, (int)num_groups,
@@ -671,6 +815,8 @@ static int forward_pins(bool lse, const char* name, int dtype, const void* pos, 
                                  (const T*)g_p, (T*)partial_wl, (T*)net_hpwl,
                                  (T*)grad_node_order, partials));
     DP_LAUNCHED("k_wa_pins");
+    // without nets above 64 pins the big-net kernel is not launched
+    if (num_big == 0) return finish_reduction(partials, g1, 1, nullptr, wl_out, dtype, s);
     hipLaunchKernelGGL(bignet, dim3(g2), dim3(256), 0, s, (const V*)pin_xy,
                        (const PinMeta*)pin_meta, (const int4*)big_nets, (int)num_big,
                        (const T*)net_weights, net_mask, (const T*)g_p, (T*)partial_wl,
@@ -688,10 +834,25 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
                        void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
                        void* workspace, void* stream) {
-  return forward_pins(false, "dp_wa_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
+  return forward_pins(kWaRef, "dp_wa_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
                       num_nets, net_weights, net_mask, slot_weights, inv_gamma, partial_wl,
                       net_hpwl, grad_node_order, wl_out, workspace, stream);
+}
+
+int dp_wa_forward_pins_centred(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                               const int32_t* slot_node, const void* node_offsets,
+                               const int32_t* node_dst, const int32_t* pin_meta,
+                               const int32_t* group_start, int64_t num_groups,
+                               const int32_t* big_nets, int64_t num_big, int64_t num_nets,
+                               const void* net_weights, const uint8_t* net_mask,
+                               const void* slot_weights, const void* inv_gamma, void* partial_wl,
+                               void* net_hpwl, void* grad_node_order, void* wl_out,
+                               void* workspace, void* stream) {
+  return forward_pins(kWaCentred, "dp_wa_forward_pins_centred", dtype, pos, num_nodes, num_pins,
+                      slot_node, node_offsets, node_dst, pin_meta, group_start, num_groups,
+                      big_nets, num_big, num_nets, net_weights, net_mask, slot_weights, inv_gamma,
+                      partial_wl, net_hpwl, grad_node_order, wl_out, workspace, stream);
 }
 
 int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
@@ -702,7 +863,7 @@ int dp_lse_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t n
                         const uint8_t* net_mask, const void* slot_weights, const void* gamma,
                         void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
                         void* workspace, void* stream) {
-  return forward_pins(true, "dp_lse_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
+  return forward_pins(kWaLse, "dp_lse_forward_pins", dtype, pos, num_nodes, num_pins, slot_node,
                       node_offsets, node_dst, pin_meta, group_start, num_groups, big_nets, num_big,
                       num_nets, net_weights, net_mask, slot_weights, gamma, partial_wl,
                       net_hpwl, grad_node_order, wl_out, workspace, stream);

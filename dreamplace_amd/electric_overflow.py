@@ -28,8 +28,11 @@ class ElectricDensityMapFunction:
     Extra keyword arguments: ``convention`` ("cpu" follows
     electric_density_map.cpp:125-140, "gpu" follows
     electric_density_map_cuda_kernel.cu:433-463), ``algorithm`` (global |
-    tiled scatter), ``large_capacity``, ``workspace`` (a uint8 tensor), and
-    ``fixed_map_out`` (int64 tensor receiving the raw fixed-point sum).
+    tiled scatter), ``large_capacity``, ``workspace`` (a uint8 tensor),
+    ``fixed_map_out`` (int64 tensor receiving the raw fixed-point sum) and
+    ``arithmetic`` ("reference": the reference's overlaps, bit-exact with its
+    map; "bin_relative": overlaps measured from each cell's first bin, see
+    DP_ARITH_BIN_RELATIVE).
     """
 
     @staticmethod
@@ -40,7 +43,7 @@ class ElectricDensityMapFunction:
                 num_movable_impacted_bins_y, num_filler_impacted_bins_x,
                 num_filler_impacted_bins_y, deterministic_flag, sorted_node_map,
                 convention="cpu", algorithm="global", large_capacity=None, workspace=None,
-                fixed_map_out=None, uniform_filler=None):
+                fixed_map_out=None, uniform_filler=None, arithmetic="reference"):
         name = "ElectricDensityMapFunction"
         N.require_gpu(pos, node_size_x_clamped, initial_density_map, name=name)
         pos = pos.view(pos.numel())
@@ -52,7 +55,7 @@ class ElectricDensityMapFunction:
         nbx, nby = int(num_bins_x), int(num_bins_y)
         if large_capacity is None:
             large_capacity = num_nodes
-        alg = N.DP_SCATTER_TILED if algorithm == "tiled" else N.DP_SCATTER_GLOBAL
+        alg = algorithm_code(algorithm, arithmetic)
         nbytes = N.query("dp_density_map_workspace_bytes", nbx, nby, num_nodes, int(large_capacity),
                          alg, N.dtype_code(pos))
         if workspace is None or workspace.numel() < nbytes:
@@ -73,14 +76,27 @@ class ElectricDensityMapFunction:
         return out
 
 
+def algorithm_code(algorithm, arithmetic="reference"):
+    """The `algorithm` argument of dp_density_map / dp_electric_force."""
+    if arithmetic not in ("reference", "bin_relative"):
+        raise ValueError("arithmetic must be 'reference' or 'bin_relative', got %r" % (arithmetic,))
+    code = N.DP_SCATTER_TILED if algorithm == "tiled" else N.DP_SCATTER_GLOBAL
+    return code | (N.DP_ARITH_BIN_RELATIVE if arithmetic == "bin_relative" else 0)
+
+
 class ElectricOverflow(nn.Module):
-    """electric_overflow.py:117-278 on the HIP path."""
+    """electric_overflow.py:117-278 on the HIP path.  Extra keyword arguments
+    beyond the reference's: ``convention``, ``algorithm`` and ``arithmetic``
+    (see ElectricDensityMapFunction)."""
 
     def __init__(self, node_size_x, node_size_y, bin_center_x, bin_center_y, target_density, xl,
                  yl, xh, yh, bin_size_x, bin_size_y, num_movable_nodes, num_terminals,
                  num_filler_nodes, padding, deterministic_flag, sorted_node_map,
-                 movable_macro_mask=None, convention="cpu", algorithm="global"):
+                 movable_macro_mask=None, convention="cpu", algorithm="global",
+                 arithmetic="reference"):
         super(ElectricOverflow, self).__init__()
+        algorithm_code(algorithm, arithmetic)  # validates
+        self.arithmetic = arithmetic
         self.node_size_x = node_size_x
         self.node_size_y = node_size_y
         self.bin_center_x = bin_center_x
@@ -180,7 +196,7 @@ class ElectricOverflow(nn.Module):
 
     @property
     def algorithm_code(self):
-        return N.DP_SCATTER_TILED if self.algorithm == "tiled" else N.DP_SCATTER_GLOBAL
+        return algorithm_code(self.algorithm, self.arithmetic)
 
     def uniform_filler_arg(self):
         """The uniform_filler argument of dp_density_map / dp_electric_force."""
@@ -218,7 +234,8 @@ class ElectricOverflow(nn.Module):
                float(self.yh), float(self.bin_size_x), float(self.bin_size_y),
                int(self.num_movable_nodes), int(self.num_terminals), self.num_bins_x,
                self.num_bins_y, _f(self.target_density),
-               N.DP_CONVERT_CPU if self.convention == "cpu" else N.DP_CONVERT_GPU,
+               (N.DP_CONVERT_CPU if self.convention == "cpu" else N.DP_CONVERT_GPU)
+               | (N.DP_ARITH_BIN_RELATIVE if self.arithmetic == "bin_relative" else 0),
                N.ptr(self.workspace(pos.device)), N.ptr(out), N.stream_ptr(pos.device))
         self.initial_density_map = out
 
@@ -235,7 +252,7 @@ class ElectricOverflow(nn.Module):
             self.num_bins_x, self.num_bins_y, self.num_movable_impacted_bins_x,
             self.num_movable_impacted_bins_y, self.num_filler_impacted_bins_x,
             self.num_filler_impacted_bins_y, self.deterministic_flag, self.sorted_node_map,
-            convention=self.convention, algorithm=self.algorithm,
+            convention=self.convention, algorithm=self.algorithm, arithmetic=self.arithmetic,
             large_capacity=self.large_capacity,
             workspace=self.workspace(pos.device, overflow=not full), fixed_map_out=fixed_map_out,
             uniform_filler=self.uniform_filler if self.use_uniform_filler else None)

@@ -13,8 +13,9 @@ Additions (keyword-only, defaults reproduce the reference):
 ``precision`` ("fp64" | "fp32" | "bf16x3") for the spectral solve,
 ``energy_mode`` ("potential": materialise phi = IDCT2(...) as the reference
 does; "spectral": evaluate the same energy from the DCT coefficients without
-the fourth transform), ``convention`` and ``algorithm`` of the density map,
-and ``backward``: "force" (the reference's field gather) or "adjoint" (the
+the fourth transform), ``convention``, ``algorithm`` and ``arithmetic``
+("reference" | "bin_relative", DP_ARITH_BIN_RELATIVE) of the density map and
+the field gather, and ``backward``: "force" (the reference's field gather) or "adjoint" (the
 exact derivative of the returned energy, dp_energy_gradient; the forward
 then also materialises phi).
 Fence regions (``region_id`` / ``fence_regions``) are not supported and raise.
@@ -70,7 +71,7 @@ class ElectricPotential(ElectricOverflow):
                  movable_macro_mask=None, fast_mode=False, region_id=None, fence_regions=None,
                  node2fence_region_map=None, placedb=None, *, precision=None,
                  energy_mode="potential", convention="cpu", algorithm="global",
-                 backward="force"):
+                 backward="force", arithmetic="reference"):
         if region_id is not None or fence_regions is not None:
             raise NotImplementedError("dreamplace_amd.ElectricPotential: fence regions are not supported")
         super(ElectricPotential, self).__init__(
@@ -79,7 +80,8 @@ class ElectricPotential(ElectricOverflow):
             bin_size_x=bin_size_x, bin_size_y=bin_size_y, num_movable_nodes=num_movable_nodes,
             num_terminals=num_terminals, num_filler_nodes=num_filler_nodes, padding=padding,
             deterministic_flag=deterministic_flag, sorted_node_map=sorted_node_map,
-            movable_macro_mask=movable_macro_mask, convention=convention, algorithm=algorithm)
+            movable_macro_mask=movable_macro_mask, convention=convention, algorithm=algorithm,
+            arithmetic=arithmetic)
         self.fast_mode = fast_mode
         self.region_id = region_id
         self.fence_regions = fence_regions

@@ -279,7 +279,7 @@ def check_netlist(flat_netpin, netpin_start, pin2net, flat_node2pin, node2pin_st
 
 
 # tiles of 64x64 bins the tiled density kernels take (their LDS histograms)
-MAX_TILES = 2048
+MAX_TILES = 4096
 
 
 def can_track_metrics(schedule, algorithm, num_bins_x, num_bins_y, padding, uniform_fillers):
@@ -336,7 +336,8 @@ class PlacementObjective:
     def __init__(self, dd, gamma, density_weight=1.0, precision=None, fused=True,
                  energy_mode="potential", convention="cpu", algorithm="global", schedule=None,
                  keep_partial_wl=False, wirelength="weighted_average", overlap=True,
-                 track_metrics=False, density_backward="force"):
+                 track_metrics=False, density_backward="force", wa_arith="reference",
+                 density_arith="reference"):
         """keep_partial_wl: also store the per-net (pre-weight) wirelength in
         `partial_wl` (what the net-weight adjoint needs); the pins schedule
         skips that store otherwise.  wirelength: "weighted_average" or
@@ -353,7 +354,16 @@ class PlacementObjective:
         `metric_ptr` is the address of those positions; a caller that changes
         them in place clears it.  density_backward: "force" (the reference's
         field gather) or "adjoint" (the exact derivative of the energy,
-        dp_energy_gradient, from the potential map the solve then writes)."""
+        dp_energy_gradient, from the potential map the solve then writes).
+        wa_arith (pins schedule, weighted average): "reference" evaluates the
+        reference's merged (a + b x) e derivative; "centred" the same model
+        relative to each net's extremes (dp_wa_forward_pins_centred), which
+        keeps FP32 near its rounding level instead of ~2e-5.  density_arith:
+        "reference" or "bin_relative" overlaps in the density scatter, the
+        fixed map and the field gather (ElectricPotential's `arithmetic`)."""
+        if wa_arith not in ("reference", "centred"):
+            raise ValueError("wa_arith must be 'reference' or 'centred', got %r" % (wa_arith,))
+        self.wa_arith = wa_arith
         if density_backward not in ("force", "adjoint"):
             raise ValueError("density_backward must be 'force' or 'adjoint', got %r"
                              % (density_backward,))
@@ -372,12 +382,19 @@ class PlacementObjective:
         # "fused" (net-parallel fused kernels) or "unfused" (reference op order)
         self.schedule = schedule or ("fused" if fused else "unfused")
         self.pin_plan = PinPlan(dd) if self.schedule == "pins" else None
+        if wa_arith == "centred" and (self.lse or self.schedule != "pins"):
+            raise NotImplementedError("the centred WA arithmetic runs on the pins schedule "
+                                      "with the weighted-average model")
+        self._wa_entry = ("dp_lse_forward_pins" if self.lse else
+                          "dp_wa_forward_pins_centred" if wa_arith == "centred" else
+                          "dp_wa_forward_pins")
         self.density = ElectricPotential(
             dd.node_size_x, dd.node_size_y, dd.bin_center_x, dd.bin_center_y, dd.target_density,
             dd.xl, dd.yl, dd.xh, dd.yh, dd.bin_size_x, dd.bin_size_y, dd.num_movable_nodes,
             dd.num_terminals, dd.num_filler_nodes, 0, True, dd.sorted_node_map,
             movable_macro_mask=dd.movable_macro_mask, precision=precision,
-            energy_mode=energy_mode, convention=convention, algorithm=algorithm)
+            energy_mode=energy_mode, convention=convention, algorithm=algorithm,
+            arithmetic=density_arith)
         self.density.compute_initial_density_map(dd.pos)
         self.plan = self.density.plan(dt, dev)
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
@@ -650,7 +667,7 @@ class PlacementObjective:
                     self._ev_fork.record(main)
                     self.side.wait_event(self._ev_fork)
                     s_wl = N.ctypes.c_void_p(self.side.cuda_stream)
-                N.call("dp_lse_forward_pins" if self.lse else "dp_wa_forward_pins", code,
+                N.call(self._wa_entry, code,
                        N.ptr(pos), n, P, N.ptr(pp.slot_node), N.ptr(pp.node_offsets),
                        N.ptr(pp.node_dst), N.ptr(pp.pin_meta), N.ptr(pp.group_start),
                        pp.num_groups, N.ptr(pp.big), pp.num_big, E, N.ptr(self.w),

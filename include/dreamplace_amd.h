@@ -48,6 +48,21 @@ enum { DP_SPECTRAL_FP64 = 0, DP_SPECTRAL_FP32 = 1, DP_SPECTRAL_BF16X3 = 2 };
 /* Density scatter algorithm: per-cell global int64 atomics, or cells binned
  * into LDS-resident bin tiles first. */
 enum { DP_SCATTER_GLOBAL = 0, DP_SCATTER_TILED = 1 };
+/* Overlap arithmetic (no counterpart in the reference), a flag OR'ed into
+ * the `algorithm` argument of dp_density_map_workspace_bytes,
+ * dp_density_map, dp_density_map_ex and dp_electric_force, and into the
+ * `convention` argument of dp_fixed_density_map.  Without it every overlap
+ * is the reference's min(x + s, bin_xh) - max(x, bin_xl) in T
+ * (density_function.h:12-36), whose x + s rounds at the order of the die
+ * coordinates.  With it each cell's stretched origin x + offset is carried
+ * with its rounding residual (TwoSum) and measured from the lower edge of
+ * the cell's first bin, so every overlap is formed from operands of the
+ * order of a few bins: the same bins and the same fixed-point recipe, the
+ * FP32 map and force within ~1e-7 of FP64 instead of ~2e-5.  Maps and
+ * gradients are bit-identical across the global and tiled paths in either
+ * arithmetic; only the reference arithmetic is bit-exact with the
+ * reference. */
+enum { DP_ARITH_BIN_RELATIVE = 0x100 };
 /* Layout of a per-node gradient: DP_LAYOUT_SPLIT is the reference's [2N]
  * (all x, then all y); DP_LAYOUT_INTERLEAVED stores (x, y) of node i at
  * [2i, 2i+1], so one node is one 8- (FP32) or 16-byte (FP64) word. */
@@ -452,6 +467,26 @@ int dp_wa_forward_pins(int dtype, const void* pos, int64_t num_nodes, int64_t nu
                        const uint8_t* net_mask, const void* slot_weights, const void* inv_gamma,
                        void* partial_wl, void* net_hpwl, void* grad_node_order, void* wl_out,
                        void* workspace, void* stream);
+
+/* dp_wa_forward_pins with the same model evaluated relative to each net's
+ * extremes (no counterpart in the reference, which evaluates
+ * weighted_average_wirelength_merged.cpp:85-111's (a + b x) e form): with
+ * u+ = x - x_max, u- = x - x_min, S = sum e, C = sum u e / S per net and
+ * axis, WL = (x_max - x_min) + C+ - C- and
+ * dWL/dx_p = (e+_p/S+)(1 + (u+_p - C+)/g) - (e-_p/S-)(1 - (u-_p - C-)/g).
+ * The exponentials are the reference's bits; the value and the derivatives
+ * are the same function without the reference form's cancellation of
+ * x/gamma-sized terms, so FP32 keeps about 1e-7 relative instead of ~2e-5.
+ * Same arguments, plan and outputs as dp_wa_forward_pins. */
+int dp_wa_forward_pins_centred(int dtype, const void* pos, int64_t num_nodes, int64_t num_pins,
+                               const int32_t* slot_node, const void* node_offsets,
+                               const int32_t* node_dst, const int32_t* pin_meta,
+                               const int32_t* group_start, int64_t num_groups,
+                               const int32_t* big_nets, int64_t num_big, int64_t num_nets,
+                               const void* net_weights, const uint8_t* net_mask,
+                               const void* slot_weights, const void* inv_gamma, void* partial_wl,
+                               void* net_hpwl, void* grad_node_order, void* wl_out,
+                               void* workspace, void* stream);
 
 /* The log-sum-exp model on the same plan and skeleton: the chain
  * LSE(PinPos(pos)) of logsumexp_wirelength_merged.cpp:14-85 (CUDA twin

@@ -333,5 +333,131 @@ int64_t dpo_fixed_scale(double xl, double yl, double xh, double yh) {
     }                                                                                        \
   }
 
+/* Bin-relative overlap arithmetic: this build's DP_ARITH_BIN_RELATIVE option,
+ * which has no counterpart in the reference (parity of these functions is
+ * pinned by mathematics in tests/test_oracle.py: their FP64 maps and forces
+ * equal the reference arithmetic's to rounding, and their FP32 maps stay
+ * within 1e-6 of FP64 where the reference arithmetic's do not).  Bins and the
+ * fixed-point recipe are the reference's (electric_density_map.cpp:254-309);
+ * only the overlap changes: the stretched origin xs = x + offset is carried
+ * with its TwoSum residual err and measured from the lower edge of the
+ * cell's first bin, d = (xs - (xl + bxl*b)) + err, and the overlap with bin
+ * bxl + q is min(d + s, q*b + b) - max(d, q*b). */
+#define DPO_DEFINE_REL(T, SFX)                                                              \
+  static inline T twosum_err_##SFX(T a, T b, T s) {                                          \
+    T bb = s - a;                                                                            \
+    return (a - (s - bb)) + (b - bb);                                                        \
+  }                                                                                          \
+  static inline T tri_rel_##SFX(T d, T node_size, int q, T bin_size) {                       \
+    T lo_q = (T)q * bin_size;                                                                \
+    T e = d + node_size, be = lo_q + bin_size;                                               \
+    T hi = (be < e) ? be : e;                                                                \
+    T lo = (d < lo_q) ? lo_q : d;                                                            \
+    return hi - lo;                                                                          \
+  }                                                                                          \
+  void dpo_triangle_density_rel_##SFX(const T* xv, const T* yv, const T* wxc, const T* wyc,\
+                                      const T* offx, const T* offy, const T* ratio_v,        \
+                                      int64_t n, int nbx, int nby, double xl_d, double yl_d, \
+                                      double bsx_d, double bsy_d, int64_t scale,             \
+                                      int64_t* buf) {                                        \
+    const T xl = (T)xl_d, yl = (T)yl_d, bsx = (T)bsx_d, bsy = (T)bsy_d;                     \
+    const T inv_x = (T)(1.0 / bsx), inv_y = (T)(1.0 / bsy);                                  \
+    const T tscale = (T)scale;                                                               \
+    _Pragma("omp parallel for schedule(static)")                                            \
+    for (int64_t i = 0; i < n; ++i) {                                                        \
+      T sx = wxc[i], sy = wyc[i];                                                            \
+      T nx = xv[i] + offx[i], ny = yv[i] + offy[i];                                          \
+      T ex = twosum_err_##SFX(xv[i], offx[i], nx), ey = twosum_err_##SFX(yv[i], offy[i], ny);\
+      T r = ratio_v[i];                                                                      \
+      int bxl = imax((int)((nx - xl) * inv_x), 0);                                           \
+      int bxh = imin((int)((nx + sx - xl) * inv_x) + 1, nbx);                                \
+      int byl = imax((int)((ny - yl) * inv_y), 0);                                           \
+      int byh = imin((int)((ny + sy - yl) * inv_y) + 1, nby);                                \
+      T dx = (nx - (xl + (T)bxl * bsx)) + ex, dy = (ny - (yl + (T)byl * bsy)) + ey;          \
+      for (int k = bxl; k < bxh; ++k) {                                                      \
+        T pxr = tri_rel_##SFX(dx, sx, k - bxl, bsx) * r;                                     \
+        for (int h = byl; h < byh; ++h) {                                                    \
+          T area = pxr * tri_rel_##SFX(dy, sy, h - byl, bsy);                                \
+          int64_t sv = (int64_t)(area * tscale);                                             \
+          _Pragma("omp atomic")                                                              \
+          buf[(int64_t)k * nby + h] += sv;                                                   \
+        }                                                                                    \
+      }                                                                                      \
+    }                                                                                        \
+  }                                                                                          \
+  /* exact overlap of a fixed cell with bin (centre bc): d = (x - bc) + b/2, then         \
+   * min(d + s, b) - max(d, 0), clamped at 0 (density_function.h:19-36, flag true). */    \
+  static inline T exact_rel_##SFX(T x, T node_size, T bin_center, T bin_size) {             \
+    T d = (x - bin_center) + bin_size / 2;                                                   \
+    T e = d + node_size;                                                                     \
+    T hi = (bin_size < e) ? bin_size : e;                                                    \
+    T lo = (d < (T)0) ? (T)0 : d;                                                            \
+    T v = hi - lo;                                                                           \
+    return ((T)0 < v) ? v : (T)0;                                                            \
+  }                                                                                          \
+  void dpo_exact_density_rel_##SFX(const T* xv, const T* yv, const T* wx, const T* wy,     \
+                                   const T* bcx, const T* bcy, int64_t n, int nbx, int nby,  \
+                                   double xl_d, double yl_d, double bsx_d, double bsy_d,     \
+                                   int64_t scale, int64_t* buf) {                            \
+    const T xl = (T)xl_d, yl = (T)yl_d, bsx = (T)bsx_d, bsy = (T)bsy_d;                     \
+    const T tscale = (T)scale;                                                               \
+    _Pragma("omp parallel for schedule(static)")                                            \
+    for (int64_t i = 0; i < n; ++i) {                                                        \
+      T bxl_ = xv[i], byl_ = yv[i];                                                          \
+      T bxh_ = bxl_ + wx[i], byh_ = byl_ + wy[i];                                            \
+      int kxl = imax((int)((bxl_ - xl) / bsx), 0);                                           \
+      int kxh = imin((int)ceil((bxh_ - xl) / bsx) + 1, nbx);                                 \
+      int kyl = imax((int)((byl_ - yl) / bsy), 0);                                           \
+      int kyh = imin((int)ceil((byh_ - yl) / bsy) + 1, nby);                                 \
+      for (int k = kxl; k < kxh; ++k) {                                                      \
+        T px = exact_rel_##SFX(bxl_, wx[i], bcx[k], bsx);                                    \
+        for (int h = kyl; h < kyh; ++h) {                                                    \
+          T py = exact_rel_##SFX(byl_, wy[i], bcy[h], bsy);                                  \
+          T area = px * py;                                                                  \
+          int64_t sv = (int64_t)(area * tscale);                                             \
+          _Pragma("omp atomic")                                                              \
+          buf[(int64_t)k * nby + h] += sv;                                                   \
+        }                                                                                    \
+      }                                                                                      \
+    }                                                                                        \
+  }                                                                                          \
+  void dpo_electric_force_rel_##SFX(T grad, const T* fx, const T* fy, const T* xv,         \
+                                    const T* yv, const T* wxc, const T* wyc, const T* offx,  \
+                                    const T* offy, const T* ratio_v, int64_t n, int nbx,     \
+                                    int nby, double xl_d, double yl_d, double bsx_d,         \
+                                    double bsy_d, T* gxo, T* gyo) {                          \
+    const T xl = (T)xl_d, yl = (T)yl_d, bsx = (T)bsx_d, bsy = (T)bsy_d;                     \
+    const T inv_x = (T)(1.0 / bsx), inv_y = (T)(1.0 / bsy);                                  \
+    _Pragma("omp parallel for schedule(static)")                                            \
+    for (int64_t i = 0; i < n; ++i) {                                                        \
+      T sx = wxc[i], sy = wyc[i];                                                            \
+      T nx = xv[i] + offx[i], ny = yv[i] + offy[i];                                          \
+      T ex = twosum_err_##SFX(xv[i], offx[i], nx), ey = twosum_err_##SFX(yv[i], offy[i], ny);\
+      T r = ratio_v[i];                                                                      \
+      int bxl = imax((int)((nx - xl) * inv_x), 0);                                           \
+      int bxh = imin((int)((nx + sx - xl) * inv_x) + 1, nbx);                                \
+      int byl = imax((int)((ny - yl) * inv_y), 0);                                           \
+      int byh = imin((int)((ny + sy - yl) * inv_y) + 1, nby);                                \
+      T dx = (nx - (xl + (T)bxl * bsx)) + ex, dy = (ny - (yl + (T)byl * bsy)) + ey;          \
+      T gx = 0, gy = 0;                                                                      \
+      for (int k = bxl; k < bxh; ++k) {                                                      \
+        T px = tri_rel_##SFX(dx, sx, k - bxl, bsx);                                          \
+        for (int h = byl; h < byh; ++h) {                                                    \
+          T py = tri_rel_##SFX(dy, sy, h - byl, bsy);                                        \
+          T area = px * py;                                                                  \
+          int64_t idx = (int64_t)k * nby + h;                                                \
+          gx += area * fx[idx];                                                              \
+          gy += area * fy[idx];                                                              \
+        }                                                                                    \
+      }                                                                                      \
+      gx *= r;                                                                               \
+      gy *= r;                                                                               \
+      gxo[i] = gx * grad;                                                                    \
+      gyo[i] = gy * grad;                                                                    \
+    }                                                                                        \
+  }
+
 DPO_DEFINE(float, f32)
 DPO_DEFINE(double, f64)
+DPO_DEFINE_REL(float, f32)
+DPO_DEFINE_REL(double, f64)

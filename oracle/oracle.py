@@ -80,18 +80,28 @@ def fixed_scale(xl, yl, xh, yh):
     return int(lib().dpo_fixed_scale(float(xl), float(yl), float(xh), float(yh)))
 
 
+def _arith(arithmetic):
+    if arithmetic not in ("reference", "bin_relative"):
+        raise ValueError("arithmetic must be 'reference' or 'bin_relative'")
+    return "rel_" if arithmetic == "bin_relative" else ""
+
+
 def triangle_density_fixed(x, y, wxc, wyc, offx, offy, ratio, num_bins_x, num_bins_y,
-                           xl, yl, bin_size_x, bin_size_y, scale, buf=None):
+                           xl, yl, bin_size_x, bin_size_y, scale, buf=None,
+                           arithmetic="reference"):
     """int64 map of sum_(cell,bin) trunc(((px*ratio)*py)*scale) over the given cells.
     electric_density_map.cpp:254-309 + atomic_ops.h:29-45.
     Pinned by: oracle/_ref density_map (bit-exact), brute-force overlap, mass
-    conservation (tests/test_oracle.py)."""
+    conservation (tests/test_oracle.py).  arithmetic="bin_relative": the same
+    bins and recipe with bin-relative overlaps (dp_oracle.c DPO_DEFINE_REL; no
+    reference counterpart), pinned by FP64 agreement with the reference
+    arithmetic and FP32 accuracy against FP64 (tests/test_oracle.py)."""
     x = _c(x)
     sfx, _ = _sfx(x.dtype)
     arrs = [_c(a, x.dtype) for a in (x, y, wxc, wyc, offx, offy, ratio)]
     if buf is None:
         buf = np.zeros(num_bins_x * num_bins_y, dtype=np.int64)
-    getattr(lib(), "dpo_triangle_density_" + sfx)(
+    getattr(lib(), "dpo_triangle_density_" + _arith(arithmetic) + sfx)(
         *[_p(a) for a in arrs], ctypes.c_int64(len(x)), ctypes.c_int(num_bins_x),
         ctypes.c_int(num_bins_y), ctypes.c_double(xl), ctypes.c_double(yl),
         ctypes.c_double(bin_size_x), ctypes.c_double(bin_size_y), ctypes.c_int64(scale), _p(buf))
@@ -99,14 +109,26 @@ def triangle_density_fixed(x, y, wxc, wyc, offx, offy, ratio, num_bins_x, num_bi
 
 
 def exact_density_fixed(x, y, wx, wy, bin_center_x, bin_center_y, num_bins_x, num_bins_y,
-                        xl, yl, xh, yh, bin_size_x, bin_size_y, scale, fixed_flag=True, buf=None):
+                        xl, yl, xh, yh, bin_size_x, bin_size_y, scale, fixed_flag=True, buf=None,
+                        arithmetic="reference"):
     """int64 exact-overlap map of fixed cells; electric_density_map.cpp:311-358.
-    Pinned by: oracle/_ref fixed_density_map (bit-exact), brute force."""
+    Pinned by: oracle/_ref fixed_density_map (bit-exact), brute force.
+    arithmetic="bin_relative" (fixed_flag only): overlaps measured from each
+    bin's lower edge (dp_oracle.c DPO_DEFINE_REL)."""
     x = _c(x)
     sfx, _ = _sfx(x.dtype)
     arrs = [_c(a, x.dtype) for a in (x, y, wx, wy, bin_center_x, bin_center_y)]
     if buf is None:
         buf = np.zeros(num_bins_x * num_bins_y, dtype=np.int64)
+    if arithmetic == "bin_relative":
+        if not fixed_flag:
+            raise ValueError("the bin-relative exact overlap has no boundary stretch")
+        getattr(lib(), "dpo_exact_density_rel_" + sfx)(
+            *[_p(a) for a in arrs], ctypes.c_int64(len(x)), ctypes.c_int(num_bins_x),
+            ctypes.c_int(num_bins_y), *[ctypes.c_double(v) for v in (xl, yl, bin_size_x, bin_size_y)],
+            ctypes.c_int64(scale), _p(buf))
+        return buf
+    _arith(arithmetic)
     getattr(lib(), "dpo_exact_density_" + sfx)(
         *[_p(a) for a in arrs], ctypes.c_int64(len(x)), ctypes.c_int(num_bins_x),
         ctypes.c_int(num_bins_y), *[ctypes.c_double(v) for v in (xl, yl, xh, yh, bin_size_x, bin_size_y)],
@@ -185,7 +207,8 @@ def bin_centers(l, h, bin_size, dtype=np.float64):
     return c
 
 
-def fixed_density_map(dp, pos, bin_center_x, bin_center_y, deterministic=True):
+def fixed_density_map(dp, pos, bin_center_x, bin_center_y, deterministic=True,
+                      arithmetic="reference"):
     """initial_density_map: exact map of fixed cells, clamp_max(bin area), * t.
     electric_density_map.cpp:155-213 (deterministic branch) and
     electric_overflow.py:229-258.  Returns [Mx, My] in dp.dtype."""
@@ -196,14 +219,15 @@ def fixed_density_map(dp, pos, bin_center_x, bin_center_y, deterministic=True):
         buf = exact_density_fixed(pos[m0:m1], pos[n + m0:n + m1], dp.node_size_x[m0:m1],
                                   dp.node_size_y[m0:m1], bin_center_x, bin_center_y,
                                   dp.num_bins_x, dp.num_bins_y, dp.xl, dp.yl, dp.xh, dp.yh,
-                                  dp.bin_size_x, dp.bin_size_y, dp.scale, True)
+                                  dp.bin_size_x, dp.bin_size_y, dp.scale, True,
+                                  arithmetic=arithmetic)
         fixed_add_cpu(buf, dp.scale, out)
         np.minimum(out, dp.dtype.type(dp.bin_size_x * dp.bin_size_y), out=out)
     out *= dp.dtype.type(dp.target_density)
     return out.reshape(dp.num_bins_x, dp.num_bins_y)
 
 
-def density_fixed_sum(dp, pos, num_filler_nodes=None):
+def density_fixed_sum(dp, pos, num_filler_nodes=None, arithmetic="reference"):
     """int64 scatter of movables and fillers (no initial map)."""
     n = pos.size // 2
     nf = dp.num_filler_nodes if num_filler_nodes is None else num_filler_nodes
@@ -214,16 +238,18 @@ def density_fixed_sum(dp, pos, num_filler_nodes=None):
     for b, e in rng:
         triangle_density_fixed(pos[b:e], pos[n + b:n + e], dp.wxc[b:e], dp.wyc[b:e], dp.offx[b:e],
                                dp.offy[b:e], dp.ratio[b:e], dp.num_bins_x, dp.num_bins_y,
-                               dp.xl, dp.yl, dp.bin_size_x, dp.bin_size_y, dp.scale, buf)
+                               dp.xl, dp.yl, dp.bin_size_x, dp.bin_size_y, dp.scale, buf,
+                               arithmetic=arithmetic)
     return buf
 
 
-def density_map(dp, pos, initial_density_map, num_filler_nodes=None, convention="cpu"):
+def density_map(dp, pos, initial_density_map, num_filler_nodes=None, convention="cpu",
+                arithmetic="reference"):
     """Deterministic density map of movables+fillers over the initial map.
     convention "cpu": init + (double)sum*2^-32 -> T  (electric_density_map.cpp:125-140);
     convention "gpu": T(u64(quant(init) + sum)) * T(2^-32)
     (electric_density_map_cuda_kernel.cu:433-463)."""
-    buf = density_fixed_sum(dp, pos, num_filler_nodes)
+    buf = density_fixed_sum(dp, pos, num_filler_nodes, arithmetic=arithmetic)
     init = np.ascontiguousarray(initial_density_map, dtype=dp.dtype).reshape(-1)
     if convention == "cpu":
         out = init.copy()
@@ -234,7 +260,7 @@ def density_map(dp, pos, initial_density_map, num_filler_nodes=None, convention=
     return out.reshape(dp.num_bins_x, dp.num_bins_y)
 
 
-def electric_force(dp, pos, field_x, field_y, grad=1.0):
+def electric_force(dp, pos, field_x, field_y, grad=1.0, arithmetic="reference"):
     """-(electric_force(...))*grad for movables and fillers, 0 for fixed cells:
     the density op's backward (electric_potential.py:219-245,
     electric_force.cpp:76-172).  Returns [2N] in dp.dtype."""
@@ -252,7 +278,7 @@ def electric_force(dp, pos, field_x, field_y, grad=1.0):
         gy = np.zeros(e - b, dtype=T)
         arrs = [_c(a, T) for a in (pos[b:e], pos[n + b:n + e], dp.wxc[b:e], dp.wyc[b:e],
                                    dp.offx[b:e], dp.offy[b:e], dp.ratio[b:e])]
-        getattr(lib(), "dpo_electric_force_" + sfx)(
+        getattr(lib(), "dpo_electric_force_" + _arith(arithmetic) + sfx)(
             ct(grad), _p(fx), _p(fy), *[_p(a) for a in arrs], ctypes.c_int64(e - b),
             ctypes.c_int(dp.num_bins_x), ctypes.c_int(dp.num_bins_y), ctypes.c_double(dp.xl),
             ctypes.c_double(dp.yl), ctypes.c_double(dp.bin_size_x), ctypes.c_double(dp.bin_size_y),
@@ -537,7 +563,8 @@ def pin_weight_sum(net_weights, flat_nodepin, nodepin_start, pin2net_map, num_no
     return out
 
 
-def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_coeff=None):
+def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_coeff=None,
+                           arithmetic="reference"):
     """obj = WA(pin_pos(pos)) + lambda*energy and its gradient for a design
     dict (dreamplace_amd.synth layout), composed from the functions above as
     PlaceObj.obj_fn / obj_and_grad_fn compose the ops
@@ -545,7 +572,8 @@ def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_co
     With `density_factor` f and the quadratic penalty coefficient c
     (PlaceObj.py:308-318): obj = WA + (f*lambda) * E*(1 + c*E), whose
     derivative in E, g*(1 + c*E) + (g*E)*c with g = f*lambda, scales the field
-    gather.  Returns (obj, grad[2N], parts)."""
+    gather.  arithmetic: the overlap arithmetic of the maps and the force
+    ("reference" or "bin_relative").  Returns (obj, grad[2N], parts)."""
     dp = DensityParams(d["node_size_x"], d["node_size_y"], d["xl"], d["yl"], d["xh"], d["yh"],
                        d["bin_size_x"], d["bin_size_y"], d["num_movable_nodes"], d["num_terminals"],
                        d["num_filler_nodes"], d["target_density"], d["movable_macro_mask"])
@@ -554,8 +582,8 @@ def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_co
     pin_pos = pin_pos_forward(pos, d["pin_offset_x"], d["pin_offset_y"], d["pin2node_map"])
     wl, partial, gi = wa_merged_forward(pin_pos, d["flat_net2pin_map"], d["flat_net2pin_start_map"],
                                         d["net_weights"], d["net_mask_ignore_large_degrees"], gamma)
-    init = fixed_density_map(dp, pos, d["bin_center_x"], d["bin_center_y"])
-    dmap = density_map(dp, pos, init)
+    init = fixed_density_map(dp, pos, d["bin_center_x"], d["bin_center_y"], arithmetic=arithmetic)
+    dmap = density_map(dp, pos, init, arithmetic=arithmetic)
     sol = poisson_solve(dmap, d["bin_size_x"], d["bin_size_y"])
     g = density_weight
     energy = sol["energy"]
@@ -566,7 +594,8 @@ def objective_and_gradient(d, density_weight, gamma, density_factor=1.0, quad_co
             c = T.type(quad_coeff)
             g = g * (T.type(1) + c * e) + (g * e) * c
             energy = e * (T.type(1) + c * e)
-    dgrad = electric_force(dp, pos, sol["field_x"].astype(T), sol["field_y"].astype(T), grad=g)
+    dgrad = electric_force(dp, pos, sol["field_x"].astype(T), sol["field_y"].astype(T), grad=g,
+                           arithmetic=arithmetic)
     gpin = wa_merged_backward(1.0, gi, d["pin2net_map"], d["net_weights"],
                               d["net_mask_ignore_large_degrees"], d["pin_mask_ignore_fixed_macros"])
     wgrad = pin_pos_backward(gpin, d["flat_node2pin_map"], d["flat_node2pin_start_map"],

@@ -81,3 +81,31 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def shifted(d, dx, dy):
+    """A copy of design `d` translated by (dx, dy): positions, die and bin
+    centres move together, so the placement problem is the same up to
+    rounding, at coordinates the size of a large die's."""
+    d2 = synth.Design(d)
+    n = d.num_nodes
+    pos = d.pos.astype(np.float64).copy()
+    pos[:n] += dx
+    pos[n:] += dy
+    d2.pos = pos.astype(d.pos.dtype)
+    d2.xl, d2.xh = d.xl + dx, d.xh + dx
+    d2.yl, d2.yh = d.yl + dy, d.yh + dy
+    d2.bin_center_x = (d.bin_center_x.astype(np.float64) + dx).astype(d.bin_center_x.dtype)
+    d2.bin_center_y = (d.bin_center_y.astype(np.float64) + dy).astype(d.bin_center_y.dtype)
+    return d2
+
+
+def with_dtype(d, dtype):
+    """A copy of design `d` with every floating-point array cast to `dtype`
+    (FP32 -> FP64 keeps the FP32 values exactly)."""
+    d2 = synth.Design(d)
+    for k, v in d.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind == "f":
+            d2[k] = v.astype(dtype)
+    d2.dtype = np.dtype(dtype).name
+    return d2

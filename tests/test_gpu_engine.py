@@ -194,21 +194,23 @@ def test_engine_filler_tile_order(gpu, dtype):
     assert torch.equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-def test_tracked_metrics_equal_the_separate_passes(gpu, dtype):
+def test_tracked_metrics_equal_the_separate_passes(gpu, dtype, arith):
     """HPWL from the wirelength pass's per-net extremes and overflow from the
     objective's own scatter give the same bits as dp_hpwl on pin positions
     and ElectricOverflow's movables-only scatter, with nets above 64 pins and
     nets the WA mask drops (HPWL counts them)."""
     from dreamplace_amd import _native as N
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
-    print("case engine tracked metrics %s" % np.dtype(dtype).name, flush=True)
+    print("case engine tracked metrics %s %s" % (np.dtype(dtype).name, arith), flush=True)
     d = design(0, dtype=dtype, big_nets=20)
     assert (np.diff(d.flat_net2pin_start_map) >= 100).any()
     dd = DeviceDesign(d, gpu)
     dd.reorder_fillers()
     ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
-                            else "fp64", algorithm="tiled", schedule="pins", track_metrics=True)
+                            else "fp64", algorithm="tiled", schedule="pins", track_metrics=True,
+                            density_arith=arith)
     ob.evaluate(dd.pos)
     assert ob.metric_ptr == dd.pos.data_ptr()
     got = ob.metric_out.clone()
@@ -231,26 +233,28 @@ def test_tracked_metrics_equal_the_separate_passes(gpu, dtype):
     assert torch.equal(got, ref), (got, ref)
     # the objective's full map is unchanged by the split scatter
     ob2 = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision="bf16x3" if dtype == np.float32
-                             else "fp64", algorithm="tiled", schedule="pins")
+                             else "fp64", algorithm="tiled", schedule="pins", density_arith=arith)
     f2, g2 = ob2.evaluate(dd.pos)
     f1, g1 = ob.evaluate(dd.pos)
     assert torch.equal(ob.density_map, ob2.density_map)
     assert f1.item() == f2.item() and torch.equal(g1, g2)
 
 
+@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-def test_filler_ranges_are_exact(gpu, dtype):
+def test_filler_ranges_are_exact(gpu, dtype, arith):
     """Fillers taken by tile range instead of binned: the same density map,
     objective and gradient bits with freshly sorted fillers, after the fillers
     have moved (up to 3 bins: some leave their tile's window and take the
     global path), and with ranges that are not a partition (replaced by one
     range); with the movable-only map of the tracked metrics too."""
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
-    print("case engine filler ranges %s" % np.dtype(dtype).name, flush=True)
+    print("case engine filler ranges %s %s" % (np.dtype(dtype).name, arith), flush=True)
     d = design(1, dtype=dtype, scale=0.01)  # 102 x 102 bins: 2 x 2 tiles
     prec = "bf16x3" if dtype == np.float32 else "fp64"
     dd = DeviceDesign(d, gpu)
-    kw = dict(precision=prec, algorithm="tiled", schedule="pins", track_metrics=True)
+    kw = dict(precision=prec, algorithm="tiled", schedule="pins", track_metrics=True,
+              density_arith=arith)
     ref = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, **kw)
     rng = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, **kw)
     dd.reorder_fillers(origin_offset=rng.filler_origin_offset())

@@ -922,3 +922,99 @@ def test_net_weighting_known_answers(oracle):
     w, c = oracle.net_weighting("lilith", np.array([1.0, 2.0]), np.array([0.5, 0.0]),
                                 np.array([0, 2, 4]), 100, net_slack=np.array([-1.0, 1.0]), wns=0.0)
     assert c.tolist() == [0.5, 0.0] and w.tolist() == [1.5, 2.0]
+
+
+# ---------------------------------------------------------------------------
+# The bin-relative overlap arithmetic (DP_ARITH_BIN_RELATIVE; no reference
+# counterpart): the same function as the reference arithmetic, evaluated from
+# operands of the order of a bin.  Pinned by agreement with the reference
+# arithmetic in FP64 and by FP32 accuracy against FP64 at large coordinates.
+# ---------------------------------------------------------------------------
+def _shifted_small(dtype):
+    """Config 1's cells and bins (bin 10000/1024 units) on a 625-unit die
+    moved to (5000, 3000), where FP32 coordinates round at 2.4e-4 units;
+    every bin edge stays exact in FP32, as at configs 1 and 4."""
+    from _util import shifted, with_dtype
+    d = shifted(design(1, scale=0.00390625), 5000.0, 3000.0)
+    assert d.num_bins_x == 64 and d.bin_size_x == 10000.0 / 1024
+    return with_dtype(d, dtype)
+
+
+def _maps(oracle, d, arithmetic):
+    """(fixed map, movable + filler int64 sum as float64, params)."""
+    dp = density_params(oracle, d)
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y,
+                                    arithmetic=arithmetic)
+    s = oracle.density_fixed_sum(dp, d.pos, arithmetic=arithmetic).astype(np.float64) / dp.scale
+    return init, s, dp
+
+
+def _smooth_field(d, dtype):
+    u = np.linspace(0, 3 * np.pi, d.num_bins_x)[:, None]
+    v = np.linspace(0, 2 * np.pi, d.num_bins_y)[None, :]
+    return (np.sin(u) * np.cos(v) + 0.3).astype(dtype), (np.cos(2 * u) * np.sin(v) - 0.1).astype(dtype)
+
+
+def test_bin_relative_fp64_equals_reference_arithmetic(oracle):
+    """FP64: fixed map, density map and force of the bin-relative arithmetic
+    equal the reference arithmetic's to rounding (the same function)."""
+    d = _shifted_small(np.float64)
+    init_r, m_r, dp = _maps(oracle, d, "reference")
+    init_b, m_b, _ = _maps(oracle, d, "bin_relative")
+    assert rel_l2(init_b, init_r) <= 1e-12
+    assert rel_l2(m_b, m_r) <= 1e-12
+    fx, fy = _smooth_field(d, np.float64)
+    g_r = oracle.electric_force(dp, d.pos, fx, fy)
+    g_b = oracle.electric_force(dp, d.pos, fx, fy, arithmetic="bin_relative")
+    assert rel_l2(g_b, g_r) <= 1e-12
+
+
+def test_bin_relative_fp32_accuracy(oracle):
+    """FP32 at die coordinates ~5000 against FP64 on the same FP32 inputs:
+    the reference arithmetic's x + s rounds at ~2.4e-4 units (~2.5e-5 of a
+    bin); the bin-relative arithmetic keeps the maps and the force within
+    1e-6.  A form that drops the TwoSum residual (or measures from the die
+    origin) stays at the reference arithmetic's error and fails."""
+    from _util import with_dtype
+    d32 = _shifted_small(np.float32)
+    d64 = with_dtype(d32, np.float64)
+    init64, m64, dp64 = _maps(oracle, d64, "reference")
+    fx, fy = _smooth_field(d32, np.float32)
+    g64 = oracle.electric_force(dp64, d64.pos, fx.astype(np.float64), fy.astype(np.float64))
+    err = {}
+    for arith in ("reference", "bin_relative"):
+        init, m, dp = _maps(oracle, d32, arith)
+        g = oracle.electric_force(dp, d32.pos, fx, fy, arithmetic=arith)
+        err[arith] = (rel_l2(init, init64), rel_l2(m, m64), rel_l2(g, g64))
+    print("relative errors (fixed map, scatter, force):", err)
+    assert max(err["bin_relative"]) <= 1e-6
+    # the stretched cells' x + s rounds in the reference arithmetic; fixed
+    # blocks of integer size do not (x + w stays exact within a binade), so
+    # there both are at the map conversion's rounding
+    for k in (1, 2):
+        assert err["reference"][k] > 5 * err["bin_relative"][k]
+    assert err["bin_relative"][0] <= 1.01 * err["reference"][0]
+
+
+def test_bin_relative_mass_and_order(oracle):
+    """Cells inside the die: the bin-relative map sums to the cells' area, and
+    any permutation of the cells gives the identical int64 map."""
+    rng = np.random.default_rng(5)
+    die, nb = 64.0, 32
+    bs = die / nb
+    n = 500
+    w, h = rng.uniform(0.5, 6, n), rng.uniform(0.5, 6, n)
+    wxc, wyc = np.maximum(w, bs * math.sqrt(2)), np.maximum(h, bs * math.sqrt(2))
+    offx, offy = (w - wxc) / 2, (h - wyc) / 2
+    ratio = w * h / (wxc * wyc)
+    x = rng.uniform(-offx, die - wxc - offx)
+    y = rng.uniform(-offy, die - wyc - offy)
+    scale = oracle.fixed_scale(0, 0, die, die)
+    kw = dict(arithmetic="bin_relative")
+    buf = oracle.triangle_density_fixed(x, y, wxc, wyc, offx, offy, ratio, nb, nb, 0.0, 0.0, bs, bs,
+                                        scale, **kw)
+    assert abs(buf.sum() / scale - (w * h).sum()) <= 1e-6
+    p = rng.permutation(n)
+    buf2 = oracle.triangle_density_fixed(x[p], y[p], wxc[p], wyc[p], offx[p], offy[p], ratio[p], nb, nb,
+                                         0.0, 0.0, bs, bs, scale, **kw)
+    assert np.array_equal(buf, buf2)
