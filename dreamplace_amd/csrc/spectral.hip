@@ -23,6 +23,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 enum MatId { L_DCT = 0, L_IDCT = 1, L_IDXST = 2, R_DCT = 3, R_IDCT = 4, R_IDXST = 5 };
 
@@ -106,7 +107,8 @@ struct TileIdx {
 // contiguous range [x*per, (x+1)*per) of a banded raster over all tiles:
 // its tiles form a compact block of rows and columns and the A and B slabs
 // they read stay in that die's L2.
-__device__ __forceinline__ TileIdx map_tile(const GemmBatch& bt) {
+template <class Batch>
+__device__ __forceinline__ TileIdx map_tile(const Batch& bt) {
   const int T = bt.TY * bt.TX, total = T * bt.count;
   const int per = (total + kXcds - 1) / kXcds;
   const int b = blockIdx.x;
@@ -471,6 +473,214 @@ __global__ void __launch_bounds__(512) k_gemm_bf16x3_t128(GemmBatch bt) {
   }
 }
 
+// ---- BF16x3 on pre-split operands (the Poisson solve) -----------------------
+// The operands arrive as bf16 hi/lo planes, both K-contiguous: A as [M][K]
+// rows and B as [N][K] (the transpose of the K x N operand).  Their producers
+// split each FP32 value v once, hi = bf16(v), lo = bf16(v - hi), exactly as
+// split8 does in the kernels above: the centring (rho), the spectral weights,
+// the first GEMM of each transform (which writes its product transposed, as
+// the second GEMM's B) and the plan (the transform matrices).  A K tile is
+// then a plain copy into LDS, with no conversion work in the GEMM.  The MFMA
+// sequence per output element (K order, lo*hi, hi*lo, hi*hi, the two K
+// halves added at the end) is the in-kernel-split kernels', so the products
+// have their bits.
+//   BM = 64: 64 x 64 tiles, four 32 x 32 wavefronts per K half, two LDS
+//   stages: the next K tile is written while the current one feeds the
+//   matrix cores, one barrier per K step.
+//   BM = 128: 128 x 128 tiles (grids of >= 256 of them), four 64 x 64
+//   wavefronts per K half, one LDS stage (two would exceed the LDS).
+struct B3Desc {
+  const __bf16* A;  // hi plane; lo plane at A + B3Batch::plane_a
+  const __bf16* B;  // hi plane; lo plane at B + B3Batch::plane_b
+  float* C;         // [M][N] FP32 (kOutT false)
+  __bf16* CT;       // [N][M] hi plane, lo plane at CT + M*N (kOutT)
+  const float* R;   // nullable: energy reference map [M][N] (kOutT false)
+  double* partials;
+};
+struct B3Batch {
+  B3Desc g[3];
+  int M, N, K;
+  int64_t plane_a, plane_b;
+  float alpha;
+  int TY, TX, count;
+};
+
+__device__ __forceinline__ const B3Desc& desc_of3(const B3Batch& bt, int g) {
+  return g == 0 ? bt.g[0] : (g == 1 ? bt.g[1] : bt.g[2]);
+}
+
+__device__ __forceinline__ void split1(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+template <int BM, bool kOutT>
+__global__ void __launch_bounds__(512) k_gemm_b3(B3Batch bt) {
+  constexpr int STAGES = BM == 64 ? 2 : 1;
+  constexpr int TE = BM * BF_LD;  // bf16 elements of one plane tile
+  constexpr int WT = BM / 64;     // 32 x 32 MFMA tiles per wavefront side
+  constexpr int RPT = BM / 64;    // rows per thread of each operand tile
+  const TileIdx ti = map_tile(bt);
+  if (!ti.ok) return;  // whole workgroup: before any barrier
+  const B3Desc& gd = desc_of3(bt, ti.g);
+  const int M = bt.M, N = bt.N, K = bt.K;
+  const int m0 = ti.ty * BM, n0 = ti.tx * BM;
+  // [K half][stage][Ah, Al, Bh, Bl]
+  __shared__ __attribute__((aligned(16))) __bf16 lds[2][STAGES][4][TE];
+  const int half = threadIdx.x >> 8;
+  const int t = threadIdx.x & 255, lane = t & 63, wid = t >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  const int kspan = K / 2, kbase = half * kspan, nsteps = kspan / BF_BK;
+  const __bf16* Ah = gd.A;
+  const __bf16* Al = gd.A + bt.plane_a;
+  const __bf16* Bh = gd.B;
+  const __bf16* Bl = gd.B + bt.plane_b;
+  // thread t copies rows (t >> 2) + 64 i of both operand tiles, 16-byte
+  // chunks c and c + 4 (c = t & 3) of each 64-element K row; entry
+  // u = 2 i + j of each register array holds row i, chunk j
+  const int lr = t >> 2, lc = t & 3;
+  constexpr int NR = 2 * RPT;
+  u32x4 rah[NR], ral[NR], rbh[NR], rbl[NR];  // native vectors: kept in registers
+#define B3_LOAD(k0)                                                      \
+  _Pragma("unroll") for (int u = 0; u < NR; ++u) {                       \
+    const int r = lr + 64 * (u >> 1), q = lc + 4 * (u & 1);              \
+    const size_t oa = (size_t)(m0 + r) * K + (k0) + 8 * q;               \
+    const size_t ob = (size_t)(n0 + r) * K + (k0) + 8 * q;               \
+    rah[u] = *(const u32x4*)(Ah + oa);                                   \
+    ral[u] = *(const u32x4*)(Al + oa);                                   \
+    rbh[u] = *(const u32x4*)(Bh + ob);                                   \
+    rbl[u] = *(const u32x4*)(Bl + ob);                                   \
+  }
+#define B3_STORE(st)                                                     \
+  _Pragma("unroll") for (int u = 0; u < NR; ++u) {                       \
+    const int r = lr + 64 * (u >> 1), q = lc + 4 * (u & 1);              \
+    const int o = r * BF_LD + 8 * q;                                     \
+    *(u32x4*)&lds[half][st][0][o] = rah[u];                              \
+    *(u32x4*)&lds[half][st][1][o] = ral[u];                              \
+    *(u32x4*)&lds[half][st][2][o] = rbh[u];                              \
+    *(u32x4*)&lds[half][st][3][o] = rbl[u];                              \
+  }
+  f32x16 acc[WT][WT] = {};
+  auto compute = [&](int st) {
+    const __bf16* Xh = lds[half][st][0];
+    const __bf16* Xl = lds[half][st][1];
+    const __bf16* Yh = lds[half][st][2];
+    const __bf16* Yl = lds[half][st][3];
+#pragma unroll
+    for (int sub = 0; sub < BF_BK / 16; ++sub) {
+      const int ko = 16 * sub + 8 * h;
+      bf16x8 xh[WT], xl[WT], yh[WT], yl[WT];
+#pragma unroll
+      for (int i = 0; i < WT; ++i) {
+        const int oa = (wm * 32 * WT + 32 * i + l32) * BF_LD + ko;
+        const int ob = (wn * 32 * WT + 32 * i + l32) * BF_LD + ko;
+        xh[i] = *(const bf16x8*)(Xh + oa);
+        xl[i] = *(const bf16x8*)(Xl + oa);
+        yh[i] = *(const bf16x8*)(Yh + ob);
+        yl[i] = *(const bf16x8*)(Yl + ob);
+      }
+#pragma unroll
+      for (int i = 0; i < WT; ++i)
+#pragma unroll
+        for (int j = 0; j < WT; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl[i], yh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh[i], yl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh[i], yh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  };
+  // the trip count is the same in both halves: every thread meets every barrier
+  B3_LOAD(kbase);
+  if constexpr (STAGES == 2) {
+    B3_STORE(0);
+    __syncthreads();
+    for (int s = 0; s < nsteps; ++s) {
+      const bool more = s + 1 < nsteps;
+      if (more) {
+        B3_LOAD(kbase + (s + 1) * BF_BK);
+      }
+      compute(s & 1);
+      if (more) {
+        B3_STORE((s + 1) & 1);
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int s = 0; s < nsteps; ++s) {
+      __syncthreads();
+      B3_STORE(0);
+      __syncthreads();
+      if (s + 1 < nsteps) {
+        B3_LOAD(kbase + (s + 1) * BF_BK);
+      }
+      compute(0);
+    }
+  }
+#undef B3_LOAD
+#undef B3_STORE
+  // the upper K half's accumulators through LDS (no longer read), added to
+  // the lower half's: (first K half) + (second K half)
+  static_assert(sizeof(lds) >= 4 * WT * WT * 16 * 64 * sizeof(float), "reduction buffer");
+  float* red = (float*)&lds[0][0][0][0];
+  __syncthreads();
+  if (half == 1) {
+#pragma unroll
+    for (int i = 0; i < WT; ++i)
+#pragma unroll
+      for (int j = 0; j < WT; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) red[(((i * WT + j) * 16 + r) * 4 + wid) * 64 + lane] = acc[i][j][r];
+  }
+  __syncthreads();
+  const float alpha = bt.alpha;
+  double e = 0;
+  if (half == 0) {
+#pragma unroll
+    for (int i = 0; i < WT; ++i)
+#pragma unroll
+      for (int j = 0; j < WT; ++j) {
+        const int col = n0 + wn * 32 * WT + 32 * j + l32;
+        const int rbase = m0 + wm * 32 * WT + 32 * i + 4 * h;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          float v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = 4 * g + u;
+            const float a = acc[i][j][r] + red[(((i * WT + j) * 16 + r) * 4 + wid) * 64 + lane];
+            v[u] = alpha * a;
+          }
+          const int row0 = rbase + 8 * g;
+          if constexpr (kOutT) {
+            // rows row0..row0+3 of column col: four consecutive entries of C^T
+            union {
+              __bf16 b[4];
+              uint2 w;
+            } hi, lo;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) split1(v[u], hi.b[u], lo.b[u]);
+            __bf16* ct = gd.CT + (size_t)col * M + row0;
+            *(uint2*)ct = hi.w;
+            *(uint2*)(ct + (size_t)M * N) = lo.w;
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const size_t o = (size_t)(row0 + u) * N + col;
+              gd.C[o] = v[u];
+              if (gd.R) e += (double)v[u] * (double)gd.R[o];
+            }
+          }
+        }
+      }
+  }
+  if (!kOutT && gd.R) {
+    __shared__ double smem[8];
+    double bs = block_sum<512>(e, smem);
+    if (threadIdx.x == 0) gd.partials[ti.ty * bt.TX + ti.tx] = bs;
+  }
+}
+
 // ---- FP64: v_mfma_f64_16x16x4_f64, K tile 16 --------------------------------
 // Each wave owns a 32x32 tile = 2x2 MFMA tiles.  LDS rows hold 16 k values
 // permuted as pos(k) = (k&3)*4 + (k>>2), so lane quarter q (= lane/16) reads
@@ -701,10 +911,203 @@ __global__ void __launch_bounds__(kReduceBlock) k_spectral_weights(
   if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
 }
 
+// BF16x3 plans also hold the six matrices as bf16 hi/lo planes, after the
+// FP32 matrices: the three M x M left matrices as A operands ([m][k], as
+// stored) and the three N x N right matrices transposed as B operands
+// ([n][k]); each matrix's hi plane, then its lo plane.
+static inline size_t split_plane_bytes(int M, int N) {
+  return 2 * sizeof(__bf16) * (3 * (size_t)M * M + 3 * (size_t)N * N);
+}
+// Elements from the planes' start to matrix `id`'s hi plane.
+__host__ __device__ inline size_t split_plane_offset(int M, int N, int id) {
+  const size_t mm = (size_t)M * M, nn = (size_t)N * N;
+  return id < 3 ? 2 * id * mm : 6 * mm + 2 * (id - 3) * nn;
+}
+
+__global__ void k_plan_split(const float* plan, __bf16* planes, int M, int N) {
+  const size_t mm = (size_t)M * M, nn = (size_t)N * N;
+  const size_t total = 3 * mm + 3 * nn;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    if (t < 3 * mm) {  // left matrix id, element (r, c) -> A plane [r][c]
+      const int id = (int)(t / mm);
+      const size_t e = t % mm;
+      __bf16* hp = planes + split_plane_offset(M, N, id);
+      split1(plan[t], hp[e], hp[mm + e]);
+    } else {  // right matrix id, element (r, c) -> B plane [c][r]
+      const size_t u = t - 3 * mm;
+      const int id = 3 + (int)(u / nn);
+      const size_t e = u % nn;
+      const size_t r = e / N, c = e % N;
+      __bf16* hp = planes + split_plane_offset(M, N, id);
+      split1(plan[t], hp[c * N + r], hp[nn + c * N + r]);
+    }
+  }
+}
+
+// k_scale_centre with its output as the first GEMM's A planes (hi at P, lo
+// at P + n) and, when r32 is given, as FP32 too (the energy reference).
+__global__ void k_scale_centre_split(const float* in, int64_t n, double s, const double* sum,
+                                     __bf16* P, float* r32) {
+  const double mean = (*sum) * s / (double)n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = (float)((double)in[i] * s - mean);
+    split1(v, P[i], P[n + i]);
+    if (r32) r32[i] = v;
+  }
+}
+
+// k_spectral_weights with the three products written as A planes (pk
+// nullable) and the same spectral energy partials.
+__global__ void __launch_bounds__(kReduceBlock) k_spectral_weights_split(
+    const float* a, int M, int N, double aspect, __bf16* px, __bf16* py, __bf16* pk,
+    double* partials) {
+  __shared__ double smem[kReduceBlock / 64];
+  const int64_t n = (int64_t)M * N;
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int u = (int)(i / N), v = (int)(i % N);
+    double wu = (2.0 * M_PI / M) * u;
+    double wv = (2.0 * M_PI / N) * v * aspect;
+    double w2 = wu * wu + wv * wv;
+    double inv = (u == 0 && v == 0) ? 0.0 : 1.0 / w2;
+    float av = a[i];
+    split1(av * (float)(wu * inv * 0.5), px[i], px[n + i]);
+    split1(av * (float)(wv * inv * 0.5), py[i], py[n + i]);
+    if (pk) split1(av * (float)inv, pk[i], pk[n + i]);
+    if (partials && !(u == 0 && v == 0)) {
+      const double ad = (double)av;
+      acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * ad * ad / w2;
+    }
+  }
+  if (!partials) return;  // uniform: the whole grid returns
+  double bs = block_sum<kReduceBlock>(acc, smem);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
+}
+
+// A batch of `count` pre-split GEMMs of one shape (kOutT: products written as
+// C^T planes).  The shape must satisfy presplit_shape.
+static bool presplit_shape(int M, int N) {
+  return M % (2 * BF_BK) == 0 && N % (2 * BF_BK) == 0 && M % BT == 0 && N % BT == 0;
+}
+static int launch_b3(B3Batch& bt, int count, bool out_t, hipStream_t s, int* nblocks) {
+  const bool big = bt.M % BT2 == 0 && bt.N % BT2 == 0 &&
+                   (int64_t)(bt.M / BT2) * (bt.N / BT2) * count >= kLargeGridTiles;
+  const int bm = big ? BT2 : BT;
+  bt.TY = bt.M / bm;
+  bt.TX = bt.N / bm;
+  bt.count = count;
+  const int64_t tiles = (int64_t)bt.TY * bt.TX * count;
+  dim3 grid((unsigned)(kXcds * ceil_div(tiles, kXcds)));
+  if (nblocks) *nblocks = bt.TY * bt.TX;
+  if (big) {
+    if (out_t)
+      hipLaunchKernelGGL((k_gemm_b3<BT2, true>), grid, dim3(512), 0, s, bt);
+    else
+      hipLaunchKernelGGL((k_gemm_b3<BT2, false>), grid, dim3(512), 0, s, bt);
+  } else {
+    if (out_t)
+      hipLaunchKernelGGL((k_gemm_b3<BT, true>), grid, dim3(512), 0, s, bt);
+    else
+      hipLaunchKernelGGL((k_gemm_b3<BT, false>), grid, dim3(512), 0, s, bt);
+  }
+  DP_LAUNCHED("k_gemm_b3");
+  return DP_OK;
+}
+
 // Workspace of the spectral ops: seven M*N slabs of the data type, then one
 // reduction slot.  The single transforms use the first slab only.
 static size_t ws_slab(int M, int N, int precision) {
   return align256((size_t)M * N * elem_bytes(precision));
+}
+
+// The BF16x3 solve on pre-split operands (k_gemm_b3): the same sequence of
+// products as the in-kernel-split path and the same bits.  Workspace slabs:
+// 0 rho (FP32, the energy reference of the potential), 1 rho planes, then
+// Yphi^T; 2 the DCT2's first product (transposed planes), then Yx^T; 3 the
+// DCT2 (FP32), then Yy^T; 4-6 the planes of a*wu*K/2, a*wv*K/2, a*K.
+static int poisson_presplit(const void* plan, int M, int N, const float* map, double bsx,
+                            double bsy, bool want_phi, float* field_x, float* field_y,
+                            float* potential, float* energy, char* w, hipStream_t s) {
+  const int precision = DP_SPECTRAL_BF16X3;
+  const size_t slab = ws_slab(M, N, precision);
+  const int64_t n = (int64_t)M * N;
+  float* rho32 = (float*)w;
+  __bf16* rhoP = (__bf16*)(w + slab);
+  __bf16* tmpT = (__bf16*)(w + 2 * slab);
+  float* a32 = (float*)(w + 3 * slab);
+  __bf16* axP = (__bf16*)(w + 4 * slab);
+  __bf16* ayP = (__bf16*)(w + 5 * slab);
+  __bf16* akP = (__bf16*)(w + 6 * slab);
+  __bf16* yphiT = rhoP;  // rho's planes are read by the first GEMM only
+  __bf16* yxT = tmpT;    // read by the second GEMM only
+  __bf16* yyT = (__bf16*)a32;  // the DCT2 is read by the weights only
+  double* part = (double*)(w + 7 * slab);
+  double* msum = part + kReduceMaxBlocks;
+  const __bf16* planes = (const __bf16*)((const char*)plan + plan_offset(M, N, 6, precision));
+  auto P = [&](int id) { return planes + split_plane_offset(M, N, id); };
+  const bool energy_phi = energy && want_phi;
+  // rho = map / (bx*by), centred, as planes
+  const int nb = (int)(ceil_div(n, kReduceBlock) < 1024 ? ceil_div(n, kReduceBlock) : 1024);
+  hipLaunchKernelGGL(k_sum_partial<float>, dim3(nb), dim3(kReduceBlock), 0, s, map, n, part);
+  DP_LAUNCHED("k_sum_partial");
+  DP_CHECK_RC(finish_reduction(part, nb, 1, nullptr, msum, -1, s));
+  hipLaunchKernelGGL(k_scale_centre_split, dim3(grid_for(n, 256)), dim3(256), 0, s, map, n,
+                     1.0 / (bsx * bsy), (const double*)msum, rhoP,
+                     energy_phi ? rho32 : nullptr);
+  DP_LAUNCHED("k_scale_centre_split");
+  B3Batch bt = {};
+  bt.alpha = 1.0f;
+  // DCT2: tmp = rho * R_DCT (written transposed), a = L_DCT * tmp
+  bt.M = M;
+  bt.N = N;
+  bt.K = N;
+  bt.plane_a = n;
+  bt.plane_b = (int64_t)N * N;
+  bt.g[0] = {rhoP, P(R_DCT), nullptr, tmpT, nullptr, nullptr};
+  DP_CHECK_RC(launch_b3(bt, 1, true, s, nullptr));
+  bt.K = M;
+  bt.plane_a = (int64_t)M * M;
+  bt.plane_b = n;
+  bt.g[0] = {P(L_DCT), tmpT, a32, nullptr, nullptr, nullptr};
+  DP_CHECK_RC(launch_b3(bt, 1, false, s, nullptr));
+  // spectral weights (and the spectral energy when there is no potential)
+  const bool spectral_energy = energy && !want_phi;
+  const int wblk = (int)(ceil_div(n, kReduceBlock) < kReduceMaxBlocks ? ceil_div(n, kReduceBlock)
+                                                                      : kReduceMaxBlocks);
+  hipLaunchKernelGGL(k_spectral_weights_split, dim3(wblk), dim3(kReduceBlock), 0, s,
+                     (const float*)a32, M, N, bsx / bsy, axP, ayP, want_phi ? akP : nullptr,
+                     spectral_energy ? part : nullptr);
+  DP_LAUNCHED("k_spectral_weights_split");
+  // inverse transforms, first GEMMs (one batch, written transposed):
+  //   Yx = ax*R_IDCT, Yy = ay*R_IDXST, Yphi = ak*R_IDCT
+  bt.K = N;
+  bt.plane_a = n;
+  bt.plane_b = (int64_t)N * N;
+  bt.g[0] = {axP, P(R_IDCT), nullptr, yxT, nullptr, nullptr};
+  bt.g[1] = {ayP, P(R_IDXST), nullptr, yyT, nullptr, nullptr};
+  int count = 2;
+  if (want_phi) {
+    bt.g[2] = {akP, P(R_IDCT), nullptr, yphiT, nullptr, nullptr};
+    count = 3;
+  }
+  DP_CHECK_RC(launch_b3(bt, count, true, s, nullptr));
+  // second GEMMs: field_x = L_IDXST*Yx, field_y = L_IDCT*Yy,
+  // potential = L_IDCT*Yphi (+ energy partials sum(potential*rho))
+  bt.K = M;
+  bt.plane_a = (int64_t)M * M;
+  bt.plane_b = n;
+  bt.g[0] = {P(L_IDXST), yxT, field_x, nullptr, nullptr, nullptr};
+  bt.g[1] = {P(L_IDCT), yyT, field_y, nullptr, nullptr, nullptr};
+  if (want_phi)
+    bt.g[2] = {P(L_IDCT), yphiT, potential, nullptr, energy_phi ? rho32 : nullptr, part};
+  int nblk = 0;
+  DP_CHECK_RC(launch_b3(bt, count, false, s, &nblk));
+  if (energy)
+    DP_CHECK_RC(finish_reduction(part, want_phi ? nblk : wblk, 1, nullptr, energy, DP_FLOAT32, s));
+  return DP_OK;
 }
 
 static int transform(int precision, const void* plan, int M, int N, const void* x, void* out,
@@ -746,7 +1149,9 @@ using namespace dp;
 extern "C" {
 
 size_t dp_spectral_plan_bytes(int M, int N, int precision) {
-  return plan_offset(M, N, 6, precision);
+  size_t b = plan_offset(M, N, 6, precision);
+  if (precision == DP_SPECTRAL_BF16X3) b += split_plane_bytes(M, N);
+  return b;
 }
 
 int dp_spectral_plan_init(int M, int N, int precision, void* plan, void* stream) {
@@ -761,6 +1166,11 @@ int dp_spectral_plan_init(int M, int N, int precision, void* plan, void* stream)
   else
     DP_ARG(false, "dp_spectral_plan_init: bad precision %d", precision);
   DP_LAUNCHED("k_plan_fill");
+  if (precision == DP_SPECTRAL_BF16X3) {
+    hipLaunchKernelGGL(k_plan_split, dim3(grid), dim3(256), 0, s, (const float*)plan,
+                       (__bf16*)((char*)plan + plan_offset(M, N, 6, precision)), M, N);
+    DP_LAUNCHED("k_plan_split");
+  }
   return DP_OK;
 }
 
@@ -790,6 +1200,10 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
   DP_ARG(!compute_potential || potential, "dp_poisson_solve: compute_potential needs potential");
   hipStream_t s = S(stream);
   DP_PROBE_OP(DP_OP_POISSON, s);
+  if (precision == DP_SPECTRAL_BF16X3 && presplit_shape(M, N))
+    return poisson_presplit(plan, M, N, (const float*)density_map, bin_size_x, bin_size_y,
+                            compute_potential != 0, (float*)field_x, (float*)field_y,
+                            (float*)potential, (float*)energy, (char*)workspace, s);
   const size_t slab = ws_slab(M, N, precision);
   char* w = (char*)workspace;
   void* rho = w;               // map/(bx*by), kept for the energy
