@@ -534,6 +534,10 @@ def main(argv=None):
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--arith", default=None, choices=["accurate", "reference"],
+                    help="arithmetic of the FP32 forms: accurate (centred wirelength, bin-relative "
+                         "density overlaps; the default) or the reference's own; FP64 runs take "
+                         "the reference's")
     ap.add_argument("--stop-overflow", type=float, default=None)
     ap.add_argument("--fixed-iterations", action="store_true",
                     help="no stop tests: run exactly --iterations steps")
@@ -545,6 +549,9 @@ def main(argv=None):
                          "use of each kernel loads it once; 0 times the first run")
     a = ap.parse_args(argv)
     dt = np.float64 if a.precision == "fp64" else np.float32
+    arith = a.arith or ("reference" if a.precision == "fp64" else "accurate")
+    akw = (dict(wa_arith="centred", density_arith="bin_relative") if arith == "accurate"
+           else dict(wa_arith="reference", density_arith="reference"))
     t0 = time.time()
     d = synth.make_design(a.config, seed=a.seed, dtype=dt, scale=a.scale)
     t_gen = time.time() - t0
@@ -556,14 +563,15 @@ def main(argv=None):
     if a.warmup > 0:
         t0 = time.time()
         gp = GlobalPlacer(d, dev, params=dict(prm, early_stop=False), precision=a.precision,
-                          algorithm="tiled", schedule="pins", energy_mode="spectral", seed=a.seed)
+                          algorithm="tiled", schedule="pins", energy_mode="spectral", seed=a.seed,
+                          **akw)
         gp.run(a.warmup)
         torch.cuda.synchronize()
         del gp
         torch.cuda.empty_cache()
         t_warm = time.time() - t0
     gp = GlobalPlacer(d, dev, params=prm, precision=a.precision, algorithm="tiled", schedule="pins",
-                      energy_mode="spectral", seed=a.seed)
+                      energy_mode="spectral", seed=a.seed, **akw)
     torch.cuda.synchronize()
     t1 = time.time()
     _, hist = gp.run(a.iterations, log_every=a.log_every)
@@ -572,7 +580,8 @@ def main(argv=None):
     x = gp.pos
     gp.constraint(x)
     fin = gp.metrics(x)
-    out = dict(config=a.config, scale=a.scale, precision=a.precision, dtype=str(np.dtype(dt)),
+    out = dict(config=a.config, scale=a.scale, precision=a.precision, arith=arith,
+               dtype=str(np.dtype(dt)),
                num_nodes=int(d["num_nodes"]), num_filler_nodes=int(d["num_filler_nodes"]),
                num_pins=int(d["num_pins"]), bins=int(round((d["xh"] - d["xl"]) / d["bin_size_x"])),
                iterations=len(hist), evaluations=int(gp.optimizer.param_groups[0]["obj_eval_count"]),
