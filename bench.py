@@ -53,10 +53,11 @@ DP_OP_DENSITY_MAP, DP_OP_ELECTRIC_FORCE, DP_OP_WA_FORWARD, DP_OP_NODE_GRAD, DP_O
 
 # kernels of each op in the pins schedule with tiled density (rocprof names)
 OP_KERNELS = {
-    "wa_forward": ["k_pin_push", "k_wa_pins (finishes the total; no nets above 64 pins)"],
+    "wa_forward": ["k_pin_push", "k_wa_pins", "k_finish (no k_wa_big: no net above 64 pins)"],
     "density_map": ["k_zero_regions", "k_bin_count", "k_bin_scan", "k_bin_fill",
                     "k_tile_scatter_f32(_acc)", "k_scatter_rest", "k_convert_cpu"],
-    "poisson": ["k_sum_partial", "k_scale_centre", "k_gemm_bf16x3", "k_spectral_weights"],
+    "poisson": ["k_sum_partial", "k_scale_centre_split", "k_gemm_b3", "k_spectral_weights_split",
+                "k_finish"],
     "electric_force": ["k_tile_gather", "k_gather_rest"],
     "node_grad": ["k_node_grad"],
 }
@@ -345,9 +346,8 @@ def summarize_trace(trace_dir, warmup, steps):
 
 
 # kernels whose HBM bytes make each op's measured traffic (per-dispatch
-# medians; the map clear belongs to the scatter; the wirelength total is
-# finished inside k_wa_pins when there are no nets above 64 pins, else by
-# k_wa_big and a k_finish)
+# medians; the map clear belongs to the scatter; the wirelength total's
+# k_finish reads a few KB and is left out)
 TRAFFIC_KERNELS = {
     "wa_forward": ["dp::k_pin_push<", "dp::k_wa_pins<", "dp::k_wa_big<"],
     "density_map": ["dp::k_zero_regions", "dp::k_bin_count<", "dp::k_bin_scan",

@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmBatch bt) {
     const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
     float v = alpha * acc[r];
     C[(size_t)row * N + col] = v;
-    if (R) e += (double)v * (double)R[(size_t)row * N + col];
+    if (R) e += (double)(v * R[(size_t)row * N + col]);
   }
   if (R) {
     __shared__ double smem[4];
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256 * SPLIT) k_gemm_bf16x3(GemmBatch bt) {
       const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
       float v = alpha * acc[r];
       C[(size_t)row * N + col] = v;
-      if (R) e += (double)v * (double)R[(size_t)row * N + col];
+      if (R) e += (double)(v * R[(size_t)row * N + col]);
     }
   }
   if (R) {
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(512) k_gemm_bf16x3_t128(GemmBatch bt) {
           const int row = m0 + wm * 64 + 32 * i + 8 * (r >> 2) + 4 * h + (r & 3);
           const float v = alpha * a;
           C[(size_t)row * N + col] = v;
-          if (R) e += (double)v * (double)R[(size_t)row * N + col];
+          if (R) e += (double)(v * R[(size_t)row * N + col]);
         }
       }
   }
@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(512) k_gemm_b3(B3Batch bt) {
             for (int u = 0; u < 4; ++u) {
               const size_t o = (size_t)(row0 + u) * N + col;
               gd.C[o] = v[u];
-              if (gd.R) e += (double)v[u] * (double)gd.R[o];
+              if (gd.R) e += (double)(v[u] * gd.R[o]);
             }
           }
         }
@@ -853,61 +853,78 @@ static int launch_gemm(int precision, GemmBatch& bt, int count, hipStream_t s, i
 // ---------------------------------------------------------------------------
 // Elementwise pieces of the solve.
 // ---------------------------------------------------------------------------
-// Partial sums of the map (FP64) for its mean.
+// Partial sums of the map for its mean: each thread sums its few elements
+// in T, the threads' sums are reduced in FP64 in a fixed order.
 template <typename T>
 __global__ void __launch_bounds__(kReduceBlock) k_sum_partial(const T* in, int64_t n,
                                                               double* partials) {
   __shared__ double smem[kReduceBlock / 64];
-  double acc = 0;
+  T acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    acc += (double)in[i];
-  double bs = block_sum<kReduceBlock>(acc, smem);
+    acc += in[i];
+  double bs = block_sum<kReduceBlock>((double)acc, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
-// rho = map / (bx*by) (electric_potential.py:160), centred: the mean is
-// subtracted in FP64 before rounding to T.  The DC coefficient of rho does not
-// reach the fields or the potential (K[0,0] = 0, and sum(phi) = 0 so the
-// energy sum(phi*rho) is unchanged), while centring halves the rounding error
-// of every other DCT coefficient.
+// rho = map / (bx*by) (electric_potential.py:160), centred: in*s - mean as
+// one fused multiply-add in T (the map's sum is a deterministic FP64
+// reduction).  The DC coefficient of rho does not reach the fields or the
+// potential (K[0,0] = 0, and sum(phi) = 0 so the energy sum(phi*rho) is
+// unchanged), so the rounding of s and of the mean only moves that
+// coefficient, while centring halves the rounding error of every other one.
 template <typename T>
-__global__ void k_scale_centre(const T* in, int64_t n, double s, const double* sum, T* out) {
-  const double mean = (*sum) * s / (double)n;
+__device__ __forceinline__ T centre_scale(T s, const double* sum, int64_t n) {
+  return (T)(*sum) * s / (T)n;
+}
+template <typename T>
+__global__ void k_scale_centre(const T* in, int64_t n, T s, const double* sum, T* out) {
+  const T mean = centre_scale(s, sum, n);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = (T)((double)in[i] * s - mean);
+    out[i] = fma(in[i], s, -mean);
 }
 
-// Spectral multipliers of electric_potential.py:504-519, evaluated in FP64
-// per element: wu = 2 pi u/M, wv = 2 pi v/N * (bx/by), K = 1/(wu^2+wv^2),
-// K[0,0] = 0.  Writes a*wu*K/2, a*wv*K/2 and (optionally) a*K.  With
-// `partials` it also forms the per-workgroup partials of the spectral energy
-// (MN/4) sum c_u c_v K a^2 (FP64) in the same pass.
+// Spectral multipliers of electric_potential.py:504-519, evaluated per
+// element in T from the two grid steps du = 2 pi/M and dv = 2 pi/N * (bx/by)
+// (rounded to T once, on the host): wu = du*u, wv = dv*v,
+// K = 1/(wu^2+wv^2), K[0,0] = 0.  In FP32 each multiplier is within a few
+// ulp of its correctly rounded value.
+template <typename T>
+struct SpectralMul {
+  T kx, ky, k, e;  // wu*K/2, wv*K/2, K, c_u c_v K (the energy's weight)
+};
+template <typename T>
+__device__ __forceinline__ SpectralMul<T> spectral_mul(int u, int v, T du, T dv) {
+  const T wu = du * (T)u, wv = dv * (T)v;
+  const T w2 = wu * wu + wv * wv;
+  const T inv = (u == 0 && v == 0) ? (T)0 : (T)1 / w2;
+  const T c = (u ? (T)2 : (T)1) * (v ? (T)2 : (T)1);
+  return {wu * inv * (T)0.5, wv * inv * (T)0.5, inv, c * inv};
+}
+
+// Writes a*wu*K/2, a*wv*K/2 and (optionally) a*K.  With `partials` it also
+// forms the per-workgroup partials of the spectral energy
+// (MN/4) sum c_u c_v K a^2 in the same pass (each thread sums its terms in T,
+// the threads' sums are reduced in FP64).
 template <typename T>
 __global__ void __launch_bounds__(kReduceBlock) k_spectral_weights(
-    const T* a, int M, int N, double aspect, T* ax, T* ay, T* ak, double* partials) {
+    const T* a, int M, int N, T du, T dv, T* ax, T* ay, T* ak, double* partials) {
   __shared__ double smem[kReduceBlock / 64];
   const int64_t n = (int64_t)M * N;
-  double acc = 0;
+  T acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int u = (int)(i / N), v = (int)(i % N);
-    double wu = (2.0 * M_PI / M) * u;
-    double wv = (2.0 * M_PI / N) * v * aspect;
-    double w2 = wu * wu + wv * wv;
-    double inv = (u == 0 && v == 0) ? 0.0 : 1.0 / w2;
-    T av = a[i];
-    ax[i] = av * (T)(wu * inv * 0.5);
-    ay[i] = av * (T)(wv * inv * 0.5);
-    if (ak) ak[i] = av * (T)inv;
-    if (partials && !(u == 0 && v == 0)) {
-      const double ad = (double)av;
-      acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * ad * ad / w2;
-    }
+    const int u = (int)(i / N), v = (int)(i % N);
+    const SpectralMul<T> m = spectral_mul<T>(u, v, du, dv);
+    const T av = a[i];
+    ax[i] = av * m.kx;
+    ay[i] = av * m.ky;
+    if (ak) ak[i] = av * m.k;
+    if (partials) acc += m.e * av * av;
   }
   if (!partials) return;  // uniform: the whole grid returns
-  double bs = block_sum<kReduceBlock>(acc, smem);
+  double bs = block_sum<kReduceBlock>((double)acc, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
 }
 
@@ -947,12 +964,12 @@ __global__ void k_plan_split(const float* plan, __bf16* planes, int M, int N) {
 
 // k_scale_centre with its output as the first GEMM's A planes (hi at P, lo
 // at P + n) and, when r32 is given, as FP32 too (the energy reference).
-__global__ void k_scale_centre_split(const float* in, int64_t n, double s, const double* sum,
+__global__ void k_scale_centre_split(const float* in, int64_t n, float s, const double* sum,
                                      __bf16* P, float* r32) {
-  const double mean = (*sum) * s / (double)n;
+  const float mean = centre_scale(s, sum, n);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = (float)((double)in[i] * s - mean);
+    const float v = fmaf(in[i], s, -mean);
     split1(v, P[i], P[n + i]);
     if (r32) r32[i] = v;
   }
@@ -961,29 +978,23 @@ __global__ void k_scale_centre_split(const float* in, int64_t n, double s, const
 // k_spectral_weights with the three products written as A planes (pk
 // nullable) and the same spectral energy partials.
 __global__ void __launch_bounds__(kReduceBlock) k_spectral_weights_split(
-    const float* a, int M, int N, double aspect, __bf16* px, __bf16* py, __bf16* pk,
+    const float* a, int M, int N, float du, float dv, __bf16* px, __bf16* py, __bf16* pk,
     double* partials) {
   __shared__ double smem[kReduceBlock / 64];
   const int64_t n = (int64_t)M * N;
-  double acc = 0;
+  float acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int u = (int)(i / N), v = (int)(i % N);
-    double wu = (2.0 * M_PI / M) * u;
-    double wv = (2.0 * M_PI / N) * v * aspect;
-    double w2 = wu * wu + wv * wv;
-    double inv = (u == 0 && v == 0) ? 0.0 : 1.0 / w2;
-    float av = a[i];
-    split1(av * (float)(wu * inv * 0.5), px[i], px[n + i]);
-    split1(av * (float)(wv * inv * 0.5), py[i], py[n + i]);
-    if (pk) split1(av * (float)inv, pk[i], pk[n + i]);
-    if (partials && !(u == 0 && v == 0)) {
-      const double ad = (double)av;
-      acc += (u ? 2.0 : 1.0) * (v ? 2.0 : 1.0) * ad * ad / w2;
-    }
+    const int u = (int)(i / N), v = (int)(i % N);
+    const SpectralMul<float> m = spectral_mul<float>(u, v, du, dv);
+    const float av = a[i];
+    split1(av * m.kx, px[i], px[n + i]);
+    split1(av * m.ky, py[i], py[n + i]);
+    if (pk) split1(av * m.k, pk[i], pk[n + i]);
+    if (partials) acc += m.e * av * av;
   }
   if (!partials) return;  // uniform: the whole grid returns
-  double bs = block_sum<kReduceBlock>(acc, smem);
+  double bs = block_sum<kReduceBlock>((double)acc, smem);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs * ((double)M * N / 4.0);
 }
 
@@ -1055,7 +1066,7 @@ static int poisson_presplit(const void* plan, int M, int N, const float* map, do
   DP_LAUNCHED("k_sum_partial");
   DP_CHECK_RC(finish_reduction(part, nb, 1, nullptr, msum, -1, s));
   hipLaunchKernelGGL(k_scale_centre_split, dim3(grid_for(n, 256)), dim3(256), 0, s, map, n,
-                     1.0 / (bsx * bsy), (const double*)msum, rhoP,
+                     (float)(1.0 / (bsx * bsy)), (const double*)msum, rhoP,
                      energy_phi ? rho32 : nullptr);
   DP_LAUNCHED("k_scale_centre_split");
   B3Batch bt = {};
@@ -1078,7 +1089,8 @@ static int poisson_presplit(const void* plan, int M, int N, const float* map, do
   const int wblk = (int)(ceil_div(n, kReduceBlock) < kReduceMaxBlocks ? ceil_div(n, kReduceBlock)
                                                                       : kReduceMaxBlocks);
   hipLaunchKernelGGL(k_spectral_weights_split, dim3(wblk), dim3(kReduceBlock), 0, s,
-                     (const float*)a32, M, N, bsx / bsy, axP, ayP, want_phi ? akP : nullptr,
+                     (const float*)a32, M, N, (float)(2.0 * M_PI / M),
+                     (float)(2.0 * M_PI / N * (bsx / bsy)), axP, ayP, want_phi ? akP : nullptr,
                      spectral_energy ? part : nullptr);
   DP_LAUNCHED("k_spectral_weights_split");
   // inverse transforms, first GEMMs (one batch, written transposed):
@@ -1228,7 +1240,7 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
     DP_LAUNCHED("k_sum_partial");
     DP_CHECK_RC(finish_reduction(part, nb, 1, nullptr, msum, -1, s));
     hipLaunchKernelGGL(k_scale_centre<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const T*)density_map, n, 1.0 / (bin_size_x * bin_size_y),
+                       (const T*)density_map, n, (T)(1.0 / (bin_size_x * bin_size_y)),
                        (const double*)msum, (T*)rho);
     DP_LAUNCHED("k_scale_centre");
     return DP_OK;
@@ -1241,7 +1253,8 @@ int dp_poisson_solve(int precision, const void* plan, int M, int N, const void* 
     wblk = (int)(ceil_div(n, kReduceBlock) < kReduceMaxBlocks ? ceil_div(n, kReduceBlock)
                                                                         : kReduceMaxBlocks);
     hipLaunchKernelGGL(k_spectral_weights<T>, dim3(wblk), dim3(kReduceBlock), 0, s,
-                       (const T*)auv, M, N, aspect, (T*)ax, (T*)ay, want_phi ? (T*)ak : nullptr,
+                       (const T*)auv, M, N, (T)(2.0 * M_PI / M), (T)(2.0 * M_PI / N * aspect),
+                       (T*)ax, (T*)ay, want_phi ? (T*)ak : nullptr,
                        spectral_energy ? part : nullptr);
     DP_LAUNCHED("k_spectral_weights");
     return DP_OK;
