@@ -541,25 +541,33 @@ __global__ void __launch_bounds__(512) k_gemm_b3(B3Batch bt) {
   // u = 2 i + j of each register array holds row i, chunk j
   const int lr = t >> 2, lc = t & 3;
   constexpr int NR = 2 * RPT;
-  u32x4 rah[NR], ral[NR], rbh[NR], rbl[NR];  // native vectors: kept in registers
-#define B3_LOAD(k0)                                                      \
+  // D register sets, each one K step's operand chunks: the loads of step
+  // s + D are issued while step s is computed, so D steps of load latency
+  // overlap (a K step's 12 MFMAs per wavefront are far shorter than one
+  // load's latency).  Native vectors, compile-time indices: registers.  At
+  // 2 waves per SIMD a wavefront has 256 registers: 64 x 64 tiles take two
+  // sets (64 registers), 128 x 128 tiles one (its 64 accumulators and 2 x 2
+  // operand fragments leave no room for a second).
+  constexpr int D = BM == 64 ? 2 : 1;
+  u32x4 rah[D][NR], ral[D][NR], rbh[D][NR], rbl[D][NR];
+#define B3_LOAD(k0, d)                                                   \
   _Pragma("unroll") for (int u = 0; u < NR; ++u) {                       \
     const int r = lr + 64 * (u >> 1), q = lc + 4 * (u & 1);              \
     const size_t oa = (size_t)(m0 + r) * K + (k0) + 8 * q;               \
     const size_t ob = (size_t)(n0 + r) * K + (k0) + 8 * q;               \
-    rah[u] = *(const u32x4*)(Ah + oa);                                   \
-    ral[u] = *(const u32x4*)(Al + oa);                                   \
-    rbh[u] = *(const u32x4*)(Bh + ob);                                   \
-    rbl[u] = *(const u32x4*)(Bl + ob);                                   \
+    rah[d][u] = *(const u32x4*)(Ah + oa);                                \
+    ral[d][u] = *(const u32x4*)(Al + oa);                                \
+    rbh[d][u] = *(const u32x4*)(Bh + ob);                                \
+    rbl[d][u] = *(const u32x4*)(Bl + ob);                                \
   }
-#define B3_STORE(st)                                                     \
+#define B3_STORE(st, d)                                                  \
   _Pragma("unroll") for (int u = 0; u < NR; ++u) {                       \
     const int r = lr + 64 * (u >> 1), q = lc + 4 * (u & 1);              \
     const int o = r * BF_LD + 8 * q;                                     \
-    *(u32x4*)&lds[half][st][0][o] = rah[u];                              \
-    *(u32x4*)&lds[half][st][1][o] = ral[u];                              \
-    *(u32x4*)&lds[half][st][2][o] = rbh[u];                              \
-    *(u32x4*)&lds[half][st][3][o] = rbl[u];                              \
+    *(u32x4*)&lds[half][st][0][o] = rah[d][u];                           \
+    *(u32x4*)&lds[half][st][1][o] = ral[d][u];                           \
+    *(u32x4*)&lds[half][st][2][o] = rbh[d][u];                           \
+    *(u32x4*)&lds[half][st][3][o] = rbl[d][u];                           \
   }
   f32x16 acc[WT][WT] = {};
   auto compute = [&](int st) {
@@ -590,31 +598,90 @@ __global__ void __launch_bounds__(512) k_gemm_b3(B3Batch bt) {
         }
     }
   };
-  // the trip count is the same in both halves: every thread meets every barrier
-  B3_LOAD(kbase);
+  // The trip count is the same in both halves and every condition below is
+  // uniform over the workgroup: every thread meets every barrier.  Register
+  // set j of the ring holds K step s with s % D == j.  When the steps are a
+  // whole number of rings, the loop issues its loads unconditionally and the
+  // last D steps load nothing: with no branch around the loads the compiler
+  // waits for the oldest set only, not for every load in flight.
+  const bool rings = nsteps % D == 0;
+  if (rings) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) B3_LOAD(kbase + d * BF_BK, d);
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      if (d < nsteps) B3_LOAD(kbase + d * BF_BK, d);
+  }
   if constexpr (STAGES == 2) {
-    B3_STORE(0);
+    // step s is computed from LDS stage s & 1 while step s + 1 waits in
+    // registers; it goes to the other stage after the computation
+    static_assert(D % 2 == 0, "stage parity follows the ring index");  // 64 x 64: D = 2
+    B3_STORE(0, 0);
     __syncthreads();
-    for (int s = 0; s < nsteps; ++s) {
-      const bool more = s + 1 < nsteps;
-      if (more) {
-        B3_LOAD(kbase + (s + 1) * BF_BK);
+    if (rings) {
+      for (int s0 = 0; s0 < nsteps - D; s0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          B3_LOAD(kbase + (s0 + j + D) * BF_BK, j);
+          compute(j & 1);
+          B3_STORE((j + 1) & 1, (j + 1) % D);
+          __syncthreads();
+        }
       }
-      compute(s & 1);
-      if (more) {
-        B3_STORE((s + 1) & 1);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {  // the last D steps
+        compute(j & 1);
+        if (j + 1 < D) B3_STORE((j + 1) & 1, (j + 1) % D);
+        __syncthreads();
       }
-      __syncthreads();
+    } else {
+      for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const int s = s0 + j;
+          if (s < nsteps) {
+            if (s + D < nsteps) B3_LOAD(kbase + (s + D) * BF_BK, j);
+            compute(s & 1);
+            if (s + 1 < nsteps) B3_STORE((s + 1) & 1, (j + 1) % D);
+            __syncthreads();
+          }
+        }
+      }
     }
   } else {
-    for (int s = 0; s < nsteps; ++s) {
-      __syncthreads();
-      B3_STORE(0);
-      __syncthreads();
-      if (s + 1 < nsteps) {
-        B3_LOAD(kbase + (s + 1) * BF_BK);
+    if (rings) {
+      for (int s0 = 0; s0 < nsteps - D; s0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          __syncthreads();  // the previous step's reads of the stage are done
+          B3_STORE(0, j);
+          __syncthreads();
+          B3_LOAD(kbase + (s0 + j + D) * BF_BK, j);
+          compute(0);
+        }
       }
-      compute(0);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {  // the last D steps
+        __syncthreads();
+        B3_STORE(0, j);
+        __syncthreads();
+        compute(0);
+      }
+    } else {
+      for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const int s = s0 + j;
+          if (s < nsteps) {
+            __syncthreads();
+            B3_STORE(0, j);
+            __syncthreads();
+            if (s + D < nsteps) B3_LOAD(kbase + (s + D) * BF_BK, j);
+            compute(0);
+          }
+        }
+      }
     }
   }
 #undef B3_LOAD
