@@ -26,8 +26,20 @@ __global__ void __launch_bounds__(kReduceBlock) k_finish(const double* partials,
     const double* p = partials + (size_t)k * kReduceMaxBlocks;
     bool is_max = (max_mask >> k) & 1;
     double acc = is_max ? -HUGE_VAL : 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-      double v = p[i];
+    // each lane takes i = lane, lane + 256, ... in that order; eight loads
+    // are issued before their eight (ordered) additions, so the lanes do
+    // not wait out one load latency per partial
+    constexpr int U = 8;
+    int i = threadIdx.x;
+    for (; i + (U - 1) * kReduceBlock < nblocks; i += U * kReduceBlock) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = p[i + u * kReduceBlock];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = is_max ? (v[u] > acc ? v[u] : acc) : acc + v[u];
+    }
+    for (; i < nblocks; i += kReduceBlock) {
+      const double v = p[i];
       acc = is_max ? (v > acc ? v : acc) : acc + v;
     }
     double r = is_max ? block_max<kReduceBlock>(acc, smem) : block_sum<kReduceBlock>(acc, smem);

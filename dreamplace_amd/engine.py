@@ -438,8 +438,9 @@ class PlacementObjective:
         self.side = None
         self._side_stream = None
         # where the wirelength pass forks off: before the scatter ("scatter",
-        # measured best: 0.879 ms at config 1 against 0.903 before the solve,
-        # 0.886 before the gather and 0.918 on one stream), "solve" or "gather"
+        # measured best: 0.775 ms at config 1 against 0.788 before the solve,
+        # 0.779 before the gather and 0.808 on one stream; 1.680 ms at config 4
+        # against 1.721, 1.736 and 1.702), "solve" or "gather"
         self.fork_at = "scatter"
         self.set_overlap(overlap)
         self.filler_ranges = None
