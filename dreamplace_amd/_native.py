@@ -91,6 +91,9 @@ _SIGNATURES = {
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_hpwl_from_nets": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),
+    "dp_filler_tile_keys": (_I, [_I, _P, _L, _L, _D, _D, _D, _D, _I, _I, _D, _D, _P, _P, _P]),
+    "dp_permute_fillers_workspace_bytes": (_S, [_I, _L, _I]),
+    "dp_permute_fillers": (_I, [_I, _P, _I, _L, _L, _P, _P, _P]),
     "dp_pin_weight_sum": (_I, [_I, _P, _P, _P, _P, _L, _L, _P, _P]),
     "dp_optimizer_workspace_bytes": (_S, []),
     "dp_nesterov_extrapolate": (_I, [_I, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

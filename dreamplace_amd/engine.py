@@ -106,7 +106,7 @@ class DeviceDesign:
         sx, sy = self.node_size_x[n - nf:], self.node_size_y[n - nf:]
         return bool((sx == sx[0]).all().item() and (sy == sy[0]).all().item())
 
-    def reorder_fillers(self, perm=None, origin_offset=(0.0, 0.0)):
+    def reorder_fillers(self, perm=None, origin_offset=(0.0, 0.0), extra=()):
         """Relabel the fillers so that those in one 64x64-bin tile of the
         density kernels are consecutive: the tiled scatter's records and the
         field gather's gradient stores then run in node order instead of
@@ -114,7 +114,8 @@ class DeviceDesign:
         objective is unchanged and each filler's gradient moves with it.
         `perm` (filler-relative, new slot -> old slot) defaults to the tile
         order of the current positions (plus origin_offset, the density
-        kernels' stretch of the fillers' origin).  Applies it to `pos`, keeps the
+        kernels' stretch of the fillers' origin).  Applies it to `pos` and to
+        the [2N] vectors in `extra` (in the same passes), keeps the
         accumulated relabelling in `filler_perm` (slot -> generated filler)
         and returns `perm` for the caller's other per-node vectors
         (permute_fillers_)."""
@@ -124,11 +125,18 @@ class DeviceDesign:
         if not self.uniform_fillers():
             raise ValueError("reorder_fillers: fillers of different sizes are not interchangeable")
         if perm is None:
-            perm = filler_tile_order(self.pos, n, nf, self.xl, self.yl, self.bin_size_x,
-                                     self.bin_size_y, self.num_bins_x, self.num_bins_y,
-                                     origin_offset)
+            if self.pos.is_cuda:
+                keys, _ = filler_tile_keys_device(self.pos, n, nf, self.xl, self.yl,
+                                                  self.bin_size_x, self.bin_size_y,
+                                                  self.num_bins_x, self.num_bins_y, origin_offset,
+                                                  counts=False)
+                perm = torch.argsort(keys, stable=True)
+            else:
+                perm = filler_tile_order(self.pos, n, nf, self.xl, self.yl, self.bin_size_x,
+                                         self.bin_size_y, self.num_bins_x, self.num_bins_y,
+                                         origin_offset)
         perm = perm.to(device=self.device, dtype=torch.int64)
-        permute_fillers_(self.t["pos"], n, nf, perm)
+        permute_fillers_many_([self.t["pos"]] + list(extra), n, nf, perm)
         fp = self.__dict__.get("filler_perm")
         self.filler_perm = perm.clone() if fp is None else fp[perm]
         return perm
@@ -250,9 +258,51 @@ def filler_tile_order(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_
 def permute_fillers_(v, num_nodes, num_fillers, perm):
     """In place: v's filler entries (both halves of a [2N] vector) take the
     order perm (filler-relative, new slot -> old slot)."""
-    n, f0 = int(num_nodes), int(num_nodes) - int(num_fillers)
-    v2 = v.view(2, n)
-    v2[:, f0:] = v2[:, f0:][:, perm]
+    permute_fillers_many_([v], num_nodes, num_fillers, perm)
+
+
+def permute_fillers_many_(vectors, num_nodes, num_fillers, perm):
+    """permute_fillers_ for several [2N] vectors: on the device one gather
+    and one store pass per four vectors (dp_permute_fillers, bit-identical
+    to the indexing below, which host tensors take)."""
+    n, nf = int(num_nodes), int(num_fillers)
+    vectors = [v for v in vectors if v is not None]
+    if nf == 0 or not vectors:
+        return
+    if not vectors[0].is_cuda:
+        f0 = n - nf
+        for v in vectors:
+            v2 = v.view(2, n)
+            v2[:, f0:] = v2[:, f0:][:, perm]
+        return
+    perm = perm.to(device=vectors[0].device, dtype=torch.int64).contiguous()
+    for k0 in range(0, len(vectors), 4):
+        group = vectors[k0:k0 + 4]
+        dt = group[0].dtype
+        for v in group:
+            if v.dtype != dt or v.numel() != 2 * n or not v.is_contiguous() or not v.is_cuda:
+                raise ValueError("permute_fillers_many_: contiguous [2N] device vectors of one dtype")
+        code = N.dtype_code(group[0])
+        ws = torch.empty(N.query("dp_permute_fillers_workspace_bytes", code, nf, len(group)),
+                         dtype=torch.uint8, device=group[0].device)
+        ptrs = (N.ctypes.c_void_p * len(group))(*[v.data_ptr() for v in group])
+        N.call("dp_permute_fillers", code, ptrs, len(group), n, nf, N.ptr(perm), N.ptr(ws),
+               N.stream_ptr(group[0].device))
+
+
+def filler_tile_keys_device(pos, num_nodes, num_fillers, xl, yl, bin_size_x, bin_size_y,
+                            num_bins_x, num_bins_y, origin_offset=(0.0, 0.0), counts=True):
+    """filler_tile_keys on the device in one pass (dp_filler_tile_keys): the
+    same keys as int16, and with `counts` the fillers per tile (int32)."""
+    n, nf = int(num_nodes), int(num_fillers)
+    ntiles = ((int(num_bins_x) + 63) // 64) * ((int(num_bins_y) + 63) // 64)
+    keys = torch.empty(max(nf, 1), dtype=torch.int16, device=pos.device)[:nf]
+    cnt = torch.empty(ntiles, dtype=torch.int32, device=pos.device) if counts else None
+    ox, oy = origin_offset
+    N.call("dp_filler_tile_keys", N.dtype_code(pos), N.ptr(pos.contiguous()), n, nf, float(xl),
+           float(yl), float(bin_size_x), float(bin_size_y), int(num_bins_x), int(num_bins_y),
+           float(ox), float(oy), N.ptr(keys), N.ptr(cnt), N.stream_ptr(pos.device))
+    return keys, cnt
 
 
 def check_netlist(flat_netpin, netpin_start, pin2net, flat_node2pin, node2pin_start, num_pins,
@@ -487,10 +537,11 @@ class PlacementObjective:
             return False
         M, Nb = ep.num_bins_x, ep.num_bins_y
         ntiles = ((M + 63) // 64) * ((Nb + 63) // 64)
-        keys = filler_tile_keys(pos, dd.num_nodes, nf, float(ep.xl), float(ep.yl),
-                                float(ep.bin_size_x), float(ep.bin_size_y), M, Nb, (uf[2], uf[3]))
+        _, counts = filler_tile_keys_device(pos, dd.num_nodes, nf, float(ep.xl), float(ep.yl),
+                                            float(ep.bin_size_x), float(ep.bin_size_y), M, Nb,
+                                            (uf[2], uf[3]))
         fr = torch.zeros(ntiles + 1, dtype=torch.int32, device=pos.device)
-        fr[1:] = torch.cumsum(torch.bincount(keys, minlength=ntiles), 0)
+        fr[1:] = torch.cumsum(counts, 0)
         self.filler_ranges = fr
         return True
 

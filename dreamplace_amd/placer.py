@@ -24,8 +24,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .engine import (DeviceDesign, PlacementObjective, can_track_metrics, movable_netlist_order,
-                     permute_fillers_)
+from .engine import DeviceDesign, PlacementObjective, can_track_metrics, movable_netlist_order
 from .nesterov import NesterovAcceleratedGradientOptimizer
 
 DEFAULTS = dict(gamma=4.0, density_weight=8e-5, stop_overflow=0.1, learning_rate=0.01,
@@ -293,14 +292,15 @@ class GlobalPlacer:
         takes the fillers by tile range from then on (no binning of them)."""
         dd = self.dd
         ob = self.objective
-        # relabels dd.pos, which is self.pos
-        perm = dd.reorder_fillers(origin_offset=ob.filler_origin_offset())
-        ob.update_filler_ranges(dd.pos)
-        n, nf = dd.num_nodes, dd.num_filler_nodes
-        fn = lambda v: permute_fillers_(v, n, nf, perm)
+        # the optimizer's per-node state and the best positions (when they
+        # are not the current ones) take the permutation in the same passes
+        # as dd.pos, which is self.pos
+        extra = []
         if self.optimizer is not None:
-            self.optimizer.permute_state_(fn, include_param=False)
-        self.best_positions.permute_(fn)
+            self.optimizer.permute_state_(extra.append, include_param=False)
+        self.best_positions.permute_(extra.append)
+        dd.reorder_fillers(origin_offset=ob.filler_origin_offset(), extra=extra)
+        ob.update_filler_ranges(dd.pos)
         self.filler_sorts += 1
 
     def boundary_box(self):

@@ -627,6 +627,38 @@ int dp_precondition(int dtype, void* grad, int64_t num_nodes, const void* pin_we
                     const void* node_areas, const void* density_weight, double alpha,
                     void* stream);
 
+/* ---- Filler relabelling ----------------------------------------------------
+ * No counterpart in the reference: fillers are interchangeable (one size, no
+ * pins, PlaceDB.py:917-936), and this library keeps them in the tiled density
+ * kernels' tile order so that those kernels take them by tile range.  A
+ * placement loop re-sorts them every few tens of iterations; these two calls
+ * are the sort's device passes. */
+
+/* key[j] (int16) = tile of filler j (node num_nodes - num_fillers + j):
+ * (bx / 64) * ceil(num_bins_y / 64) + by / 64, the tile numbering of the
+ * tiled density kernels, for bx = clamp(floor(((x + origin_x) - xl) *
+ * (1 / bin_size_x)), 0, num_bins_x - 1) evaluated in `dtype` in that order
+ * (origin: the density kernels' stretch offset of a filler's origin), by
+ * likewise.  tile_count (nullable, int32 [ntiles]): cleared, then the
+ * fillers per tile.  At most 4096 tiles (DP_ERR_UNSUPPORTED beyond).  A
+ * stable sort of the keys gives the tile order; the prefix sums of the
+ * counts give dp_density_map_ex's filler_tile_start. */
+int dp_filler_tile_keys(int dtype, const void* pos, int64_t num_nodes, int64_t num_fillers,
+                        double xl, double yl, double bin_size_x, double bin_size_y,
+                        int num_bins_x, int num_bins_y, double origin_x, double origin_y,
+                        int16_t* keys, int32_t* tile_count, void* stream);
+
+/* In place, for `count` (at most 8) [2 * num_nodes] vectors of `dtype`
+ * (vectors: a host array of device pointers): the filler entries of both
+ * halves take the order perm (int64 [num_fillers], a permutation of
+ * [0, num_fillers), new slot -> old slot): v[h*N + f0 + j] <- old
+ * v[h*N + f0 + perm[j]], f0 = num_nodes - num_fillers.  One gather into the
+ * workspace (dp_permute_fillers_workspace_bytes(dtype, num_fillers, count))
+ * and one store back. */
+size_t dp_permute_fillers_workspace_bytes(int dtype, int64_t num_fillers, int count);
+int dp_permute_fillers(int dtype, void* const* vectors, int count, int64_t num_nodes,
+                       int64_t num_fillers, const int64_t* perm, void* workspace, void* stream);
+
 /* ---- Bookshelf input / output (host only) ------------------------------- */
 /* Node orientations and pin directions as PlaceDB names them. */
 enum {

@@ -194,6 +194,57 @@ def test_engine_filler_tile_order(gpu, dtype):
     assert torch.equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("origin", [(0.0, 0.0), (0.37, -0.21)], ids=["origin0", "origin-shifted"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("cfg,kw", [(0, {}), (1, {"scale": 0.01}), (4, {"scale": 0.004, "max_fillers": 200000})],
+                         ids=["config0", "config1-small", "config4-small"])
+def test_filler_tile_keys_native(gpu, cfg, kw, dtype, origin):
+    """dp_filler_tile_keys gives the keys of the indexing form (including
+    fillers on and beyond the die's edges, which clamp), per-tile counts
+    equal to a bincount, and the same stable order."""
+    from dreamplace_amd.engine import filler_tile_keys, filler_tile_keys_device
+    print("case filler tile keys cfg%d %s %s" % (cfg, np.dtype(dtype).name, origin), flush=True)
+    d = design(cfg, dtype=dtype, **kw)
+    n, nf = d.num_nodes, d.num_filler_nodes
+    pos = torch.from_numpy(np.ascontiguousarray(d.pos)).to(gpu)
+    # a few fillers pushed past each edge of the die
+    f0 = n - nf
+    pos[f0:f0 + 3] = torch.tensor([d.xl - 50.0, d.xh + 50.0, d.xh], dtype=pos.dtype)
+    pos[n + f0 + 3:n + f0 + 6] = torch.tensor([d.yl - 7.0, d.yh + 7.0, d.yh], dtype=pos.dtype)
+    args = (pos, n, nf, float(d.xl), float(d.yl), float(d.bin_size_x), float(d.bin_size_y),
+            d.num_bins_x, d.num_bins_y, origin)
+    ref = filler_tile_keys(*args)
+    keys, counts = filler_tile_keys_device(*args)
+    torch.cuda.synchronize()
+    ntiles = ((d.num_bins_x + 63) // 64) * ((d.num_bins_y + 63) // 64)
+    assert keys.dtype == torch.int16 and keys.numel() == nf
+    assert torch.equal(keys.long(), ref)
+    assert torch.equal(counts.long(), torch.bincount(ref, minlength=ntiles))
+    assert torch.equal(torch.argsort(keys, stable=True), torch.argsort(ref, stable=True))
+
+
+@pytest.mark.parametrize("count", [1, 3, 6])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_permute_fillers_native(gpu, dtype, count):
+    """dp_permute_fillers on several [2N] vectors (in groups of four) gives
+    the indexing form's bits; the movable and fixed entries stay."""
+    from dreamplace_amd.engine import permute_fillers_many_
+    print("case permute fillers %s x%d" % (dtype, count), flush=True)
+    n, nf = 100003, 77777
+    g = torch.Generator(device="cpu").manual_seed(11)
+    vecs = [torch.randn(2 * n, generator=g, dtype=torch.float64).to(gpu, dtype) for _ in range(count)]
+    perm = torch.randperm(nf, generator=g).to(gpu)
+    want = []
+    for v in vecs:
+        w = v.clone().view(2, n)
+        w[:, n - nf:] = w[:, n - nf:][:, perm]
+        want.append(w.view(-1))
+    permute_fillers_many_(vecs, n, nf, perm)
+    torch.cuda.synchronize()
+    for v, w in zip(vecs, want):
+        assert torch.equal(v, w)
+
+
 @pytest.mark.parametrize("arith", ["reference", "bin_relative"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 def test_tracked_metrics_equal_the_separate_passes(gpu, dtype, arith):
