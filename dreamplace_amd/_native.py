@@ -86,8 +86,8 @@ _SIGNATURES = {
                                  _P, _P, _P, _P, _P, _P, _P]),
     "dp_wa_pins_slot_weights": (_I, [_I, _L, _P, _P, _P, _P, _P]),
     "dp_node_grad_from_pins": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P]),
-    "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _P, _P,
-                                         _P, _P, _P]),
+    "dp_node_grad_preconditioned": (_I, [_I, _P, _P, _P, _P, _I, _L, _L, _P, _P, _P, _D, _D, _P,
+                                         _P, _P, _P, _P]),
     "dp_hpwl": (_I, [_I, _P, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_hpwl_from_nets": (_I, [_I, _P, _P, _L, _P, _P, _P, _P, _P]),
     "dp_move_boundary": (_I, [_I, _P, _L, _P, _P, _D, _D, _D, _D, _L, _L, _P]),

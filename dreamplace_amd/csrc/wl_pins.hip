@@ -589,7 +589,10 @@ __global__ void __launch_bounds__(256) k_wa_big(
 // grad[n] = (*grad_scale) * sum of gnode over node n's CSR range + add[n]
 // (add nullable, split or interleaved).  Fillers (n >= nphys) get add[n] only.
 // kPre: then divided by max(pws[n] + (alpha * *dw) * area[n], 1), the
-// preconditioner of PlaceObj.py:69-102 applied to the finished gradient.
+// preconditioner of PlaceObj.py:69-102 applied to the finished gradient;
+// with farea >= 0 the fillers (n >= nphys: no pins, so pws[n] = 0, and one
+// area) take farea without loading either array (0 + c * farea is the same
+// value).
 // Thread t takes nodes t, t + grid, ...; with `partials` it also sums
 // (grad - g_prev)^2 over x and y in FP64, in k_step_norms's order (the
 // step-size denominator of step_nobb, NesterovAcceleratedGradientOptimizer.py:131).
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(256) k_node_grad(
     const typename Vec2<T>::type* __restrict__ gnode, const int* __restrict__ node2pin_start,
     const T* __restrict__ grad, const T* __restrict__ add, int add_interleaved, int64_t nphys,
     int64_t nnodes, T* gx_out, T* gy_out, const T* __restrict__ pws, const T* __restrict__ area,
-    const T* __restrict__ dw, T alpha, const T* __restrict__ g_prev, double* partials) {
+    const T* __restrict__ dw, T alpha, T farea, const T* __restrict__ g_prev, double* partials) {
   __shared__ double smem[4];
   double sgg = 0;
   const T g = *grad;
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(256) k_node_grad(
     }
     if (kPre) {
       const T c = alpha * *dw;
-      T p = pws[n] + c * area[n];
+      T p = (farea >= T(0) && n >= nphys) ? c * farea : pws[n] + c * area[n];
       p = p < T(1) ? T(1) : p;
       sx = sx / p;
       sy = sy / p;
@@ -889,8 +892,8 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
                      const int32_t* flat_node2pin_start_map, const void* add, int add_layout,
                      int64_t num_physical_nodes, int64_t num_nodes, void* grad_pos_out,
                      const void* pws, const void* area, const void* dw, double alpha,
-                     const void* grad_prev, void* workspace, double* dg_norm2_out,
-                     void* stream) {
+                     double filler_area, const void* grad_prev, void* workspace,
+                     double* dg_norm2_out, void* stream) {
   DP_ARG(grad && grad_node_order && flat_node2pin_start_map && grad_pos_out,
          "%s: null pointer", name);
   DP_ARG(!dg_norm2_out || (grad_prev && workspace), "%s: the norm needs grad_prev and a workspace",
@@ -914,7 +917,7 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
                                  (const T*)add, add_layout == DP_LAYOUT_INTERLEAVED,
                                  num_physical_nodes, num_nodes, out, out + num_nodes,
                                  (const T*)pws, (const T*)area, (const T*)dw, (T)alpha,
-                                 (const T*)grad_prev, part));
+                                 (T)filler_area, (const T*)grad_prev, part));
     DP_LAUNCHED("k_node_grad");
     if (part) return finish_reduction(part, nb, 1, nullptr, dg_norm2_out, -1, s);
     return DP_OK;
@@ -927,22 +930,22 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
                            void* grad_pos_out, void* stream) {
   return node_grad("dp_node_grad_from_pins", dtype, grad, grad_node_order, flat_node2pin_start_map,
                    add, add_layout, num_physical_nodes, num_nodes, grad_pos_out, nullptr, nullptr,
-                   nullptr, 0.0, nullptr, nullptr, nullptr, stream);
+                   nullptr, 0.0, -1.0, nullptr, nullptr, nullptr, stream);
 }
 
 int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
                                 const int32_t* flat_node2pin_start_map, const void* add,
                                 int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                                 const void* pin_weight_sum, const void* node_areas,
-                                const void* density_weight, double alpha, void* grad_pos_out,
-                                const void* grad_prev, void* workspace, double* dg_norm2_out,
-                                void* stream) {
+                                const void* density_weight, double alpha, double filler_area,
+                                void* grad_pos_out, const void* grad_prev, void* workspace,
+                                double* dg_norm2_out, void* stream) {
   DP_ARG(pin_weight_sum && node_areas && density_weight,
          "dp_node_grad_preconditioned: null pointer");
   return node_grad("dp_node_grad_preconditioned", dtype, grad, grad_node_order,
                    flat_node2pin_start_map, add, add_layout, num_physical_nodes, num_nodes,
-                   grad_pos_out, pin_weight_sum, node_areas, density_weight, alpha, grad_prev,
-                   workspace, dg_norm2_out, stream);
+                   grad_pos_out, pin_weight_sum, node_areas, density_weight, alpha, filler_area,
+                   grad_prev, workspace, dg_norm2_out, stream);
 }
 
 }  // extern "C"

@@ -685,9 +685,12 @@ class PlacementObjective:
 
     def evaluate(self, pos, precondition=None, grad_out=None, grad_ref=None):
         """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers.
-        precondition = (pin_weight_sum, node_areas, alpha) also applies
-        PlaceObj's preconditioner to the gradient (with the current density
-        weight), fused into the node-gradient pass on the pins schedule.
+        precondition = (pin_weight_sum, node_areas, alpha[, filler_area])
+        also applies PlaceObj's preconditioner to the gradient (with the
+        current density weight), fused into the node-gradient pass on the pins
+        schedule; filler_area >= 0 states that every filler has that area
+        (and, having no pins, a zero pin-weight sum), so that pass reads
+        neither array for them.
         grad_out: a contiguous [2N] tensor of the positions' type and device
         that receives the gradient (and is returned) instead of the
         objective's own buffer, so a caller that keeps gradients from step to
@@ -749,7 +752,9 @@ class PlacementObjective:
             if self.track_metrics:
                 self._reduce_metrics(pos, s)
             if precondition is not None:
-                pws, areas, alpha = precondition
+                pws, areas, alpha = precondition[:3]
+                # the fillers' one area, when the caller knows it (else -1)
+                farea = precondition[3] if len(precondition) > 3 else -1.0
                 fuse = grad_ref is not None and grad_ref.data_ptr() != grad.data_ptr()
                 if fuse and self._dg2 is None:
                     self._dg2 = torch.empty(1, dtype=torch.float64, device=pos.device)
@@ -758,8 +763,8 @@ class PlacementObjective:
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),
                        N.ptr(pp.grad_node_order), N.ptr(dd.flat_node2pin_start_map),
                        N.ptr(self._density_grad), self.grad_layout, dd.num_physical_nodes, n,
-                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), N.ptr(grad),
-                       N.ptr(grad_ref) if fuse else None, N.ptr(self._norm_ws) if fuse else None,
+                       N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), float(farea),
+                       N.ptr(grad), N.ptr(grad_ref) if fuse else None, N.ptr(self._norm_ws) if fuse else None,
                        N.ptr(self._dg2) if fuse else None, s)
                 if fuse:
                     self.last_dg2 = self._dg2
@@ -799,7 +804,7 @@ class PlacementObjective:
                    dd.num_physical_nodes, n, N.ptr(self.wl_grad), s)
             torch.add(self.wl_grad, self._density_grad, out=grad)
         if precondition is not None:
-            pws, areas, alpha = precondition
+            pws, areas, alpha = precondition[:3]
             N.call("dp_precondition", code, N.ptr(grad), n, N.ptr(pws), N.ptr(areas),
                    N.ptr(self.lam), float(alpha), s)
         energy = self._energy(pos)

@@ -226,6 +226,11 @@ class GlobalPlacer:
         n, dev, code = dd.num_nodes, dd.device, N.dtype_code(dd.pos)
         self.code = code
         self.areas = (dd.node_size_x * dd.node_size_y).contiguous()
+        # fillers of one size have one area (and no pins): the preconditioned
+        # gradient pass then reads neither array for them
+        nf = dd.num_filler_nodes
+        self.filler_area = (float(self.areas[n - nf].item()) if nf > 0 and dd.uniform_fillers()
+                            else -1.0)
         self.pws = torch.empty(n, dtype=dd.dtype, device=dev)
         N.call("dp_pin_weight_sum", code, N.ptr(dd.net_weights), N.ptr(dd.flat_node2pin_map),
                N.ptr(dd.flat_node2pin_start_map), N.ptr(dd.pin2net_map), dd.num_physical_nodes, n,
@@ -338,7 +343,7 @@ class GlobalPlacer:
         (the gradient into grad_out when given).  With grad_ref it returns
         (objective, gradient, sum((gradient - grad_ref)^2) as a device FP64
         scalar formed in the gradient's own pass, or None where it is not)."""
-        pre = (self.pws, self.areas, self.p["precondition_alpha"])
+        pre = (self.pws, self.areas, self.p["precondition_alpha"], self.filler_area)
         f, g = self.objective.evaluate(x, precondition=pre, grad_out=grad_out, grad_ref=grad_ref)
         if self.init_density is None:
             # PlaceObj.obj_fn records the density of its first evaluation

@@ -530,14 +530,17 @@ int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_or
  * a [2N] gradient not aliasing the output, and a dp_optimizer_workspace_bytes
  * workspace are needed): sum((grad_pos_out - grad_prev)^2), the Lipschitz
  * step's denominator (NesterovAcceleratedGradientOptimizer.py:131), formed in
- * the same pass with the bits dp_step_norms gives. */
+ * the same pass with the bits dp_step_norms gives.  filler_area >= 0: the
+ * fillers (nodes num_physical_nodes .. num_nodes - 1, which have no pins, so
+ * their pin_weight_sum is 0) all have this area, and neither array is read
+ * for them (same results); negative: every node's entries are read. */
 int dp_node_grad_preconditioned(int dtype, const void* grad, const void* grad_node_order,
                                 const int32_t* flat_node2pin_start_map, const void* add,
                                 int add_layout, int64_t num_physical_nodes, int64_t num_nodes,
                                 const void* pin_weight_sum, const void* node_areas,
-                                const void* density_weight, double alpha, void* grad_pos_out,
-                                const void* grad_prev, void* workspace, double* dg_norm2_out,
-                                void* stream);
+                                const void* density_weight, double alpha, double filler_area,
+                                void* grad_pos_out, const void* grad_prev, void* workspace,
+                                double* dg_norm2_out, void* stream);
 
 /* Replaces hpwl_cuda.forward (dreamplace/ops/hpwl/src/hpwl_cuda.cpp:27-64):
  * *hpwl_out = sum_n w_n*(max-min)_x + (max-min)_y over unmasked nets, FP64

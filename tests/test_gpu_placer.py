@@ -233,6 +233,11 @@ def test_fused_preconditioner_equals_separate(gpu, dtype):
            N.stream_ptr(gpu))
     _, g2 = ob.evaluate(dd.pos, precondition=(pws, areas, 1.0))
     assert torch.equal(g2, sep)
+    # the fillers' one area given instead of read per node: the same bits
+    nf = dd.num_filler_nodes
+    assert nf > 0 and dd.uniform_fillers()
+    _, g3 = ob.evaluate(dd.pos, precondition=(pws, areas, 1.0, float(areas[n - nf].item())))
+    assert torch.equal(g3, sep)
 
 
 def test_filler_sorting_keeps_the_loop(gpu):
