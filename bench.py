@@ -22,6 +22,11 @@ per-cell density gradient and total gradient (l2err_step_*).  The solve
 alone on one map (l2err_solve_*) and the reference-arithmetic form's whole
 step (l2err_step_*_reference_arith) are reported beside it.
 
+The placement loop also re-sorts the fillers into the density kernels' tile
+order every 25 steps; that pass is timed on the same design and reported
+with the step it amortizes over (ms_per_filler_resort,
+ms_per_step_with_resort_every_25).
+
 Roofline: each op of the step (one C-ABI entry point, all of its kernels) is
 timed live with HIP events recorded by the library around its launches on
 its stream; achieved GB/s = the op's algorithmic bytes / its duration.
@@ -127,6 +132,24 @@ def time_steps(torch, obj, pos, steps, warmup):
     wall = (time.perf_counter() - t0) * 1e3
     per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     return wall / steps, per
+
+
+def resort_ms(torch, dd, obj, reps):
+    """Median ms of the placer's filler re-sort (GlobalPlacer.sort_fillers)
+    on this design: dd.reorder_fillers with five more [2N] vectors (the
+    optimizer's four and the best positions) and the objective's filler
+    ranges.  At the bench's fixed positions the order found is the current
+    one, which the passes do not exploit."""
+    extra = [dd.pos.clone() for _ in range(5)]
+    times = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dd.reorder_fillers(origin_offset=obj.filler_origin_offset(), extra=extra)
+        obj.update_filler_ranges(dd.pos)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(times[1:])
 
 
 def probe(torch, N, obj, pos, pid, reps):
@@ -764,6 +787,15 @@ def main():
         obj32.keep_partial_wl = False
         log("[bench] fp32/%s with the net-weight gradient: %.3f ms/step" % (args.precision, ms_nw))
         result.update(ms_per_step_with_net_weight_grad=round(ms_nw, 4))
+    if args.filler_order == "tile":
+        # what the placement loop adds per step: the fillers' re-sort into
+        # tile order every 25 steps (keys, sort, the positions and the four
+        # optimizer vectors and the best positions permuted, new filler
+        # ranges), amortized over those 25 steps
+        ms_rs = resort_ms(torch, dd32, obj32, 3)
+        result.update(ms_per_filler_resort=round(ms_rs, 4),
+                      ms_per_step_with_resort_every_25=round(ms32 + ms_rs / 25.0, 4))
+        log("[bench] filler re-sort: %.3f ms (%.4f ms per step every 25 steps)" % (ms_rs, ms_rs / 25))
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)  # already in dd32's order
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",

@@ -36,10 +36,10 @@ __global__ void __launch_bounds__(256) k_filler_keys(const T* __restrict__ x,
        j += (int64_t)gridDim.x * blockDim.x) {
     T bx = floor(((x[j] + ox) - xl) * inv_x);
     T by = floor(((y[j] + oy) - yl) * inv_y);
-    bx = bx < (T)0 ? (T)0 : bx;
-    bx = bx > mx ? mx : bx;
-    by = by < (T)0 ? (T)0 : by;
-    by = by > my ? my : by;
+    bx = bx > (T)0 ? bx : (T)0;  // a NaN position takes tile 0
+    bx = bx < mx ? bx : mx;
+    by = by > (T)0 ? by : (T)0;
+    by = by < my ? by : my;
     const int k = ((int)bx / kKeyTile) * nty + (int)by / kKeyTile;
     keys[j] = (int16_t)k;
     if (count) atomicAdd(&hist[k], 1);
@@ -61,7 +61,9 @@ __global__ void k_permute_gather(PermuteVectors pv, int64_t n, int64_t f0, int64
                                  const int64_t* __restrict__ perm, T* tmp) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nf;
        j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t src = f0 + perm[j];
+    int64_t q = perm[j];
+    q = q < 0 ? 0 : (q >= nf ? nf - 1 : q);  // in bounds whatever the caller passed
+    const int64_t src = f0 + q;
     for (int k = 0; k < pv.count; ++k) {
       const T* v = (const T*)pv.v[k];
       tmp[(2 * k) * nf + j] = v[src];

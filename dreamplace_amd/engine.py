@@ -135,6 +135,11 @@ class DeviceDesign:
                 perm = filler_tile_order(self.pos, n, nf, self.xl, self.yl, self.bin_size_x,
                                          self.bin_size_y, self.num_bins_x, self.num_bins_y,
                                          origin_offset)
+        else:
+            perm = torch.as_tensor(perm)
+            if perm.numel() != nf or not torch.equal(torch.sort(perm.long().cpu()).values,
+                                                     torch.arange(nf)):
+                raise ValueError("reorder_fillers: perm must be a permutation of the %d fillers" % nf)
         perm = perm.to(device=self.device, dtype=torch.int64)
         permute_fillers_many_([self.t["pos"]] + list(extra), n, nf, perm)
         fp = self.__dict__.get("filler_perm")

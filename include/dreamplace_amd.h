@@ -657,7 +657,9 @@ int dp_filler_tile_keys(int dtype, const void* pos, int64_t num_nodes, int64_t n
  * [0, num_fillers), new slot -> old slot): v[h*N + f0 + j] <- old
  * v[h*N + f0 + perm[j]], f0 = num_nodes - num_fillers.  One gather into the
  * workspace (dp_permute_fillers_workspace_bytes(dtype, num_fillers, count))
- * and one store back. */
+ * and one store back.  Entries of perm outside [0, num_fillers) are clamped
+ * into it (no access leaves the vectors; the result is then not a
+ * relabelling). */
 size_t dp_permute_fillers_workspace_bytes(int dtype, int64_t num_fillers, int count);
 int dp_permute_fillers(int dtype, void* const* vectors, int count, int64_t num_nodes,
                        int64_t num_fillers, const int64_t* perm, void* workspace, void* stream);
