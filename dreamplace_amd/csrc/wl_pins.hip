@@ -275,8 +275,12 @@ __global__ void __launch_bounds__(256) k_wa_pins(
     }
     wave_sync();
     const bool is_head = on && lane == head;
-    // nets are packed largest first: lane 0 holds the group's largest degree
-    const bool tree = kMode == kWaCentred && __builtin_amdgcn_readfirstlane(deg) > kTreeDeg;
+    // nets are packed largest first: lane 0 holds the group's largest degree.
+    // A segment of d lanes is reduced into its head after the steps of
+    // offsets below d; later steps add nothing to any head (lane + o is past
+    // the segment), so the trees stop at the group's largest degree.
+    const int maxdeg = __builtin_amdgcn_readfirstlane(deg);
+    const bool tree = kMode == kWaCentred && maxdeg > kTreeDeg;
     const int seg_end = head + deg;  // one past this lane's net's last lane
     T xmax = -big_value<T>(), xmin = big_value<T>(), ymax = -big_value<T>(), ymin = big_value<T>();
     if (tree) {
@@ -286,6 +290,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       T c = on ? y : -big_value<T>(), dd = on ? y : big_value<T>();
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
+        if (o >= maxdeg) break;  // uniform
         const T a2 = __shfl_down(a, o, 64), b2 = __shfl_down(b, o, 64);
         const T c2 = __shfl_down(c, o, 64), d2 = __shfl_down(dd, o, 64);
         if (lane + o < seg_end) {
@@ -378,6 +383,7 @@ __global__ void __launch_bounds__(256) k_wa_pins(
       sum[7] = eny;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
+        if (o >= maxdeg) break;  // uniform
         const bool in = lane + o < seg_end;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
