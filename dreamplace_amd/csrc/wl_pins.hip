@@ -150,9 +150,10 @@ __global__ void __launch_bounds__(256) k_pin_push(const T* __restrict__ x, const
 //     the order of the net's span.
 constexpr int kWaRef = 0, kWaLse = 1, kWaCentred = 2;
 // kWaCentred groups whose first (largest) net has more pins than this reduce
-// their per-net extremes and sums across lanes in six shuffle steps instead
-// of the head lane's serial loops over the net's pins (the order of the sums
-// then differs from the reference's pin order, which only kWaRef keeps).
+// their per-net extremes and sums across lanes in shuffle steps (as many as
+// that net needs, at most six) instead of the head lane's serial loops over
+// the net's pins (the order of the sums then differs from the reference's
+// pin order, which only kWaRef keeps).
 constexpr int kTreeDeg = 8;
 
 // A pin's four exponentials as one aligned LDS word.
