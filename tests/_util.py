@@ -109,3 +109,19 @@ def with_dtype(d, dtype):
             d2[k] = v.astype(dtype)
     d2.dtype = np.dtype(dtype).name
     return d2
+
+
+def fine_grid(d, nb, shrink):
+    """A copy of design `d` on an nb x nb bin grid with every node (and pin
+    offset) shrunk by `shrink`, so that most cells still span at most 3 x 3
+    of the finer bins: the tiled density kernels' tile path at nb / 64 tiles
+    per side."""
+    d2 = synth.Design(d)
+    dt = d.pos.dtype
+    for k in ("node_size_x", "node_size_y", "pin_offset_x", "pin_offset_y"):
+        d2[k] = (d[k].astype(np.float64) * shrink).astype(d[k].dtype)
+    bsx, bsy = (d.xh - d.xl) / nb, (d.yh - d.yl) / nb
+    d2.update(num_bins_x=nb, num_bins_y=nb, bin_size_x=bsx, bin_size_y=bsy, wa_gamma=4.0 * bsx)
+    d2.bin_center_x = synth._bin_centers(d.xl, d.xh, bsx, nb).astype(dt)
+    d2.bin_center_y = synth._bin_centers(d.yl, d.yh, bsy, nb).astype(dt)
+    return d2

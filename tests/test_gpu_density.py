@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from _util import density_params, design, off_grid, to_dev
+from _util import density_params, design, fine_grid, off_grid, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -216,3 +216,40 @@ def test_backward_after_rebinning(gpu):
             op(other, mode="overflow")
         e.backward()
         assert torch.equal(pos.grad, ref), interleave
+
+
+@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_tiled_at_the_tile_limit(oracle, gpu, dtype, arith):
+    """4096 x 4096 bins, 4096 tiles (the tiled kernels' limit), 3.3M cells
+    shrunk to span at most 3 x 3 bins: the tiled scatter's int64 sum and
+    converted map are the oracle's bits and the tiled gather is the oracle's
+    force within rounding."""
+    from dreamplace_amd.electric_potential import ElectricPotential
+    print("case tiled density at 4096 tiles %s %s" % (np.dtype(dtype).name, arith), flush=True)
+    d = fine_grid(design(1, dtype=dtype, scale=0.25), 4096, 0.2)
+    t = to_dev(d, gpu)
+    op = ElectricPotential(
+        t["node_size_x"], t["node_size_y"], t["bin_center_x"], t["bin_center_y"],
+        d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+        d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+        t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"],
+        algorithm="tiled", arithmetic=arith)
+    op.compute_initial_density_map(t["pos"])
+    nb = d.num_bins_x * d.num_bins_y
+    raw = torch.empty(nb, dtype=torch.int64, device=gpu)
+    dmap = op.density_map(t["pos"], fixed_map_out=raw)
+    torch.cuda.synchronize()
+    dp = density_params(oracle, d)
+    kw = dict(arithmetic=arith)
+    assert np.array_equal(raw.cpu().numpy(), oracle.density_fixed_sum(dp, d.pos, **kw))
+    init = oracle.fixed_density_map(dp, d.pos, d.bin_center_x, d.bin_center_y, **kw)
+    assert np.array_equal(dmap.cpu().numpy(), oracle.density_map(dp, d.pos, init, **kw))
+    rng = np.random.default_rng(11)
+    fx = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    fy = rng.standard_normal((d.num_bins_x, d.num_bins_y)).astype(dtype)
+    g = torch.tensor(0.75, dtype=t["pos"].dtype, device=gpu)
+    out = op.electric_force(t["pos"], torch.from_numpy(fx).to(gpu), torch.from_numpy(fy).to(gpu), g)
+    ref = oracle.electric_force(dp, d.pos, fx, fy, grad=0.75, **kw)
+    tol = 1e-12 if dtype == np.float64 else 1e-6
+    assert np.max(np.abs(out.cpu().numpy() - ref)) <= tol * np.abs(ref).max()

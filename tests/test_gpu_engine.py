@@ -291,17 +291,14 @@ def test_tracked_metrics_equal_the_separate_passes(gpu, dtype, arith):
     assert f1.item() == f2.item() and torch.equal(g1, g2)
 
 
-@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
-@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-def test_filler_ranges_are_exact(gpu, dtype, arith):
-    """Fillers taken by tile range instead of binned: the same density map,
-    objective and gradient bits with freshly sorted fillers, after the fillers
-    have moved (up to 3 bins: some leave their tile's window and take the
-    global path), and with ranges that are not a partition (replaced by one
-    range); with the movable-only map of the tracked metrics too."""
+def _check_filler_ranges(d, gpu, dtype, arith):
+    """Fillers taken by tile range instead of binned give the density map,
+    objective and gradient bits of the binned path: freshly sorted, after
+    the fillers have moved (up to 3 bins: some leave their tile's window and
+    take the global path), and with ranges that are not a partition
+    (replaced by one range); with the movable-only map of the tracked
+    metrics too."""
     from dreamplace_amd.engine import DeviceDesign, PlacementObjective
-    print("case engine filler ranges %s %s" % (np.dtype(dtype).name, arith), flush=True)
-    d = design(1, dtype=dtype, scale=0.01)  # 102 x 102 bins: 2 x 2 tiles
     prec = "bf16x3" if dtype == np.float32 else "fp64"
     dd = DeviceDesign(d, gpu)
     kw = dict(precision=prec, algorithm="tiled", schedule="pins", track_metrics=True,
@@ -333,6 +330,24 @@ def test_filler_ranges_are_exact(gpu, dtype, arith):
     rng.filler_ranges[1] = bad[-1] + 5  # not monotone, past the end
     same()
     rng.filler_ranges.copy_(bad)
+
+
+@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_filler_ranges_are_exact(gpu, dtype, arith):
+    """_check_filler_ranges on 102 x 102 bins (2 x 2 tiles)."""
+    print("case engine filler ranges %s %s" % (np.dtype(dtype).name, arith), flush=True)
+    _check_filler_ranges(design(1, dtype=dtype, scale=0.01), gpu, dtype, arith)
+
+
+def test_filler_ranges_at_the_tile_limit(gpu):
+    """_check_filler_ranges on 4096 x 4096 bins, 4096 tiles (the tiled
+    kernels' limit), with 3.3M cells shrunk so that every one spans at most
+    3 x 3 bins (FP32, bin-relative arithmetic, BF16x3 solve)."""
+    from _util import fine_grid
+    print("case engine filler ranges at 4096 tiles", flush=True)
+    d = fine_grid(design(1, dtype=np.float32, scale=0.25), 4096, 0.2)
+    _check_filler_ranges(d, gpu, np.float32, "bin_relative")
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
