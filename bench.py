@@ -25,7 +25,9 @@ step (l2err_step_*_reference_arith) are reported beside it.
 The placement loop also re-sorts the fillers into the density kernels' tile
 order every 25 steps; that pass is timed on the same design and reported
 with the step it amortizes over (ms_per_filler_resort,
-ms_per_step_with_resort_every_25).
+ms_per_step_with_resort_every_25), and so is the step at positions that
+have drifted from the sorted order by half a bin
+(ms_per_step_drifted_half_bin).
 
 Roofline: each op of the step (one C-ABI entry point, all of its kernels) is
 timed live with HIP events recorded by the library around its launches on
@@ -796,6 +798,17 @@ def main():
         result.update(ms_per_filler_resort=round(ms_rs, 4),
                       ms_per_step_with_resort_every_25=round(ms32 + ms_rs / 25.0, 4))
         log("[bench] filler re-sort: %.3f ms (%.4f ms per step every 25 steps)" % (ms_rs, ms_rs / 25))
+        # between re-sorts the cells drift from the tile order: the same step
+        # at positions moved by a normal step of 0.5 bin per coordinate (some
+        # fillers leave their tile's window and take the drift list)
+        gen = torch.Generator(device="cpu").manual_seed(1)
+        b = float(d.bin_size_x)
+        pos_d = (pos32.double().cpu() + 0.5 * b * torch.randn(pos32.numel(), generator=gen,
+                                                               dtype=torch.float64)).to(pos32)
+        ms_d, _ = time_steps(torch, obj32, pos_d, max(args.steps // 2, 5), 2)
+        del pos_d
+        result.update(ms_per_step_drifted_half_bin=round(ms_d, 4))
+        log("[bench] fp32/%s at positions drifted by 0.5 bin: %.3f ms/step" % (args.precision, ms_d))
     if not args.skip_fp64:
         dd64 = DeviceDesign(d, dev, dtype=torch.float64)  # already in dd32's order
         obj64 = PlacementObjective(dd64, gamma, density_weight=lam, precision="fp64",
