@@ -74,13 +74,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def op_bytes(d, schedule):
+def op_bytes(d, schedule, fillers_final=False):
     """Algorithmic HBM bytes of one call of each HBM-bound op, from the
     per-unit figures of the hot-path analysis (FP32 data, int32 indices):
     pin_pos fwd 28 B/pin; WA fwd (stored derivative) 20 B/pin + 13 B/net;
     scatter 28 B/cell + 20 B/bin; force 36 B/cell + 8 B/bin; WA bwd fused
     with pin_pos bwd 20 B/pin + 8 B/physical node, plus 16 B/node for adding
-    the density gradient into the node gradient."""
+    the density gradient into the node gradient (physical nodes only when the
+    force writes the fillers' finished gradients, fillers_final)."""
     Ns = d.num_movable_nodes + d.num_filler_nodes
     B = d.num_bins_x * d.num_bins_y
     P, E, n, nphys = d.num_pins, d.num_nets, d.num_nodes, d.num_physical_nodes
@@ -89,7 +90,7 @@ def op_bytes(d, schedule):
         wa += 28 * P  # pin positions are formed inside the op
     ops = {"wa_forward": wa, "density_map": 28 * Ns + 20 * B, "electric_force": 36 * Ns + 8 * B}
     if schedule in ("pins", "fused"):
-        ops["node_grad"] = 20 * P + 8 * nphys + 16 * n
+        ops["node_grad"] = 20 * P + 8 * nphys + 16 * (nphys if fillers_final else n)
     return ops
 
 
@@ -752,7 +753,7 @@ def main():
         log("[bench] fp32/%s one stream: %.3f ms/step" % (args.precision, ms_serial))
 
     # per-op live timing
-    hbm_bytes = op_bytes(d, args.schedule)
+    hbm_bytes = op_bytes(d, args.schedule, obj32.fillers_final)
     names = [k for k in PROBES if k in hbm_bytes or k == "poisson"]
     op_ms = {k: probe(torch, N, obj32, pos32, PROBES[k], 3) for k in names}
     dom = max(hbm_bytes, key=lambda k: op_ms[k])

@@ -20,6 +20,7 @@ DP_CONVERT_CPU, DP_CONVERT_GPU = 0, 1
 DP_SPECTRAL_FP64, DP_SPECTRAL_FP32, DP_SPECTRAL_BF16X3 = 0, 1, 2
 DP_SCATTER_GLOBAL, DP_SCATTER_TILED = 0, 1
 DP_LAYOUT_SPLIT, DP_LAYOUT_INTERLEAVED = 0, 1
+DP_LAYOUT_FILLERS_FINAL = 0x10
 DP_ARITH_BIN_RELATIVE = 0x100
 DP_NET_WEIGHTING_ADAMS, DP_NET_WEIGHTING_LILITH = 0, 1
 
@@ -47,6 +48,9 @@ _SIGNATURES = {
                                   _I, _D, _I, _P, _P, _P]),
     "dp_electric_force": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
                                _I, _I, _I, _L, _I, _P, _P, _P, _P]),
+    "dp_electric_force_fillers_final": (_I, [_I, _P, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D,
+                                             _D, _D, _L, _L, _I, _I, _I, _L, _I, _P, _P, _P, _P,
+                                             _P, _D, _D, _P]),
     "dp_energy_gradient": (_I, [_I, _P, _P, _P, _L, _P, _P, _P, _P, _P, _D, _D, _D, _D, _L, _L,
                                 _I, _I, _I, _P, _P, _P]),
     "dp_reduce_workspace_bytes": (_S, [_L]),

@@ -718,11 +718,12 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
                                                      const int* nwork, Recs<T> rec,
                                                      const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
-                                                     int64_t num_nodes, int interleaved, T* out) {
+                                                     GradOut<T> go) {
   typedef typename Vec2<T>::type V;
   __shared__ V sf[kTileH * kTileH];
   const int W = *nwork * kGatherSplit;
   const T g = grad ? *grad : T(1);
+  const T c = grad_out_coef(go);
   // round-robin over the parts: a die-contiguous assignment (parts of a tile
   // sharing one L2) measured slower, as the tiles' loads are uneven
   for (int w = blockIdx.x; w < W; w += gridDim.x) {
@@ -780,7 +781,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_gather(TriArgs<T> a, int 
           }
         }
         const T r = b.r[u];
-        put_grad(out, num_nodes, b.idx[u], -((gx * r) * g), -((gy * r) * g), interleaved);
+        put_cell_grad(go, c, b.idx[u], -((gx * r) * g), -((gy * r) * g));
       }
     }
     __syncthreads();
@@ -829,9 +830,10 @@ template <typename T>
 __device__ __forceinline__ void drift_gather(const TriArgs<T>& a, const int* drift,
                                              const int* ndrift, const T* __restrict__ fx,
                                              const T* __restrict__ fy, const T* grad,
-                                             int64_t num_nodes, int interleaved, T* out) {
+                                             const GradOut<T>& go) {
   const int cnt = *ndrift;
   const T g = grad ? *grad : T(1);
+  const T c = grad_out_coef(go);
   for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < cnt; q0 += gridDim.x * blockDim.x) {
     const int64_t i = drift[q0];
     const T x0 = a.x[i], y0 = a.y[i];
@@ -857,7 +859,7 @@ __device__ __forceinline__ void drift_gather(const TriArgs<T>& a, const int* dri
         }
       }
     }
-    put_grad(out, num_nodes, i, -((gx * r) * g), -((gy * r) * g), interleaved);
+    put_cell_grad(go, c, i, -((gx * r) * g), -((gy * r) * g));
   }
 }
 
@@ -894,12 +896,13 @@ __device__ __forceinline__ void rec_gather_large(const TriArgs<T>& a, const Recs
                                                  int64_t total, const int* ntail,
                                                  const T* __restrict__ fx,
                                                  const T* __restrict__ fy, const T* grad,
-                                                 int64_t num_nodes, int interleaved, T* out) {
+                                                 const GradOut<T>& go) {
   const int cnt = *ntail;
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const T g = grad ? *grad : T(1);
+  const T c = grad_out_coef(go);
   for (int q = wave; q < cnt; q += nwaves) {
     const int64_t j = total - 1 - q;
     const T x = rec.x[j], y = rec.y[j], sx = rec.sx[j], sy = rec.sy[j], r = rec.r[j];
@@ -922,9 +925,9 @@ __device__ __forceinline__ void rec_gather_large(const TriArgs<T>& a, const Recs
     gy = wave_sum(gy);
     if (lane == 0) {
       if (empty)
-        put_grad(out, num_nodes, rec.idx[j], T(0), T(0), interleaved);
+        put_cell_grad(go, c, rec.idx[j], T(0), T(0));
       else
-        put_grad(out, num_nodes, rec.idx[j], -((gx * r) * g), -((gy * r) * g), interleaved);
+        put_cell_grad(go, c, rec.idx[j], -((gx * r) * g), -((gy * r) * g));
     }
   }
 }
@@ -953,12 +956,12 @@ __global__ void __launch_bounds__(256) k_gather_rest(TriArgs<T> a, int64_t t0, i
                                                      Recs<T> rec, int64_t total,
                                                      const int* ntail, const T* __restrict__ fx,
                                                      const T* __restrict__ fy, const T* grad,
-                                                     int64_t num_nodes, int interleaved, T* out) {
+                                                     GradOut<T> go) {
   for (int64_t i = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t1;
        i += (int64_t)gridDim.x * blockDim.x)
-    put_grad(out, num_nodes, i, T(0), T(0), interleaved);
-  drift_gather(a, drift, ndrift, fx, fy, grad, num_nodes, interleaved, out);
-  rec_gather_large(a, rec, total, ntail, fx, fy, grad, num_nodes, interleaved, out);
+    put_grad(go.out, go.n, i, T(0), T(0), go.interleaved);
+  drift_gather(a, drift, ndrift, fx, fy, grad, go);
+  rec_gather_large(a, rec, total, ntail, fx, fy, grad, go);
 }
 
 template <typename T>
@@ -1033,9 +1036,9 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
 }
 
 template <typename T>
-int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
-                 const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
-                 hipStream_t s) {
+int gather_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t n_fill, const T* fx, const T* fy,
+                 const T* grad, char* tiled_ws, const GradOut<T>& go, hipStream_t s) {
+  const int64_t num_nodes = go.n;
   const int64_t total = n_mov + n_fill;
   TiledLayout L = tiled_layout(a.nbx, a.nby, total, sizeof(T), a.acc != 0);
   DP_REQUIRE(L.ntiles <= kMaxTiles, DP_ERR_UNSUPPORTED,
@@ -1050,7 +1053,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
     auto gath = a.acc ? k_tile_gather<T, true> : k_tile_gather<T, false>;
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
               hipLaunchKernelGGL(gath, dim3(2048), dim3(kTileThreads), 0, s, a, L.nty, work,
-                                 nwork, rec, fx, fy, grad, num_nodes, interleaved, out));
+                                 nwork, rec, fx, fy, grad, go));
     DP_LAUNCHED("k_tile_gather");
   }
   // the terminals, the fillers taken by range whose span left their tile's
@@ -1058,7 +1061,7 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
   // ranges) and the tail records
   hipLaunchKernelGGL(k_gather_rest<T>, dim3(256), dim3(256), 0, s, a, t0, t1,
                      (const int*)(tiled_ws + L.off_drift), nwork + 2, rec, total, ntail, fx, fy,
-                     grad, num_nodes, interleaved, out);
+                     grad, go);
   DP_LAUNCHED("k_gather_rest");
   return DP_OK;
 }
@@ -1067,8 +1070,8 @@ int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t 
   template int scatter_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, char*,        \
                                 unsigned long long*, unsigned long long*, const int*, bool,  \
                                 hipStream_t);                                                \
-  template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, int64_t, const T*,      \
-                               const T*, const T*, char*, int, T*, hipStream_t);
+  template int gather_tiled<T>(const TriArgs<T>&, int64_t, int64_t, const T*, const T*,    \
+                               const T*, char*, const GradOut<T>&, hipStream_t);
 DP_TILED_INST(float)
 DP_TILED_INST(double)
 

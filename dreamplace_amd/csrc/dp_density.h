@@ -166,6 +166,49 @@ __device__ __forceinline__ void put_grad(T* out, int64_t num_nodes, int64_t i, T
   }
 }
 
+// Where the tiled gather puts a cell's gradient: put_grad into `out`, except
+// that with `fin` the fillers (i >= f0) get their finished objective
+// gradient in `fin` (split [2n]) instead: 0 + g, as the node-gradient pass
+// adds a pinless node's density gradient to its zero wirelength sum, then,
+// with `dw`, divided by max(alpha * (*dw) * farea, 1), that pass's
+// preconditioner for a node of area farea and no pins.  Same operations in
+// the same order, so the same bits as the two passes in sequence.
+template <typename T>
+struct GradOut {
+  T* out;
+  int64_t n;
+  int interleaved;
+  T* fin;
+  int64_t f0;
+  const T* dw;
+  T alpha, farea;
+};
+
+// The preconditioner's density factor alpha * (*dw), read once per thread.
+template <typename T>
+__device__ __forceinline__ T grad_out_coef(const GradOut<T>& o) {
+  return o.fin && o.dw ? o.alpha * *o.dw : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void put_cell_grad(const GradOut<T>& o, T c, int64_t i, T gx, T gy) {
+  if (o.fin && i >= o.f0) {
+    T vx = T(0), vy = T(0);
+    vx += gx;
+    vy += gy;
+    if (o.dw) {
+      T p = c * o.farea;
+      p = p < T(1) ? T(1) : p;
+      vx = vx / p;
+      vy = vy / p;
+    }
+    o.fin[i] = vx;
+    o.fin[o.n + i] = vy;
+  } else {
+    put_grad(o.out, o.n, i, gx, gy, o.interleaved);
+  }
+}
+
 template <typename T>
 TriArgs<T> make_tri_args(const void* pos, int64_t num_nodes, const void* wxc, const void* wyc,
                          const void* offx, const void* offy, const void* ratio, double xl,
@@ -188,9 +231,8 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
                   char* tiled_ws, unsigned long long* map, unsigned long long* filler_map,
                   const int* filler_ranges, bool zero_map, hipStream_t s);
 template <typename T>
-int gather_tiled(const TriArgs<T>& a, int64_t num_nodes, int64_t n_mov, int64_t n_fill,
-                 const T* fx, const T* fy, const T* grad, char* tiled_ws, int interleaved, T* out,
-                 hipStream_t s);
+int gather_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t n_fill, const T* fx, const T* fy,
+                 const T* grad, char* tiled_ws, const GradOut<T>& go, hipStream_t s);
 
 // Density workspace layout: [int64 map][int64 filler map][queue count (256 B)]
 // [queue][tiled region].

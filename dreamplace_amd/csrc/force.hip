@@ -195,16 +195,18 @@ extern "C" int dp_energy_gradient(int dtype, const void* grad_scale, const void*
   });
 }
 
-extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
-                                 const void* field_map_y, const void* pos, int64_t num_nodes,
-                                 const void* node_size_x_clamped, const void* node_size_y_clamped,
-                                 const void* offset_x, const void* offset_y, const void* ratio,
-                                 double xl, double yl, double bin_size_x, double bin_size_y,
-                                 int64_t num_movable_nodes, int64_t num_filler_nodes,
-                                 int num_bins_x, int num_bins_y, int algorithm,
-                                 int64_t large_capacity, int grad_layout,
-                                 const double* uniform_filler, void* workspace, void* grad_out,
-                                 void* stream) {
+// The field gather of dp_electric_force; with filler_grad_out (tiled only)
+// the fillers' finished gradients go there (GradOut).
+static int electric_force(int dtype, const void* grad_scale, const void* field_map_x,
+                          const void* field_map_y, const void* pos, int64_t num_nodes,
+                          const void* node_size_x_clamped, const void* node_size_y_clamped,
+                          const void* offset_x, const void* offset_y, const void* ratio, double xl,
+                          double yl, double bin_size_x, double bin_size_y,
+                          int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x,
+                          int num_bins_y, int algorithm, int64_t large_capacity, int grad_layout,
+                          const double* uniform_filler, void* workspace, void* grad_out,
+                          void* filler_grad_out, const void* density_weight, double alpha,
+                          double filler_area, void* stream) {
   DP_ARG(field_map_x && field_map_y && pos && node_size_x_clamped && node_size_y_clamped &&
              offset_x && offset_y && ratio && grad_out && workspace,
          "dp_electric_force: null pointer");
@@ -228,7 +230,7 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
   char* p = (char*)workspace + density_queue_offset(nb);
   int* qcount = (int*)p;
   int* qlist = (int*)(p + 256);
-  char* tiled = (char*)workspace + density_tiled_offset(nb, large_capacity);
+  char* tiled_ws = (char*)workspace + density_tiled_offset(nb, large_capacity);
   if (num_nodes == 0) return DP_OK;
   return DP_DISPATCH(dtype, "dp_electric_force", [&] {
     TriArgs<T> a = make_tri_args<T>(pos, num_nodes, node_size_x_clamped, node_size_y_clamped,
@@ -236,10 +238,16 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
                                     num_bins_x, num_bins_y, 1, uniform_filler,
                                     num_nodes - num_filler_nodes);
     a.acc = acc;
-    if (algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y))
-      return gather_tiled<T>(a, num_nodes, num_movable_nodes, num_filler_nodes,
-                             (const T*)field_map_x, (const T*)field_map_y, (const T*)grad_scale,
-                             tiled, il, (T*)grad_out, s);
+    const bool tiled = algorithm == DP_SCATTER_TILED && tiled_supported(num_bins_x, num_bins_y);
+    DP_REQUIRE(!filler_grad_out || tiled, DP_ERR_UNSUPPORTED,
+               "dp_electric_force_fillers_final: needs the tiled gather");
+    if (tiled) {
+      GradOut<T> go = {(T*)grad_out, num_nodes, il, (T*)filler_grad_out,
+                       num_nodes - num_filler_nodes, (const T*)density_weight, (T)alpha,
+                       (T)filler_area};
+      return gather_tiled<T>(a, num_movable_nodes, num_filler_nodes, (const T*)field_map_x,
+                             (const T*)field_map_y, (const T*)grad_scale, tiled_ws, go, s);
+    }
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, qcount, (int64_t)1, 0);
     DP_PROBED(DP_KERNEL_ELECTRIC_FORCE, s,
               hipLaunchKernelGGL(k_force_small<T>, dim3((unsigned)ceil_div(num_nodes, 256)),
@@ -254,4 +262,41 @@ extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* 
     DP_LAUNCHED("k_force_large");
     return DP_OK;
   });
+}
+
+extern "C" int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x,
+                                 const void* field_map_y, const void* pos, int64_t num_nodes,
+                                 const void* node_size_x_clamped, const void* node_size_y_clamped,
+                                 const void* offset_x, const void* offset_y, const void* ratio,
+                                 double xl, double yl, double bin_size_x, double bin_size_y,
+                                 int64_t num_movable_nodes, int64_t num_filler_nodes,
+                                 int num_bins_x, int num_bins_y, int algorithm,
+                                 int64_t large_capacity, int grad_layout,
+                                 const double* uniform_filler, void* workspace, void* grad_out,
+                                 void* stream) {
+  return electric_force(dtype, grad_scale, field_map_x, field_map_y, pos, num_nodes,
+                        node_size_x_clamped, node_size_y_clamped, offset_x, offset_y, ratio, xl,
+                        yl, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
+                        num_bins_x, num_bins_y, algorithm, large_capacity, grad_layout,
+                        uniform_filler, workspace, grad_out, nullptr, nullptr, 0.0, -1.0, stream);
+}
+
+extern "C" int dp_electric_force_fillers_final(
+    int dtype, const void* grad_scale, const void* field_map_x, const void* field_map_y,
+    const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
+    const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
+    const void* ratio, double xl, double yl, double bin_size_x, double bin_size_y,
+    int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x, int num_bins_y,
+    int algorithm, int64_t large_capacity, int grad_layout, const double* uniform_filler,
+    void* workspace, void* grad_out, void* grad_pos_out, const void* density_weight, double alpha,
+    double filler_area, void* stream) {
+  DP_ARG(grad_pos_out, "dp_electric_force_fillers_final: null grad_pos_out");
+  DP_ARG(!density_weight || filler_area >= 0,
+         "dp_electric_force_fillers_final: the preconditioner needs the fillers' area");
+  return electric_force(dtype, grad_scale, field_map_x, field_map_y, pos, num_nodes,
+                        node_size_x_clamped, node_size_y_clamped, offset_x, offset_y, ratio, xl,
+                        yl, bin_size_x, bin_size_y, num_movable_nodes, num_filler_nodes,
+                        num_bins_x, num_bins_y, algorithm, large_capacity, grad_layout,
+                        uniform_filler, workspace, grad_out, grad_pos_out, density_weight, alpha,
+                        filler_area, stream);
 }

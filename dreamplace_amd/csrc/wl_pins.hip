@@ -603,17 +603,29 @@ __global__ void __launch_bounds__(256) k_wa_big(
 // Thread t takes nodes t, t + grid, ...; with `partials` it also sums
 // (grad - g_prev)^2 over x and y in FP64, in k_step_norms's order (the
 // step-size denominator of step_nobb, NesterovAcceleratedGradientOptimizer.py:131).
+// fillers_done: the fillers' outputs are already final (the field gather
+// wrote them); they are only read back for the sum, or not visited at all.
 template <typename T, bool kPre>
 __global__ void __launch_bounds__(256) k_node_grad(
     const typename Vec2<T>::type* __restrict__ gnode, const int* __restrict__ node2pin_start,
     const T* __restrict__ grad, const T* __restrict__ add, int add_interleaved, int64_t nphys,
     int64_t nnodes, T* gx_out, T* gy_out, const T* __restrict__ pws, const T* __restrict__ area,
-    const T* __restrict__ dw, T alpha, T farea, const T* __restrict__ g_prev, double* partials) {
+    const T* __restrict__ dw, T alpha, T farea, const T* __restrict__ g_prev, double* partials,
+    int fillers_done) {
   __shared__ double smem[4];
   double sgg = 0;
   const T g = *grad;
-  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nnodes;
+  const int64_t nend = fillers_done && !partials ? nphys : nnodes;
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nend;
        n += (int64_t)gridDim.x * blockDim.x) {
+    if (fillers_done && n >= nphys) {
+      // only with partials: the same terms in the same order
+      const double dx = (double)(gx_out[n] - g_prev[n]);
+      sgg += dx * dx;
+      const double dy = (double)(gy_out[n] - g_prev[nnodes + n]);
+      sgg += dy * dy;
+      continue;
+    }
     T sx = 0, sy = 0;
     if (n < nphys) {
       for (int j = node2pin_start[n]; j < node2pin_start[n + 1]; ++j) {
@@ -907,6 +919,8 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
          name);
   DP_ARG(!dg_norm2_out || grad_prev != grad_pos_out, "%s: grad_prev must not be the output", name);
   DP_ARG(num_physical_nodes <= num_nodes, "%s: bad node counts", name);
+  const int fillers_done = (add_layout & DP_LAYOUT_FILLERS_FINAL) != 0;
+  add_layout &= ~DP_LAYOUT_FILLERS_FINAL;
   DP_ARG(add_layout == DP_LAYOUT_SPLIT || add_layout == DP_LAYOUT_INTERLEAVED,
          "%s: bad add_layout %d", name, add_layout);
   if (num_nodes == 0) return DP_OK;
@@ -916,15 +930,16 @@ static int node_grad(const char* name, int dtype, const void* grad, const void* 
     typedef typename Vec2<T>::type V;
     T* out = (T*)grad_pos_out;
     auto kern = pws ? k_node_grad<T, true> : k_node_grad<T, false>;
-    const int nb = node_reduce_blocks(num_nodes);
     double* part = dg_norm2_out ? (double*)workspace : nullptr;
+    // the sum's order is fixed by the grid over all nodes
+    const int nb = node_reduce_blocks(fillers_done && !part ? num_physical_nodes : num_nodes);
     DP_PROBED(DP_KERNEL_PIN_POS_BACKWARD, s,
               hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, (const V*)grad_node_order,
                                  (const int*)flat_node2pin_start_map, (const T*)grad,
                                  (const T*)add, add_layout == DP_LAYOUT_INTERLEAVED,
                                  num_physical_nodes, num_nodes, out, out + num_nodes,
                                  (const T*)pws, (const T*)area, (const T*)dw, (T)alpha,
-                                 (T)filler_area, (const T*)grad_prev, part));
+                                 (T)filler_area, (const T*)grad_prev, part, fillers_done));
     DP_LAUNCHED("k_node_grad");
     if (part) return finish_reduction(part, nb, 1, nullptr, dg_norm2_out, -1, s);
     return DP_OK;

@@ -517,6 +517,15 @@ class PlacementObjective:
             self.metric_out = torch.empty(3, dtype=torch.float64, device=dev)
             self.metric_ws = torch.empty(N.query("dp_wa_workspace_bytes", E), dtype=torch.uint8,
                                          device=dev)
+        # the fillers' gradient combine (and preconditioner) done by the field
+        # gather itself (dp_electric_force_fillers_final): the same bits, one
+        # pass over the fillers fewer; where it applies (pins schedule, tiled
+        # gather, fillers of one size, the reference's force backward)
+        ep = self.density
+        self.fillers_final = (dd.num_filler_nodes > 0 and density_backward != "adjoint"
+                              and can_track_metrics(self.schedule, ep.algorithm, M, Nb,
+                                                    ep.padding, ep.use_uniform_filler))
+        self._ff_last = None  # (gradient buffer, preconditioned) of the last evaluation
 
     def filler_origin_offset(self):
         """The stretch offset of the fillers' origin in the density kernels
@@ -581,10 +590,27 @@ class PlacementObjective:
 
     @property
     def density_grad(self):
-        """Density gradient of the last evaluation in the reference [2N] layout."""
+        """Density gradient of the last evaluation in the reference [2N] layout.
+        Where the field gather finished the fillers' gradients itself, theirs
+        are read back from that evaluation's gradient (0 + g: the same values,
+        a negative zero read as zero); not available after a preconditioned
+        one, whose filler gradients are divided."""
         if self.grad_layout == N.DP_LAYOUT_INTERLEAVED:
-            return self._density_grad.view(-1, 2).t().reshape(-1)
-        return self._density_grad
+            g = self._density_grad.view(-1, 2).t().reshape(-1)
+        else:
+            g = self._density_grad
+        if self._ff_last is not None:
+            buf, pre = self._ff_last
+            if pre:
+                raise RuntimeError("density_grad: the last evaluation was preconditioned and "
+                                   "kept only the fillers' finished gradients")
+            n = self.dd.num_nodes
+            f0 = n - self.dd.num_filler_nodes
+            if g.data_ptr() == self._density_grad.data_ptr():
+                g = g.clone()
+            g[f0:n] = buf[f0:n]
+            g[n + f0:] = buf[n + f0:]
+        return g
 
     def set_gamma(self, gamma):
         self.inv_gamma.fill_(1.0 / gamma)
@@ -663,7 +689,9 @@ class PlacementObjective:
             self.lam_eff.copy_(g * (1 + c * e) + (g * e) * c)
         return self.lam_eff
 
-    def _density_backward(self, pos):
+    def _density_backward(self, pos, ff=None):
+        """ff: (gradient buffer, density weight or None, alpha, filler area)
+        for dp_electric_force_fillers_final, else None."""
         ep = self.density
         w = self._density_grad_weight(pos)
         if self.density_backward == "adjoint":
@@ -679,14 +707,20 @@ class PlacementObjective:
         fx, fy = self.field_x, self.field_y
         if fx.dtype != pos.dtype:
             fx, fy = fx.to(pos.dtype), fy.to(pos.dtype)
-        N.call("dp_electric_force", N.dtype_code(pos), N.ptr(w), N.ptr(fx), N.ptr(fy),
-               N.ptr(pos), pos.numel() // 2, N.ptr(ep.node_size_x_clamped),
-               N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x), N.ptr(ep.offset_y),
-               N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
-               float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
-               ep.num_bins_x, ep.num_bins_y, ep.algorithm_code, ep.large_capacity,
-               self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
-               N.ptr(self._density_grad), N.stream_ptr(pos.device))
+        args = (N.dtype_code(pos), N.ptr(w), N.ptr(fx), N.ptr(fy),
+                N.ptr(pos), pos.numel() // 2, N.ptr(ep.node_size_x_clamped),
+                N.ptr(ep.node_size_y_clamped), N.ptr(ep.offset_x), N.ptr(ep.offset_y),
+                N.ptr(ep.ratio), float(ep.xl), float(ep.yl), float(ep.bin_size_x),
+                float(ep.bin_size_y), int(ep.num_movable_nodes), int(ep.num_filler_nodes),
+                ep.num_bins_x, ep.num_bins_y, ep.algorithm_code, ep.large_capacity,
+                self.grad_layout, ep.uniform_filler_arg(), N.ptr(ep.workspace(pos.device)),
+                N.ptr(self._density_grad))
+        if ff is None:
+            N.call("dp_electric_force", *args, N.stream_ptr(pos.device))
+        else:
+            buf, dw, alpha, farea = ff
+            N.call("dp_electric_force_fillers_final", *args, N.ptr(buf), N.ptr(dw), float(alpha),
+                   float(farea), N.stream_ptr(pos.device))
 
     def evaluate(self, pos, precondition=None, grad_out=None, grad_ref=None):
         """(obj 0-dim tensor, grad [2N] tensor); both are reused buffers.
@@ -716,8 +750,18 @@ class PlacementObjective:
         code = N.dtype_code(pos)
         s = N.stream_ptr(pos.device)
         n, P, E = dd.num_nodes, dd.num_pins, dd.num_nets
+        self._ff_last = None
         if self.schedule == "pins":
             pp = self.pin_plan
+            # the fillers' gradients finished by the field gather, where that
+            # applies: unpreconditioned, or preconditioned with the fillers' area
+            ff = None
+            if self.fillers_final:
+                if precondition is None:
+                    ff = (grad, None, 0.0, -1.0)
+                elif len(precondition) > 3 and precondition[3] >= 0:
+                    ff = (grad, self.lam, float(precondition[2]), float(precondition[3]))
+            layout = self.grad_layout | (N.DP_LAYOUT_FILLERS_FINAL if ff is not None else 0)
 
             def wirelength_forward():
                 s_wl = s
@@ -750,7 +794,9 @@ class PlacementObjective:
             self._density_solve()
             if fork_at == "gather":
                 wirelength_forward()
-            self._density_backward(pos)
+            self._density_backward(pos, ff)
+            if ff is not None:
+                self._ff_last = (grad, ff[1] is not None)
             if self.side is not None:
                 # join: the node gradient and the objective read the wirelength outputs
                 torch.cuda.current_stream(pos.device).wait_event(self._ev_join)
@@ -767,7 +813,7 @@ class PlacementObjective:
                                                 dtype=torch.uint8, device=pos.device)
                 N.call("dp_node_grad_preconditioned", code, N.ptr(self.one),
                        N.ptr(pp.grad_node_order), N.ptr(dd.flat_node2pin_start_map),
-                       N.ptr(self._density_grad), self.grad_layout, dd.num_physical_nodes, n,
+                       N.ptr(self._density_grad), layout, dd.num_physical_nodes, n,
                        N.ptr(pws), N.ptr(areas), N.ptr(self.lam), float(alpha), float(farea),
                        N.ptr(grad), N.ptr(grad_ref) if fuse else None, N.ptr(self._norm_ws) if fuse else None,
                        N.ptr(self._dg2) if fuse else None, s)
@@ -777,7 +823,7 @@ class PlacementObjective:
             else:
                 N.call("dp_node_grad_from_pins", code, N.ptr(self.one), N.ptr(pp.grad_node_order),
                        N.ptr(dd.flat_node2pin_start_map), N.ptr(self._density_grad),
-                       self.grad_layout, dd.num_physical_nodes, n, N.ptr(grad), s)
+                       layout, dd.num_physical_nodes, n, N.ptr(grad), s)
         elif self.schedule == "fused":
             N.call("dp_wa_forward_from_nodes", code, N.ptr(pos), n, N.ptr(dd.pin_offset_x),
                    N.ptr(dd.pin_offset_y), N.ptr(dd.pin2node_map), P, N.ptr(dd.flat_net2pin_map),

@@ -67,6 +67,13 @@ enum { DP_ARITH_BIN_RELATIVE = 0x100 };
  * (all x, then all y); DP_LAYOUT_INTERLEAVED stores (x, y) of node i at
  * [2i, 2i+1], so one node is one 8- (FP32) or 16-byte (FP64) word. */
 enum { DP_LAYOUT_SPLIT = 0, DP_LAYOUT_INTERLEAVED = 1 };
+/* Flag OR-ed into the add_layout of dp_node_grad_from_pins and
+ * dp_node_grad_preconditioned: the fillers' entries of grad_pos_out already
+ * hold their finished gradient, as dp_electric_force_fillers_final wrote them
+ * (with the same density_weight, alpha and filler_area when preconditioned);
+ * only the other nodes are computed (and the fillers' entries read back for
+ * dg_norm2_out), and add's filler entries are not read. */
+enum { DP_LAYOUT_FILLERS_FINAL = 0x10 };
 
 const char* dp_last_error(void);
 int dp_version(void);
@@ -218,6 +225,30 @@ int dp_electric_force(int dtype, const void* grad_scale, const void* field_map_x
                       int64_t num_filler_nodes, int num_bins_x, int num_bins_y, int algorithm,
                       int64_t large_capacity, int grad_layout, const double* uniform_filler,
                       void* workspace, void* grad_out, void* stream);
+
+/* No counterpart in the reference: dp_electric_force with the gradient
+ * combine of the fillers folded in, for a placement loop that combines the
+ * two gradients on the device (PlaceObj.obj_and_grad, PlaceObj.py:394-411,
+ * then PreconditionOp, PlaceObj.py:69-102).  Fillers have no pins, so their
+ * objective gradient is their density gradient alone: for every filler i
+ * (i >= num_nodes - num_filler_nodes) grad_pos_out (split [2N], `dtype`)
+ * receives 0 + g_i, with g_i what dp_electric_force would write to grad_out,
+ * divided, when density_weight (device scalar) is given, by
+ * max(alpha * (*density_weight) * filler_area, 1) (filler_area >= 0, the
+ * fillers' one area).  These are the bits dp_node_grad_from_pins /
+ * dp_node_grad_preconditioned give a filler; pass DP_LAYOUT_FILLERS_FINAL to
+ * them so that they skip the fillers.  Every other node's density gradient
+ * goes to grad_out as dp_electric_force writes it; grad_out's filler entries
+ * are left unwritten.  Tiled gather only (DP_ERR_UNSUPPORTED otherwise). */
+int dp_electric_force_fillers_final(
+    int dtype, const void* grad_scale, const void* field_map_x, const void* field_map_y,
+    const void* pos, int64_t num_nodes, const void* node_size_x_clamped,
+    const void* node_size_y_clamped, const void* offset_x, const void* offset_y,
+    const void* ratio, double xl, double yl, double bin_size_x, double bin_size_y,
+    int64_t num_movable_nodes, int64_t num_filler_nodes, int num_bins_x, int num_bins_y,
+    int algorithm, int64_t large_capacity, int grad_layout, const double* uniform_filler,
+    void* workspace, void* grad_out, void* grad_pos_out, const void* density_weight, double alpha,
+    double filler_area, void* stream);
 
 /* The exact gradient of the density energy E = sum(phi * rho) that
  * ElectricPotentialFunction.forward returns (electric_potential.py:137-186;
@@ -515,8 +546,9 @@ int dp_wa_pins_slot_weights(int dtype, int64_t num_pins, int32_t* pin_meta,
 
 /* Backward of that chain plus the gradient combine: grad_pos[n] =
  * (*grad) * (sum of grad_node_order over node n's CSR range) + add[n]
- * (add nullable, in add_layout: DP_LAYOUT_SPLIT or DP_LAYOUT_INTERLEAVED),
- * read contiguously; fillers get add only.  grad_pos_out is split. */
+ * (add nullable, in add_layout: DP_LAYOUT_SPLIT or DP_LAYOUT_INTERLEAVED,
+ * optionally with DP_LAYOUT_FILLERS_FINAL), read contiguously; fillers get
+ * add only.  grad_pos_out is split. */
 int dp_node_grad_from_pins(int dtype, const void* grad, const void* grad_node_order,
                            const int32_t* flat_node2pin_start_map, const void* add,
                            int add_layout, int64_t num_physical_nodes, int64_t num_nodes,

@@ -380,3 +380,59 @@ def test_engine_movable_relabelling(gpu, dtype):
     assert torch.equal(dd.current_order(res[1][2]), dd.pos)
     with pytest.raises(ValueError):
         dd.reorder_movables(torch.zeros(d.num_movable_nodes, dtype=torch.int64))
+
+
+@pytest.mark.parametrize("ranges", [False, True], ids=["binned", "ranges"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_fillers_final_is_bit_identical(gpu, dtype, ranges):
+    """The field gather finishing the fillers' gradients itself
+    (dp_electric_force_fillers_final with DP_LAYOUT_FILLERS_FINAL) gives the
+    bits of the node-gradient pass doing it, zero signs included:
+    unpreconditioned, preconditioned with the fused step norm, with the
+    fillers binned or taken by tile range after some have left their tile's
+    window (the drift list)."""
+    from dreamplace_amd import _native as N
+    from dreamplace_amd.engine import DeviceDesign, PlacementObjective
+    print("case fillers final %s %s" % (np.dtype(dtype).name, ranges), flush=True)
+    prec = "bf16x3" if dtype == np.float32 else "fp64"
+    d = design(1, dtype=dtype, scale=0.01)
+    dd = DeviceDesign(d, gpu)
+    ob = PlacementObjective(dd, d.wa_gamma, density_weight=0.3, precision=prec, algorithm="tiled",
+                            schedule="pins", density_arith="bin_relative")
+    assert ob.fillers_final
+    n, nf = dd.num_nodes, dd.num_filler_nodes
+    if ranges:
+        dd.reorder_fillers(origin_offset=ob.filler_origin_offset())
+        assert ob.update_filler_ranges(dd.pos)
+        b = float(d.bin_size_x)
+        gen = torch.Generator(device="cpu").manual_seed(3)
+        step = ((torch.rand(2, nf, generator=gen, dtype=torch.float64) - 0.5) * 6 * b)
+        with torch.no_grad():
+            dd.pos.view(2, n)[:, n - nf:] += step.to(gpu, dd.dtype)
+    code = N.dtype_code(dd.pos)
+    pws = torch.empty(n, dtype=dd.dtype, device=gpu)
+    N.call("dp_pin_weight_sum", code, N.ptr(dd.net_weights), N.ptr(dd.flat_node2pin_map),
+           N.ptr(dd.flat_node2pin_start_map), N.ptr(dd.pin2net_map), dd.num_physical_nodes, n,
+           N.ptr(pws), N.stream_ptr(gpu))
+    areas = (dd.node_size_x * dd.node_size_y).contiguous()
+    farea = float(areas[n - 1].item())
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    gref = torch.randn(2 * n, generator=gen, dtype=torch.float64).to(gpu, dd.dtype)
+    bits = torch.int64 if dtype == np.float64 else torch.int32
+    outs = []
+    for on in (False, True):
+        ob.fillers_final = on
+        f, g = ob.evaluate(dd.pos)
+        res = [f.item(), g.clone().view(bits), ob.density_grad.clone()]
+        _, gp = ob.evaluate(dd.pos, precondition=(pws, areas, 0.7, farea), grad_ref=gref)
+        res += [gp.clone().view(bits), ob.last_dg2.item()]
+        if on:
+            with pytest.raises(RuntimeError):
+                ob.density_grad  # noqa: B018 -- preconditioned: the fillers' were not kept
+        outs.append(res)
+    a, b = outs
+    assert a[0] == b[0]
+    assert torch.equal(a[1], b[1])
+    assert torch.equal(a[2], b[2])
+    assert torch.equal(a[3], b[3])
+    assert a[4] == b[4]
