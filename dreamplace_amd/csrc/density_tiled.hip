@@ -605,6 +605,48 @@ __device__ __forceinline__ void load_range(const TriArgs<T>& a, int j0, int stri
   }
 }
 
+// acc += (long long)f for an FP32 contribution f = area * scale >= 0 (or
+// -0), with fewer instructions than the generic conversion: |f| < 1 (a
+// zero contribution) adds nothing, and the truncation is split at 2^32
+// exactly: hi = floor(f / 2^32); lo = f - hi * 2^32 is integral for
+// f >= 2^32 (a float that large has no fraction, and its bits below 2^32
+// fit 24) and is f itself below, where the conversion truncates it.
+__device__ __forceinline__ void lds_add_fixed_nonneg(unsigned long long* p, float f) {
+  const float m = fabsf(f);
+  if (m >= 1.0f) {
+    const float hi = floorf(m * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, m);
+    atomicAdd(p, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+  }
+}
+
+// A cell's contributions to the LDS tile: bins (lx + q, ly + l) for q < nx,
+// l < ny, each (long long)(pxr[q] * py[l] * scale) where not 0.  kNonNeg:
+// every factor is known non-negative (lds_add_fixed_nonneg, FP32 only).
+template <bool kNonNeg, typename T>
+__device__ __forceinline__ void add_cell_bins(unsigned long long* acc, const T pxr[3],
+                                              const T py[3], int nx, int ny, int lx, int ly,
+                                              T scale) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (q < nx) {
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        if (l < ny) {
+          T area = pxr[q] * py[l];
+          unsigned long long* word = &acc[(lx + q) * kTileH + (ly + l)];
+          if constexpr (kNonNeg) {
+            lds_add_fixed_nonneg(word, area * scale);
+          } else {
+            long long v = (long long)(area * scale);
+            if (v != 0) atomicAdd(word, (unsigned long long)v);
+          }
+        }
+      }
+    }
+  }
+}
+
 template <bool kAcc, typename T>
 __device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const int4* work,
                                              const int* nwork, const Recs<T>& rec,
@@ -643,18 +685,18 @@ __device__ __forceinline__ void tile_scatter(const TriArgs<T>& a, int nty, const
         rec_overlaps<kAcc>(a, x, y, sx, sy, b.ex[u], b.ey[u], bxl, byl, pxr, py);
 #pragma unroll
         for (int q = 0; q < 3; ++q) pxr[q] = pxr[q] * r;
+        if constexpr (sizeof(T) == 4) {
+          // an overlap can round below zero at a span's edge: such a cell's
+          // wavefront takes the generic conversion (a uniform branch)
+          bool neg = false;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          if (q < nx) {
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-              if (l < ny) {
-                T area = pxr[q] * py[l];
-                long long v = (long long)(area * a.scale);
-                if (v != 0) atomicAdd(&acc[(lx + q) * kTileH + (ly + l)], (unsigned long long)v);
-              }
-            }
-          }
+          for (int q = 0; q < 3; ++q) neg |= (q < nx && pxr[q] < T(0)) || (q < ny && py[q] < T(0));
+          if (__ballot(neg) == 0ull)
+            add_cell_bins<true>(acc, pxr, py, nx, ny, lx, ly, a.scale);
+          else
+            add_cell_bins<false>(acc, pxr, py, nx, ny, lx, ly, a.scale);
+        } else {
+          add_cell_bins<false>(acc, pxr, py, nx, ny, lx, ly, a.scale);
         }
       }
     }
