@@ -253,3 +253,32 @@ def test_tiled_at_the_tile_limit(oracle, gpu, dtype, arith):
     ref = oracle.electric_force(dp, d.pos, fx, fy, grad=0.75, **kw)
     tol = 1e-12 if dtype == np.float64 else 1e-6
     assert np.max(np.abs(out.cpu().numpy() - ref)) <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("arith", ["reference", "bin_relative"])
+def test_tiled_negative_contributions(gpu, arith):
+    """Negative contributions (here from negated ratios of one movable cell in
+    seven) take the FP32 tile scatter's generic conversion, the others its
+    two-conversion truncation: the int64 map equals the global scatter's bit
+    for bit, which converts every contribution generically."""
+    from dreamplace_amd.electric_potential import ElectricPotential
+    print("case tiled negative contributions %s" % arith, flush=True)
+    d = design(1, dtype=np.float32, scale=0.002)
+    t = to_dev(d, gpu)
+    raws = []
+    for algorithm, negate in (("tiled", False), ("global", True), ("tiled", True)):
+        op = ElectricPotential(
+            t["node_size_x"], t["node_size_y"], t["bin_center_x"], t["bin_center_y"],
+            d.target_density, d.xl, d.yl, d.xh, d.yh, d.bin_size_x, d.bin_size_y,
+            d.num_movable_nodes, d.num_terminals, d.num_filler_nodes, 0, True,
+            t["sorted_node_map"], movable_macro_mask=t["movable_macro_mask"],
+            algorithm=algorithm, arithmetic=arith)
+        if negate:
+            op.ratio[:d.num_movable_nodes:7] *= -1
+        op.compute_initial_density_map(t["pos"])
+        raw = torch.empty(d.num_bins_x * d.num_bins_y, dtype=torch.int64, device=gpu)
+        op.density_map(t["pos"], fixed_map_out=raw)
+        raws.append(raw)
+    torch.cuda.synchronize()
+    assert not torch.equal(raws[0], raws[1])  # the negated cells count
+    assert torch.equal(raws[1], raws[2])
