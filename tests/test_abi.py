@@ -72,3 +72,21 @@ def test_argument_errors_return_codes(lib):
     assert lib.dp_profile_kernel(-1, None, None) == 0
     with pytest.raises(RuntimeError, match="dp_move_boundary failed"):
         N.call("dp_move_boundary", 7, fake, 4, fake, fake, 0., 0., 1., 1., 2, 0, None)
+
+
+def test_fillers_final_argument_errors(lib):
+    """dp_electric_force_fillers_final checks its extra arguments before any
+    launch: the finished-gradient output, and the fillers' area wherever the
+    preconditioner is asked for."""
+    from dreamplace_amd import _native as N
+    fake = ctypes.c_void_p(0x1000)
+    base = [0, fake, fake, fake, fake, 10, fake, fake, fake, fake, fake, 0., 0., 1., 1., 5, 3,
+            10, 10, N.DP_SCATTER_TILED, 10, N.DP_LAYOUT_INTERLEAVED, None, fake, fake]
+    with pytest.raises(RuntimeError, match="null grad_pos_out"):
+        N.call("dp_electric_force_fillers_final", *base, None, None, 0.0, -1.0, None)
+    with pytest.raises(RuntimeError, match="fillers' area"):
+        N.call("dp_electric_force_fillers_final", *base, fake, fake, 1.0, -1.0, None)
+    # the node-gradient passes accept the flag and reject other bits
+    with pytest.raises(RuntimeError, match="bad add_layout"):
+        N.call("dp_node_grad_from_pins", 0, fake, fake, fake, fake,
+               N.DP_LAYOUT_INTERLEAVED | 0x20, 5, 10, fake, None)
