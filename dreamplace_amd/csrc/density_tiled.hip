@@ -54,7 +54,7 @@ constexpr int kHalo = 2;
 constexpr int kTileH = kTile + 2 + 2 * kHalo;
 constexpr int kMaxTiles = 4096;           // LDS histogram limit (4096^2 bins)
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
-constexpr int kFillThreads = 1024;        // threads of a fill workgroup
+constexpr int kFillThreads = 512;         // threads of a fill workgroup
 constexpr int kChunk = 8192;              // cells per scatter/gather work item
 constexpr int kGatherSplit = 2;           // the gather does each work item in this many parts
 // threads of a scatter/gather workgroup: its (64+2)^2-bin LDS tile (35 KB)
@@ -63,11 +63,14 @@ constexpr int kTileThreads = 512;
 
 // Cells per LDS-staged fill sub-batch: 26 (FP32) or 46 (FP64) bytes of LDS
 // per cell, 8 more (FP32) or 16 (FP64) with the bin-relative arithmetic's
-// residuals; with the 48 KB of tile arrays, 152 / 150 KB (FP32) and
-// 140 / 110 KB (FP64) in all.
+// residuals: 52 KB (FP32; 51 with the residuals) and 46 KB (FP64; 31), plus
+// 12 bytes per tile (3 KB at 256 tiles, 48 KB at the 4096-tile limit).
+// Workgroups of 512 threads with this much LDS leave room on a CU beside the
+// wirelength kernels, which the step runs concurrently (0.733 against 0.750
+// ms per config-1 step with 1024-thread workgroups and twice the sub-batch).
 template <typename T, bool kAcc>
 struct FillSub {
-  static constexpr int value = sizeof(T) == 4 ? (kAcc ? 3072 : 4096) : (kAcc ? 1024 : 2048);
+  static constexpr int value = sizeof(T) == 4 ? (kAcc ? 1536 : 2048) : (kAcc ? 512 : 1024);
 };
 
 // Records: one array per field, `total` entries each.  ex, ey (the
@@ -349,17 +352,21 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
 }
 
 // Exclusive scan of in[0, n) into out[0, n) by a workgroup of kFillThreads
-// (n <= 4 * kFillThreads: four consecutive entries per thread).  wsum holds
-// kFillThreads / 64 ints.  Ends with a barrier.
+// (n <= kMaxTiles: E = kMaxTiles / kFillThreads consecutive entries per
+// thread).  wsum holds kFillThreads / 64 ints.  Ends with a barrier.
 __device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, int n, int* wsum) {
   constexpr int NW = kFillThreads / 64;
-  static_assert(kMaxTiles <= 4 * kFillThreads, "scan covers four entries per thread");
+  constexpr int E = kMaxTiles / kFillThreads;
+  static_assert(E * kFillThreads == kMaxTiles, "scan covers E entries per thread");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int k0 = 4 * threadIdx.x;
-  int v[4];
+  const int k0 = E * threadIdx.x;
+  int v[E];
+  int s = 0;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) v[j] = k0 + j < n ? in[k0 + j] : 0;
-  const int s = v[0] + v[1] + v[2] + v[3];
+  for (int j = 0; j < E; ++j) {
+    v[j] = k0 + j < n ? in[k0 + j] : 0;
+    s += v[j];
+  }
   int incl = s;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -381,7 +388,7 @@ __device__ __forceinline__ void block_exclusive_scan(const int* in, int* out, in
   __syncthreads();
   int base = wsum[wid] + incl - s;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < E; ++j) {
     if (k0 + j < n) out[k0 + j] = base;
     base += v[j];
   }
@@ -399,9 +406,12 @@ __global__ void __launch_bounds__(kFillThreads) k_bin_fill(TriArgs<T> a, int64_t
   constexpr int PER = S / kFillThreads;
   static_assert(PER * kFillThreads == S, "whole sub-batches per thread");
   constexpr int SE = kAcc ? S : 1;
-  __shared__ int s_base[kMaxTiles];  // next record slot of (this workgroup, tile)
-  __shared__ int s_hist[kMaxTiles];  // sub-batch cells per tile
-  __shared__ int s_off[kMaxTiles];   // exclusive scan of s_hist
+  // 3 * ntiles ints of dynamic LDS: per tile, the next record slot of (this
+  // workgroup, tile), the sub-batch's cells, and their exclusive scan
+  extern __shared__ int s_tiles[];
+  int* s_base = s_tiles;
+  int* s_hist = s_tiles + ntiles;
+  int* s_off = s_tiles + 2 * ntiles;
   __shared__ int s_wsum[kFillThreads / 64];
   __shared__ T s_x[S], s_y[S], s_sx[S], s_sy[S], s_r[S];
   __shared__ T s_ex[SE], s_ey[SE];
@@ -1059,7 +1069,8 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
     DP_LAUNCHED("k_bin_scan");
     auto fill = a.acc ? k_bin_fill<T, true> : k_bin_fill<T, false>;
     DP_PROBED(DP_KERNEL_DENSITY_BIN, s,
-              hipLaunchKernelGGL(fill, dim3(kBinGroups), dim3(kFillThreads), 0, s, a, n_mov,
+              hipLaunchKernelGGL(fill, dim3(kBinGroups), dim3(kFillThreads),
+                                 3 * sizeof(int) * (size_t)L.ntiles, s, a, n_mov,
                                  fill_begin, total, cs, L.nty, L.ntiles, (const int*)tile_start,
                                  (const int*)wg_base, rec));
     DP_LAUNCHED("k_bin_fill");
