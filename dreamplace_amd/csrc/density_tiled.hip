@@ -55,6 +55,7 @@ constexpr int kTileH = kTile + 2 + 2 * kHalo;
 constexpr int kMaxTiles = 4096;           // LDS histogram limit (4096^2 bins)
 constexpr int kBinGroups = 512;           // workgroups of the count/fill passes
 constexpr int kFillThreads = 512;         // threads of a fill workgroup
+constexpr int kScanThreads = 256;         // threads of the one scan workgroup
 constexpr int kChunk = 8192;              // cells per scatter/gather work item
 constexpr int kGatherSplit = 2;           // the gather does each work item in this many parts
 // threads of a scatter/gather workgroup: its (64+2)^2-bin LDS tile (35 KB)
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
                                                    int64_t total, ClassSplit cs, int nty, int ntiles,
                                                    int* tile_count, int* wg_base, int* ntail,
                                                    Recs<T> rec) {
-  __shared__ int hist[kMaxTiles];
+  extern __shared__ int hist[];  // ntiles ints of dynamic LDS
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   int64_t b, e;
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
   }
 }
 
-// One workgroup of 1024 threads: a tile's records are its class-0 run, then
+// One workgroup of kScanThreads threads: a tile's records are its class-0 run, then
 // its class-1 run; tile_start[c * ntiles + k] = first record of (class c,
 // tile k) (exclusive scan over tiles of both counts); work items (tile,
 // begin, end, class) of at most kChunk cells.
@@ -296,22 +297,22 @@ __global__ void __launch_bounds__(256) k_bin_count(TriArgs<T> a, int64_t n_mov, 
 // class-2 work items.  Ranges that are not a monotone partition of [0, nf)
 // are replaced by one range of all fillers on tile 0, which the scatter and
 // gather still do exactly (their out-of-window path), only slower.
-__global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int ntiles,
+__global__ void __launch_bounds__(kScanThreads) k_bin_scan(const int* tile_count, int ntiles,
                                                    int* tile_start, int4* work, int* nwork,
                                                    const int* frange, int nf) {
-  __shared__ int s_cells[1024];
-  __shared__ int s_items[1024];
+  __shared__ int s_cells[kScanThreads];
+  __shared__ int s_items[kScanThreads];
   __shared__ int s_bad;
   if (threadIdx.x == 0) s_bad = frange && (frange[0] != 0 || frange[ntiles] != nf) ? 1 : 0;
   __syncthreads();
   if (frange)
-    for (int k = threadIdx.x; k < ntiles; k += 1024)
+    for (int k = threadIdx.x; k < ntiles; k += kScanThreads)
       if (frange[k] > frange[k + 1]) atomicOr(&s_bad, 1);
   __syncthreads();
   const bool bad = s_bad != 0;
   auto f_lo = [&](int k) { return !frange ? 0 : bad ? (k == 0 ? 0 : nf) : frange[k]; };
   auto f_hi = [&](int k) { return !frange ? 0 : bad ? nf : frange[k + 1]; };
-  const int per = (ntiles + 1023) / 1024;
+  const int per = (ntiles + kScanThreads - 1) / kScanThreads;
   const int k0 = threadIdx.x * per;
   int cells = 0, items = 0;
   for (int k = k0; k < k0 + per && k < ntiles; ++k) {
@@ -324,7 +325,7 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
   s_items[threadIdx.x] = items;
   __syncthreads();
   // Hillis-Steele inclusive scans
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < kScanThreads; off <<= 1) {
     int vc = threadIdx.x >= off ? s_cells[threadIdx.x - off] : 0;
     int vi = threadIdx.x >= off ? s_items[threadIdx.x - off] : 0;
     __syncthreads();
@@ -348,7 +349,7 @@ __global__ void __launch_bounds__(1024) k_bin_scan(const int* tile_count, int nt
     for (int j = lo; j < hi; j += kChunk)
       work[ibase++] = make_int4(k, j, j + kChunk < hi ? j + kChunk : hi, 2);
   }
-  if (threadIdx.x == 1023) *nwork = s_items[1023];
+  if (threadIdx.x == kScanThreads - 1) *nwork = s_items[kScanThreads - 1];
 }
 
 // Exclusive scan of in[0, n) into out[0, n) by a workgroup of kFillThreads
@@ -1061,10 +1062,11 @@ int scatter_tiled(const TriArgs<T>& a, int64_t n_mov, int64_t fill_begin, int64_
     // with filler ranges they are not binned at all (class 2 work items)
     const ClassSplit cs = ranges ? class_split(n_mov, n_mov, kBinGroups)
                                  : class_split(a.uni ? n_mov : total, total, kBinGroups);
-    hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256), 0, s, a, n_mov, fill_begin,
+    hipLaunchKernelGGL(k_bin_count<T>, dim3(kBinGroups), dim3(256),
+                       sizeof(int) * (size_t)L.ntiles, s, a, n_mov, fill_begin,
                        total, cs, L.nty, L.ntiles, tile_count, wg_base, ntail, rec);
     DP_LAUNCHED("k_bin_count");
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, (const int*)tile_count, L.ntiles,
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanThreads), 0, s, (const int*)tile_count, L.ntiles,
                        tile_start, work, nwork, ranges ? filler_ranges : nullptr, (int)n_fill);
     DP_LAUNCHED("k_bin_scan");
     auto fill = a.acc ? k_bin_fill<T, true> : k_bin_fill<T, false>;
