@@ -493,9 +493,9 @@ class PlacementObjective:
         self.side = None
         self._side_stream = None
         # where the wirelength pass forks off: before the scatter ("scatter",
-        # measured best: 0.775 ms at config 1 against 0.788 before the solve,
-        # 0.779 before the gather and 0.808 on one stream; 1.680 ms at config 4
-        # against 1.721, 1.736 and 1.702), "solve" or "gather"
+        # measured best: 0.729 ms at config 1 against 0.750 before the solve,
+        # 0.736 before the gather and 0.766-0.770 on one stream; 1.494 ms at
+        # config 4 against 1.524, 1.547 and 1.498-1.507), "solve" or "gather"
         self.fork_at = "scatter"
         self.set_overlap(overlap)
         self.filler_ranges = None
