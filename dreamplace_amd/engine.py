@@ -592,9 +592,11 @@ class PlacementObjective:
     def density_grad(self):
         """Density gradient of the last evaluation in the reference [2N] layout.
         Where the field gather finished the fillers' gradients itself, theirs
-        are read back from that evaluation's gradient (0 + g: the same values,
-        a negative zero read as zero); not available after a preconditioned
-        one, whose filler gradients are divided."""
+        are read back from that evaluation's gradient buffer (0 + g: the same
+        values, a negative zero read as zero), which must not have been
+        written since; not available after a preconditioned evaluation, whose
+        filler gradients are divided (set `fillers_final = False` before an
+        evaluation whose density gradient is wanted in full)."""
         if self.grad_layout == N.DP_LAYOUT_INTERLEAVED:
             g = self._density_grad.view(-1, 2).t().reshape(-1)
         else:
